@@ -1,0 +1,400 @@
+// C ABI of the FRW library (declared in include/frw.h).  Every entry point
+// returns a status code; errors are recorded per thread and never thrown
+// across the boundary.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "frw_internal.h"
+
+namespace frw {
+int4 padded_bounds(const std::vector<int4>& r);
+int read_layout(const char* path, int layer, std::vector<int4>* rects, int* n_layers);
+int validate_rects(const std::vector<int4>& r);
+int generate_layout(const frw_gen_params* p, std::vector<int4>* out);
+int save_layout(const std::vector<int4>& rs, const char* path);
+int write_results(const std::vector<int4>& rs, const int64_t* counts, int64_t n_walks, const char* path);
+
+namespace {
+thread_local std::string g_err;
+}
+
+int set_error(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int check_hip(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return FRW_OK;
+    return set_error(e == hipErrorOutOfMemory ? FRW_E_NOMEM : FRW_E_DEVICE,
+                     std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace frw
+
+using namespace frw;
+
+#define FRW_GUARD_BEGIN try {
+#define FRW_GUARD_END                                                   \
+    }                                                                   \
+    catch (const std::bad_alloc&) { return set_error(FRW_E_NOMEM, "host allocation failed"); } \
+    catch (...) { return set_error(FRW_E_INVALID, "unexpected exception"); }
+
+extern "C" {
+
+const char* frw_last_error_string(void) { return g_err.c_str(); }
+
+const char* frw_version(void) { return "frw-mi355x 0.1 (gfx950)"; }
+
+// ---------------------------------------------------------------- layouts
+
+static int make_layout(std::vector<int4>&& rects, frw_layout** out)
+{
+    int rc = validate_rects(rects);
+    if (rc != FRW_OK) return rc;
+    frw_layout* l = new frw_layout;
+    l->rects = std::move(rects);
+    l->bounds = padded_bounds(l->rects);
+    *out = l;
+    return FRW_OK;
+}
+
+int frw_layout_load(const char* path, int32_t layer, frw_layout** out)
+{
+    if (!path || !out) return set_error(FRW_E_INVALID, "null argument");
+    FRW_GUARD_BEGIN
+    std::vector<int4> r;
+    int rc = read_layout(path, layer, &r, nullptr);
+    if (rc != FRW_OK) return rc;
+    return make_layout(std::move(r), out);
+    FRW_GUARD_END
+}
+
+int frw_layout_layer_count(const char* path, int32_t* n_layers)
+{
+    if (!path || !n_layers) return set_error(FRW_E_INVALID, "null argument");
+    FRW_GUARD_BEGIN
+    int n = 0;
+    int rc = read_layout(path, 0, nullptr, &n);
+    *n_layers = n;
+    return (rc == FRW_E_LAYER) ? FRW_OK : rc;
+    FRW_GUARD_END
+}
+
+int frw_layout_from_rects(const int32_t* xyxy, int64_t n, frw_layout** out)
+{
+    if ((!xyxy && n > 0) || n < 0 || !out) return set_error(FRW_E_INVALID, "bad rect array");
+    if (n > (int64_t)1 << 30) return set_error(FRW_E_RANGE, "too many rects");
+    FRW_GUARD_BEGIN
+    std::vector<int4> r((size_t)n);
+    for (int64_t i = 0; i < n; ++i) r[i] = make_int4(xyxy[4 * i], xyxy[4 * i + 1], xyxy[4 * i + 2], xyxy[4 * i + 3]);
+    return make_layout(std::move(r), out);
+    FRW_GUARD_END
+}
+
+int frw_layout_generate(const frw_gen_params* p, frw_layout** out)
+{
+    if (!p || !out) return set_error(FRW_E_INVALID, "null argument");
+    FRW_GUARD_BEGIN
+    std::vector<int4> r;
+    int rc = generate_layout(p, &r);
+    if (rc != FRW_OK) return rc;
+    return make_layout(std::move(r), out);
+    FRW_GUARD_END
+}
+
+int frw_layout_save(const frw_layout* l, const char* path)
+{
+    if (!l || !path) return set_error(FRW_E_INVALID, "null argument");
+    FRW_GUARD_BEGIN
+    return save_layout(l->rects, path);
+    FRW_GUARD_END
+}
+
+int64_t frw_layout_size(const frw_layout* l) { return l ? (int64_t)l->rects.size() : -1; }
+
+int frw_layout_rects(const frw_layout* l, int32_t* xyxy_out)
+{
+    if (!l || !xyxy_out) return set_error(FRW_E_INVALID, "null argument");
+    std::memcpy(xyxy_out, l->rects.data(), sizeof(int4) * l->rects.size());
+    return FRW_OK;
+}
+
+int frw_layout_bounds(const frw_layout* l, int32_t out[4])
+{
+    if (!l || !out) return set_error(FRW_E_INVALID, "null argument");
+    out[0] = l->bounds.x; out[1] = l->bounds.y; out[2] = l->bounds.z; out[3] = l->bounds.w;
+    return FRW_OK;
+}
+
+void frw_layout_free(frw_layout* l) { delete l; }
+
+int frw_green_thresholds(uint32_t out[FRW_NSAMPLE])
+{
+    if (!out) return set_error(FRW_E_INVALID, "null argument");
+    green_thresholds(out);
+    return FRW_OK;
+}
+
+// ---------------------------------------------------------------- tree
+
+void frw_tree_free(frw_tree* t)
+{
+    if (!t) return;
+    if (t->d_rects) hipFree(t->d_rects);
+    if (t->d_ids) hipFree(t->d_ids);
+    if (t->d_nodes) hipFree(t->d_nodes);
+    if (t->d_thresholds) hipFree(t->d_thresholds);
+    if (t->d_work) hipFree(t->d_work);
+    if (t->d_err) hipFree(t->d_err);
+    if (t->d_surface) hipFree(t->d_surface);
+    delete t;
+}
+
+int frw_tree_build(const frw_layout* l, const frw_tree_params* p, void* stream, frw_tree** out)
+{
+    if (!l || !out) return set_error(FRW_E_INVALID, "null argument");
+    if (l->rects.empty()) return set_error(FRW_E_RANGE, "layout has no rects");
+    const int cap = (p && p->leaf_cap > 0) ? p->leaf_cap : 16;
+    if (cap > 64) return set_error(FRW_E_RANGE, "leaf_cap must be <= 64");
+    FRW_GUARD_BEGIN
+    hipStream_t s = (hipStream_t)stream;
+    frw_tree* t = new frw_tree;
+    hipGetDevice(&t->device);
+    t->bounds = l->bounds;
+    t->n_rects = (int64_t)l->rects.size();
+    t->leaf_cap = cap;
+    t->host_rects = l->rects;
+    const double W = (double)l->bounds.z - (double)l->bounds.x;
+    const double H = (double)l->bounds.w - (double)l->bounds.y;
+    double dist = (H > W) ? H : W;
+    t->dist0 = dist * 0.05;  // BIGGEST_SQUARE_INIT_FACTOR (randomWalk/defines.h:21)
+    // Ambiguity band of the replayed bisection: 2^(k-48) where every
+    // coordinate and distance is below 2^(k-1); FP64 rounding errors of the
+    // compared quantities stay below 2^(k-53).
+    const double cmax = std::max({std::fabs((double)l->bounds.x), std::fabs((double)l->bounds.y),
+                                  std::fabs((double)l->bounds.z), std::fabs((double)l->bounds.w), W, H, 1.0});
+    int k = 0;
+    while (std::ldexp(1.0, k) <= 2.0 * cmax) ++k;
+    t->margin = std::ldexp(1.0, k - 48);
+    int rc = build_tree(t, l->rects, l->bounds, cap, s);
+    if (rc == FRW_OK) {
+        uint32_t T[kNSample];
+        green_thresholds(T);
+        rc = check_hip(hipMalloc(&t->d_thresholds, sizeof(T)), "hipMalloc thresholds");
+        if (rc == FRW_OK) rc = check_hip(hipMemcpy(t->d_thresholds, T, sizeof(T), hipMemcpyHostToDevice), "copy thresholds");
+        if (rc == FRW_OK) rc = check_hip(hipMalloc(&t->d_work, sizeof(unsigned long long)), "hipMalloc work");
+        if (rc == FRW_OK) rc = check_hip(hipMalloc(&t->d_err, sizeof(int)), "hipMalloc err");
+        if (rc == FRW_OK) rc = check_hip(hipMemset(t->d_err, 0, sizeof(int)), "memset err");
+        if (rc == FRW_OK) rc = check_hip(hipMalloc(&t->d_surface, 4 * sizeof(double)), "hipMalloc surface");
+    }
+    if (rc != FRW_OK) {
+        frw_tree_free(t);
+        return rc;
+    }
+    *out = t;
+    return FRW_OK;
+    FRW_GUARD_END
+}
+
+int frw_tree_info_get(const frw_tree* t, frw_tree_info* info)
+{
+    if (!t || !info) return set_error(FRW_E_INVALID, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    info->n_rects = t->n_rects;
+    info->n_nodes = t->n_nodes;
+    info->depth = t->depth;
+    info->leaf_cap = t->leaf_cap;
+    info->bounds[0] = t->bounds.x; info->bounds[1] = t->bounds.y;
+    info->bounds[2] = t->bounds.z; info->bounds[3] = t->bounds.w;
+    info->morton_bits = t->morton_bits;
+    info->build_ms = t->build_ms;
+    return FRW_OK;
+}
+
+int frw_tree_export(const frw_tree* t, int32_t* nodes, int32_t* rects, int32_t* ids)
+{
+    if (!t) return set_error(FRW_E_INVALID, "null tree");
+    int rc = FRW_OK;
+    if (nodes && rc == FRW_OK)
+        rc = check_hip(hipMemcpy(nodes, t->d_nodes, sizeof(Node) * t->n_nodes, hipMemcpyDeviceToHost), "export nodes");
+    if (rects && rc == FRW_OK)
+        rc = check_hip(hipMemcpy(rects, t->d_rects, sizeof(int4) * t->n_rects, hipMemcpyDeviceToHost), "export rects");
+    if (ids && rc == FRW_OK)
+        rc = check_hip(hipMemcpy(ids, t->d_ids, sizeof(int32_t) * t->n_rects, hipMemcpyDeviceToHost), "export ids");
+    return rc;
+}
+
+// ---------------------------------------------------------------- queries
+
+static int check_err_flag(const frw_tree* t, hipStream_t s)
+{
+    int err = 0;
+    int rc = check_hip(hipMemcpyAsync(&err, t->d_err, sizeof(int), hipMemcpyDeviceToHost, s), "read err flag");
+    if (rc == FRW_OK) rc = check_hip(hipStreamSynchronize(s), "sync");
+    if (rc != FRW_OK) return rc;
+    if (err) return set_error(FRW_E_KERNEL, "kernel error flag " + std::to_string(err));
+    return FRW_OK;
+}
+
+int frw_query_collide(const frw_tree* t, const double* boxes, int64_t n, uint8_t* out, void* stream)
+{
+    if (!t || n < 0 || (n > 0 && (!boxes || !out))) return set_error(FRW_E_INVALID, "bad query arguments");
+    return launch_q1(t, boxes, n, out, (hipStream_t)stream);
+}
+
+int frw_query_point(const frw_tree* t, const double* pts, int64_t n, int32_t* rect_id, void* stream)
+{
+    if (!t || n < 0 || (n > 0 && (!pts || !rect_id))) return set_error(FRW_E_INVALID, "bad query arguments");
+    return launch_q2(t, pts, n, rect_id, (hipStream_t)stream);
+}
+
+int frw_query_nearest(const frw_tree* t, const double* pts, int64_t n, double* dist, void* stream)
+{
+    if (!t || n < 0 || (n > 0 && (!pts || !dist))) return set_error(FRW_E_INVALID, "bad query arguments");
+    return launch_nearest(t, pts, n, dist, (hipStream_t)stream);
+}
+
+int frw_query_maxsquare(const frw_tree* t, const double* pts, int64_t n, int32_t mode,
+                        double* boxes_out, void* stream)
+{
+    if (!t || n < 0 || (n > 0 && (!pts || !boxes_out))) return set_error(FRW_E_INVALID, "bad query arguments");
+    if (mode != FRW_MODE_EXACT && mode != FRW_MODE_BISECT) return set_error(FRW_E_INVALID, "bad mode");
+    int rc = launch_q3(t, pts, n, mode, boxes_out, (hipStream_t)stream);
+    if (rc != FRW_OK) return rc;
+    return check_err_flag(t, (hipStream_t)stream);
+}
+
+int frw_gauss_surface(const frw_tree* t, int32_t master, double out[4])
+{
+    if (!t || !out) return set_error(FRW_E_INVALID, "null argument");
+    if (master < 0 || master >= t->n_rects) return set_error(FRW_E_RANGE, "master out of range");
+    int rc = launch_gauss(t, t->host_rects[master], t->d_surface, nullptr);
+    if (rc == FRW_OK) rc = check_hip(hipMemcpy(out, t->d_surface, 4 * sizeof(double), hipMemcpyDeviceToHost), "copy surface");
+    return rc;
+}
+
+// ---------------------------------------------------------------- walks
+
+static int check_walk_params(const frw_tree* t, const frw_walk_params* p)
+{
+    if (!t || !p) return set_error(FRW_E_INVALID, "null argument");
+    if (p->master < 0 || p->master >= t->n_rects) return set_error(FRW_E_RANGE, "master out of range");
+    if (p->n_walks <= 0) return set_error(FRW_E_RANGE, "walk count must be positive");
+    if (p->first_walk < 0 || p->first_walk + p->n_walks > ((int64_t)1 << 32))
+        return set_error(FRW_E_RANGE, "walk ids must fit in 32 bits");
+    if (p->mode != FRW_MODE_EXACT && p->mode != FRW_MODE_BISECT)
+        return set_error(FRW_E_INVALID, "unsupported walk mode");
+    if (p->max_hops < 0 || p->max_hops > 0xFFFFFFFFll) return set_error(FRW_E_RANGE, "bad max_hops");
+    return FRW_OK;
+}
+
+int frw_walk_async(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o, void* stream)
+{
+    int rc = check_walk_params(t, p);
+    if (rc != FRW_OK) return rc;
+    if (!o || !o->counts || !o->hops) return set_error(FRW_E_INVALID, "walk outputs need counts and hops");
+    FRW_GUARD_BEGIN
+    return launch_walk(t, p, o, (hipStream_t)stream);
+    FRW_GUARD_END
+}
+
+int frw_walk(const frw_tree* t, const frw_walk_params* p, int64_t* counts_host, frw_walk_summary* summary)
+{
+    int rc = check_walk_params(t, p);
+    if (rc != FRW_OK) return rc;
+    FRW_GUARD_BEGIN
+    const int64_t nb = t->n_rects + 2;
+    int64_t* d_counts = nullptr;
+    int64_t* d_hops = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<int64_t> counts((size_t)nb);
+    int64_t hops = 0;
+    float ms = 0.f;
+    rc = check_hip(hipMalloc(&d_counts, sizeof(int64_t) * nb), "hipMalloc counts");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hops, sizeof(int64_t)), "hipMalloc hops");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_counts, 0, sizeof(int64_t) * nb), "memset counts");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_hops, 0, sizeof(int64_t)), "memset hops");
+    if (rc == FRW_OK) {
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        frw_walk_outputs o{d_counts, d_hops, nullptr, nullptr};
+        hipEventRecord(e0, nullptr);
+        rc = launch_walk(t, p, &o, nullptr);
+        hipEventRecord(e1, nullptr);
+    }
+    if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "walk");
+    if (rc == FRW_OK) rc = check_err_flag(t, nullptr);
+    if (rc == FRW_OK) rc = check_hip(hipMemcpy(counts.data(), d_counts, sizeof(int64_t) * nb, hipMemcpyDeviceToHost), "copy counts");
+    if (rc == FRW_OK) rc = check_hip(hipMemcpy(&hops, d_hops, sizeof(int64_t), hipMemcpyDeviceToHost), "copy hops");
+    if (rc == FRW_OK) hipEventElapsedTime(&ms, e0, e1);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (d_counts) hipFree(d_counts);
+    if (d_hops) hipFree(d_hops);
+    if (rc != FRW_OK) return rc;
+    if (counts_host) std::memcpy(counts_host, counts.data(), sizeof(int64_t) * nb);
+    if (summary) {
+        summary->n_walks = p->n_walks;
+        summary->hops = hops;
+        summary->n_master = counts[p->master];
+        summary->n_escaped = counts[t->n_rects];
+        summary->n_stuck = counts[t->n_rects + 1];
+        // getAvgPathLen (randomWalk/CPU/getAvgPathLen.cpp:96,113): hits not
+        // equal (by value) to the master, escapes and stuck walks included.
+        int64_t same = 0;
+        const int4 m = t->host_rects[p->master];
+        for (int64_t i = 0; i < t->n_rects; ++i) {
+            const int4 r = t->host_rects[i];
+            if (r.x == m.x && r.y == m.y && r.z == m.z && r.w == m.w) same += counts[i];
+        }
+        summary->fraction_other = (double)(p->n_walks - same) / (double)p->n_walks;
+        summary->walk_ms = ms;
+    }
+    return FRW_OK;
+    FRW_GUARD_END
+}
+
+int frw_walk_traffic(const frw_tree* t, const frw_walk_params* p, int64_t out[3])
+{
+    int rc = check_walk_params(t, p);
+    if (rc != FRW_OK) return rc;
+    if (!out) return set_error(FRW_E_INVALID, "null output");
+    FRW_GUARD_BEGIN
+    const int64_t nb = t->n_rects + 2;
+    int64_t *d_counts = nullptr, *d_hops = nullptr, *d_tr = nullptr;
+    rc = check_hip(hipMalloc(&d_counts, sizeof(int64_t) * nb), "hipMalloc counts");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hops, sizeof(int64_t)), "hipMalloc hops");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_tr, 2 * sizeof(int64_t)), "hipMalloc traffic");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_counts, 0, sizeof(int64_t) * nb), "memset");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_hops, 0, sizeof(int64_t)), "memset");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_tr, 0, 2 * sizeof(int64_t)), "memset");
+    if (rc == FRW_OK) {
+        frw_walk_outputs o{d_counts, d_hops, nullptr, nullptr};
+        rc = launch_walk(t, p, &o, nullptr, (unsigned long long*)d_tr);
+    }
+    if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "walk");
+    if (rc == FRW_OK) rc = check_err_flag(t, nullptr);
+    if (rc == FRW_OK) rc = check_hip(hipMemcpy(out, d_tr, 2 * sizeof(int64_t), hipMemcpyDeviceToHost), "copy");
+    if (rc == FRW_OK) rc = check_hip(hipMemcpy(out + 2, d_hops, sizeof(int64_t), hipMemcpyDeviceToHost), "copy");
+    if (d_counts) hipFree(d_counts);
+    if (d_hops) hipFree(d_hops);
+    if (d_tr) hipFree(d_tr);
+    return rc;
+    FRW_GUARD_END
+}
+
+int frw_write_results(const frw_layout* l, const int64_t* counts_host, int64_t n_walks, const char* path)
+{
+    if (!l || !counts_host || !path || n_walks <= 0) return set_error(FRW_E_INVALID, "bad arguments");
+    FRW_GUARD_BEGIN
+    return write_results(l->rects, counts_host, n_walks, path);
+    FRW_GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,369 @@
+// Level-synchronous Morton quadtree build on gfx950.
+//
+// Replaces the reference's recursive dynamic-parallelism build
+// (randomWalk/Cuda/createquadtree.cu:97-371, one block per node):
+//   1. key per rect: the deepest quadtree cell that holds the whole rect is
+//      the longest common 2-bit prefix of the Morton codes of its two corners;
+//      key = (cell code padded with zeros) << 6 | cell level.
+//   2. one device radix sort of (key, rect id).  Sorted by key, the rects of
+//      any cell form one contiguous run, and the rects that straddle the
+//      cell's centre lines (they stop at that cell) come first in the run.
+//   3. level by level: every frontier node finds its own rects and its four
+//      child runs by binary search over the sorted keys; a node becomes a leaf
+//      when its run holds <= leaf_cap rects (or it is a unit cell).  Child
+//      groups are allocated by an exclusive scan, so the flat array is in
+//      breadth-first order and deterministic.
+//   4. each node's box = tight bounding box of its run (block reduction).
+// Query answers depend only on the rect set, never on this structure.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+
+#include "frw_internal.h"
+
+namespace frw {
+namespace {
+
+struct Frontier {
+    uint64_t code;   // cell code padded with zeros (2K bits)
+    int32_t node;    // index in the flat node array
+    int32_t level;
+    int32_t s, e;    // run of sorted rects in this subtree
+};
+
+__device__ inline uint64_t spread_bits(uint32_t v)
+{
+    uint64_t x = v;
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+
+// Quadrant order inside a group: 0 = (low x, low y) = top-left (y down),
+// 1 = top-right, 2 = bottom-left, 3 = bottom-right.
+__device__ inline uint64_t morton(uint32_t ux, uint32_t uy)
+{
+    return spread_bits(ux) | (spread_bits(uy) << 1);
+}
+
+__global__ void rect_keys(const int4* __restrict__ rects, int n, int4 B, int K,
+                          uint64_t* __restrict__ keys, int32_t* __restrict__ vals)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int4 r = rects[i];
+    const uint64_t m0 = morton((uint32_t)(r.x - B.x), (uint32_t)(r.y - B.y));
+    const uint64_t m1 = morton((uint32_t)(r.z - B.x), (uint32_t)(r.w - B.y));
+    const uint64_t d = m0 ^ m1;
+    int level = K;
+    if (d) {
+        const int msb = 63 - __clzll((long long)d);
+        level = (2 * K - 1 - msb) / 2;
+    }
+    const int low = 2 * (K - level);
+    const uint64_t code = low >= 64 ? 0ull : (m0 >> low) << low;
+    keys[i] = (code << 6) | (uint64_t)level;
+    vals[i] = i;
+}
+
+__global__ void gather_rects(const int4* __restrict__ rects, const int32_t* __restrict__ order,
+                             int n, int4* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = rects[order[i]];
+}
+
+// first index in [lo, hi) with keys[idx] >= v
+__device__ inline int lower_bound_key(const uint64_t* keys, int lo, int hi, uint64_t v)
+{
+    while (lo < hi) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (keys[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline int upper_bound_key(const uint64_t* keys, int lo, int hi, uint64_t v)
+{
+    while (lo < hi) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (keys[mid] <= v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void split_level(const uint64_t* __restrict__ keys, const Frontier* __restrict__ fr,
+                            int nf, int K, int cap, Node* __restrict__ nodes,
+                            int2* __restrict__ runs, int32_t* __restrict__ internal,
+                            int32_t* __restrict__ own_end)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    const Frontier f = fr[i];
+    Node nd;
+    nd.level = f.level;
+    nd.first_child = -1;
+    nd.box = make_int4(INT_MAX, INT_MAX, INT_MIN, INT_MIN);
+    runs[f.node] = make_int2(f.s, f.e);
+    int is_int = 0, oe = f.e;
+    if (f.s == f.e) {
+        nd.rbeg = nd.rend = 0;  // empty quadrant
+    } else {
+        const uint64_t own_key = (f.code << 6) | (uint64_t)f.level;
+        oe = upper_bound_key(keys, f.s, f.e, own_key);
+        if (f.e - f.s <= cap || f.level >= K || oe == f.e) {
+            nd.rbeg = f.s;
+            nd.rend = f.e;
+        } else {
+            nd.rbeg = f.s;
+            nd.rend = oe;
+            is_int = 1;
+        }
+    }
+    nodes[f.node] = nd;
+    internal[i] = is_int;
+    own_end[i] = oe;
+}
+
+__global__ void emit_children(const uint64_t* __restrict__ keys, const Frontier* __restrict__ fr,
+                              int nf, int K, const int32_t* __restrict__ internal,
+                              const int32_t* __restrict__ group, const int32_t* __restrict__ own_end,
+                              int base, Node* __restrict__ nodes, Frontier* __restrict__ next)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf || !internal[i]) return;
+    const Frontier f = fr[i];
+    const int g = group[i];
+    const int first = base + 4 * g;
+    nodes[f.node].first_child = first;
+    const int shift = 2 * (K - f.level - 1);
+    int b[5];
+    b[0] = own_end[i];
+    b[4] = f.e;
+#pragma unroll
+    for (int c = 1; c < 4; ++c)
+        b[c] = lower_bound_key(keys, b[c - 1], f.e, (f.code | ((uint64_t)c << shift)) << 6);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        Frontier ch;
+        ch.code = f.code | ((uint64_t)c << shift);
+        ch.node = first + c;
+        ch.level = f.level + 1;
+        ch.s = b[c];
+        ch.e = b[c + 1];
+        next[4 * g + c] = ch;
+    }
+}
+
+// One block per node: tight bounding box of its run.
+__global__ void __launch_bounds__(256) node_boxes(const int4* __restrict__ rects,
+                                                  const int2* __restrict__ runs, int n_nodes,
+                                                  Node* __restrict__ nodes)
+{
+    const int ni = blockIdx.x;
+    if (ni >= n_nodes) return;
+    const int2 run = runs[ni];
+    int x0 = INT_MAX, y0 = INT_MAX, x1 = INT_MIN, y1 = INT_MIN;
+    for (int r = run.x + (int)threadIdx.x; r < run.y; r += blockDim.x) {
+        const int4 q = rects[r];
+        x0 = min(x0, q.x); y0 = min(y0, q.y); x1 = max(x1, q.z); y1 = max(y1, q.w);
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        x0 = min(x0, __shfl_xor(x0, off));
+        y0 = min(y0, __shfl_xor(y0, off));
+        x1 = max(x1, __shfl_xor(x1, off));
+        y1 = max(y1, __shfl_xor(y1, off));
+    }
+    __shared__ int4 part[4];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0) part[wave] = make_int4(x0, y0, x1, y1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int4 b = part[0];
+        for (int w = 1; w < (int)(blockDim.x / 64); ++w) {
+            b.x = min(b.x, part[w].x); b.y = min(b.y, part[w].y);
+            b.z = max(b.z, part[w].z); b.w = max(b.w, part[w].w);
+        }
+        nodes[ni].box = b;
+    }
+}
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    int reserve(size_t n, hipStream_t s, bool keep = false, size_t keep_n = 0)
+    {
+        if (n <= cap) return FRW_OK;
+        size_t nc = std::max<size_t>(n, cap * 2);
+        T* q = nullptr;
+        if (hipMalloc(&q, nc * sizeof(T)) != hipSuccess) return set_error(FRW_E_NOMEM, "hipMalloc failed in tree build");
+        if (keep && p && keep_n)
+            if (hipMemcpyAsync(q, p, keep_n * sizeof(T), hipMemcpyDeviceToDevice, s) != hipSuccess)
+                return set_error(FRW_E_DEVICE, "copy failed in tree build");
+        if (p) {
+            hipStreamSynchronize(s);
+            hipFree(p);
+        }
+        p = q;
+        cap = nc;
+        return FRW_OK;
+    }
+    void release()
+    {
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+inline int blocks_for(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+}  // namespace
+
+int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 B, int leaf_cap, hipStream_t s)
+{
+    const int n = (int)rects.size();
+    const int64_t W = (int64_t)B.z - B.x, H = (int64_t)B.w - B.y;
+    int K = 0;
+    while (K < 30 && ((int64_t)1 << K) <= std::max(W, H)) ++K;
+    if (((int64_t)1 << K) <= std::max(W, H) || K > 29)
+        return set_error(FRW_E_RANGE, "layout bounds wider than 2^29");
+    t->morton_bits = K;
+
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, s);
+
+    int rc = FRW_OK;
+    int4* d_in = nullptr;
+    uint64_t *d_keys = nullptr, *d_keys_sorted = nullptr;
+    int32_t* d_vals = nullptr;
+    void* d_tmp = nullptr;
+    size_t tmp_bytes = 0;
+    DevBuf<Frontier> fa, fb;
+    DevBuf<int32_t> internal, group, own_end;
+    DevBuf<int2> runs;
+    DevBuf<Node> nodes;
+    std::vector<Frontier> root(4);
+    int n_nodes = 4, nf = 4, depth = 0;
+    const int nn = std::max(n, 1);
+
+#define FRW_TRY(x)                                          \
+    do {                                                    \
+        if ((rc = check_hip((x), #x)) != FRW_OK) goto done; \
+    } while (0)
+#define FRW_TRYRC(x)                        \
+    do {                                    \
+        if ((rc = (x)) != FRW_OK) goto done; \
+    } while (0)
+
+    FRW_TRY(hipMalloc(&d_in, sizeof(int4) * nn));
+    FRW_TRY(hipMalloc(&t->d_rects, sizeof(int4) * nn));
+    FRW_TRY(hipMalloc(&t->d_ids, sizeof(int32_t) * nn));
+    FRW_TRY(hipMalloc(&d_keys, sizeof(uint64_t) * nn));
+    FRW_TRY(hipMalloc(&d_keys_sorted, sizeof(uint64_t) * nn));
+    FRW_TRY(hipMalloc(&d_vals, sizeof(int32_t) * nn));
+    if (n > 0) FRW_TRY(hipMemcpyAsync(d_in, rects.data(), sizeof(int4) * n, hipMemcpyHostToDevice, s));
+
+    if (n > 0) {
+        rect_keys<<<blocks_for(n, 256), 256, 0, s>>>(d_in, n, B, K, d_keys, d_vals);
+        FRW_TRY(hipGetLastError());
+        const int end_bit = 2 * K + 6;
+        FRW_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, d_keys, d_keys_sorted, d_vals,
+                                                   t->d_ids, n, 0, end_bit, s));
+        FRW_TRY(hipMalloc(&d_tmp, std::max<size_t>(tmp_bytes, 16)));
+        FRW_TRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp_bytes, d_keys, d_keys_sorted, d_vals,
+                                                   t->d_ids, n, 0, end_bit, s));
+        gather_rects<<<blocks_for(n, 256), 256, 0, s>>>(d_in, t->d_ids, n, t->d_rects);
+        FRW_TRY(hipGetLastError());
+    }
+
+    // level 0: the root group (root + three empty quadrants)
+    root[0] = Frontier{0ull, 0, 0, 0, n};
+    for (int c = 1; c < 4; ++c) root[c] = Frontier{0ull, c, 0, 0, 0};
+    FRW_TRYRC(fa.reserve(4, s));
+    FRW_TRYRC(nodes.reserve(1024, s));
+    FRW_TRYRC(runs.reserve(1024, s));
+    FRW_TRY(hipMemcpyAsync(fa.p, root.data(), sizeof(Frontier) * 4, hipMemcpyHostToDevice, s));
+
+    while (nf > 0) {
+        depth++;
+        FRW_TRYRC(internal.reserve(nf, s));
+        FRW_TRYRC(group.reserve(nf, s));
+        FRW_TRYRC(own_end.reserve(nf, s));
+        split_level<<<blocks_for(nf, 256), 256, 0, s>>>(d_keys_sorted, fa.p, nf, K, leaf_cap,
+                                                        nodes.p, runs.p, internal.p, own_end.p);
+        FRW_TRY(hipGetLastError());
+        size_t need = 0;
+        FRW_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, need, internal.p, group.p, nf, s));
+        if (need > tmp_bytes || !d_tmp) {
+            if (d_tmp) {
+                hipStreamSynchronize(s);
+                hipFree(d_tmp);
+            }
+            tmp_bytes = std::max<size_t>(need, 16);
+            FRW_TRY(hipMalloc(&d_tmp, tmp_bytes));
+        }
+        FRW_TRY(hipcub::DeviceScan::ExclusiveSum(d_tmp, need, internal.p, group.p, nf, s));
+        int last_g = 0, last_i = 0;
+        FRW_TRY(hipMemcpyAsync(&last_g, group.p + nf - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        FRW_TRY(hipMemcpyAsync(&last_i, internal.p + nf - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        FRW_TRY(hipStreamSynchronize(s));
+        const int n_int = last_g + last_i;
+        if (n_int == 0) break;
+        const int nf_next = 4 * n_int;
+        FRW_TRYRC(nodes.reserve((size_t)n_nodes + nf_next, s, true, n_nodes));
+        FRW_TRYRC(runs.reserve((size_t)n_nodes + nf_next, s, true, n_nodes));
+        FRW_TRYRC(fb.reserve(nf_next, s));
+        emit_children<<<blocks_for(nf, 256), 256, 0, s>>>(d_keys_sorted, fa.p, nf, K, internal.p,
+                                                          group.p, own_end.p, n_nodes, nodes.p,
+                                                          fb.p);
+        FRW_TRY(hipGetLastError());
+        n_nodes += nf_next;
+        std::swap(fa, fb);
+        nf = nf_next;
+    }
+
+    node_boxes<<<n_nodes, 256, 0, s>>>(t->d_rects, runs.p, n_nodes, nodes.p);
+    FRW_TRY(hipGetLastError());
+    FRW_TRY(hipMalloc(&t->d_nodes, sizeof(Node) * n_nodes));
+    FRW_TRY(hipMemcpyAsync(t->d_nodes, nodes.p, sizeof(Node) * n_nodes, hipMemcpyDeviceToDevice, s));
+    hipEventRecord(e1, s);
+    FRW_TRY(hipStreamSynchronize(s));
+    {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        t->build_ms = ms;
+    }
+    t->n_nodes = n_nodes;
+    t->depth = depth;
+
+done:
+#undef FRW_TRY
+#undef FRW_TRYRC
+    hipStreamSynchronize(s);
+    if (d_in) hipFree(d_in);
+    if (d_keys) hipFree(d_keys);
+    if (d_keys_sorted) hipFree(d_keys_sorted);
+    if (d_vals) hipFree(d_vals);
+    if (d_tmp) hipFree(d_tmp);
+    fa.release();
+    fb.release();
+    internal.release();
+    group.release();
+    own_end.release();
+    runs.release();
+    nodes.release();
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+
+}  // namespace frw

@@ -1,0 +1,359 @@
+// Device-side building blocks of the FRW hot path for gfx950 (CDNA4).
+//
+// Geometry conventions (reference randomWalk/CPU/rectHost.h:6-47): a rect is
+// TL=(x0,y0), BR=(x1,y1), y grows downward.  Rects and tree-node boxes are
+// int32 (16-byte records); walker positions and squares are FP64 so that every
+// hop reproduces the reference's FP64 arithmetic operation by operation (an
+// FP32 position has a 1.0 ulp at 10^7 and cannot hold the reference's p+-1 /
+// 0.095 steps).  All rect tests compare an int32 (converted exactly) against
+// an FP64 coordinate, or take the sign of a single rounded difference, so they
+// are exact.  Every translation unit including this header is compiled with
+// -ffp-contract=off.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "frw_types.h"
+
+namespace frw {
+
+// ---------------------------------------------------------------- Philox
+// Philox4x32-10 (Salmon et al., SC'11).  Walk stream: counter =
+// (walk, job, hop/4, 'WALK'), key = seed; word hop%4 is the hop's uniform.
+__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+constexpr uint32_t kWalkDomain = 0x57414C4Bu;  // 'WALK'
+
+// ---------------------------------------------------------------- sampling
+// Panel index = largest i with T[i] <= u (T[0] = 0): an 8-step branch-free
+// binary search over the 200 thresholds of the Green CDF.
+__device__ inline int sample_panel(const uint32_t* T, uint32_t u)
+{
+    int lo = 0;
+#pragma unroll
+    for (int step = 128; step >= 1; step >>= 1) {
+        const int probe = lo + step;
+        if (probe < kNSample && T[probe] <= u) lo = probe;
+    }
+    return lo;
+}
+
+// Reference getPointFromNindex (randomWalk/CPU/rectHost.cpp:102-161): same
+// branches, same FP64 operation order, integer-truncated perimeter and the
+// int cast on the top side.
+__host__ __device__ inline void point_from_panel(double sx0, double sy0, double sx1, double sy1,
+                                                 int index, double& px, double& py)
+{
+    const double perimeter = (double)(int)(2 * (sx1 - sx0) + 2 * (sy1 - sy0));
+    const double v = perimeter / (double)kNSample;
+    const double h = sy1 - sy0;
+    const double w = sx1 - sx0;
+    const double a = index * v;
+    const double b = (index - 1) * v;
+    if (a < w) {
+        px = (double)(int)(b + v / 2 + sx0);
+        py = sy0;
+    } else if (b < w && a > w) {
+        px = sx1;
+        py = sy0;
+    } else if (a < (w + h)) {
+        px = sx1;
+        py = (b > w) ? (sy0 + b - w + v / 2) : (sy0 + a - w + (a - w) / 2);
+    } else if (b < (w + h) && a > (w + h)) {
+        px = sx1;
+        py = sy1;
+    } else if (a < (2 * w + h)) {
+        py = sy1;
+        px = (b > w + h) ? (sx1 - (b - w - h + v / 2)) : (sx1 - (a - w - h + (a - w - h) / 2));
+    } else if (b < (2 * w + h) && a > (2 * w + h)) {
+        px = sx0;
+        py = sy1;
+    } else {
+        px = sx0;
+        py = (b > 2 * w + h) ? (sy1 - (b - 2 * w - h + v / 2))
+                             : (sy1 - (a - 2 * w - h + (a - 2 * w - h) / 2));
+    }
+}
+
+// ---------------------------------------------------------------- geometry
+struct DBox {
+    double x0, y0, x1, y1;
+};
+
+// Tree::isInBounds(point), strict (randomWalk/CPU/quadTree.cpp:49-59).
+__device__ inline bool in_bounds_strict(const int4& B, double px, double py)
+{
+    return px > (double)B.x && py > (double)B.y && px < (double)B.z && py < (double)B.w;
+}
+
+// Tree::isInBounds(rect), inclusive (randomWalk/CPU/quadTree.cpp:38-47).
+__device__ inline bool in_bounds_box(const int4& B, double x0, double y0, double x1, double y1)
+{
+    return y0 >= (double)B.y && x0 >= (double)B.x && y1 <= (double)B.w && x1 <= (double)B.z;
+}
+
+// rectsCollision (randomWalk/CPU/rectHost.cpp:43-52): closed, touching collides.
+__device__ inline bool box_hits_rect(const int4& r, double x0, double y0, double x1, double y1)
+{
+    return (double)r.z >= x0 && x1 >= (double)r.x && (double)r.w >= y0 && y1 >= (double)r.y;
+}
+
+// Signed L-infinity distance from p to a rect or node box; <= 0 exactly when
+// the closed box contains p.
+__device__ inline double linf(const int4& r, double px, double py)
+{
+    const double dx = fmax((double)r.x - px, px - (double)r.z);
+    const double dy = fmax((double)r.y - py, py - (double)r.w);
+    return fmax(dx, dy);
+}
+
+// Rect equality by value (the reference compares RectHost by value when it
+// ignores the master, randomWalk/CPU/quadTree.cpp:219-226).
+__device__ inline bool same_rect(const int4& a, const int4& b)
+{
+    return a.x == b.x && a.y == b.y && a.z == b.z && a.w == b.w;
+}
+
+// ---------------------------------------------------------------- tree view
+struct DevTree {
+    const int4* rects;     // sorted by Morton key, 16 B each
+    const int32_t* ids;    // original index of each sorted rect
+    const Node* nodes;     // groups of 4 siblings, 128 B aligned
+    int4 bounds;           // absorbing root bounds
+    int32_t n_rects;
+    int32_t n_nodes;
+    int32_t depth;         // number of levels
+    int32_t stack_cap;     // per-lane traversal stack entries
+    double margin;         // ambiguity band of the Q3 schedule emulation
+    double dist0;          // 0.05 * max(W, H), first Q3 step
+};
+
+// Optional per-lane traffic counters: node records (32 B) and rect records
+// (16 B) read by the traversals.
+struct Counters {
+    uint32_t nodes;
+    uint32_t rects;
+};
+
+// Per-lane traversal stack in LDS: stack[slot * stride + lane_in_block].
+struct LaneStack {
+    int32_t* base;
+    int stride;
+    __device__ int32_t& at(int slot) const { return base[slot * stride]; }
+};
+
+// Q1 (checkCollisions, randomWalk/CPU/quadTree.cpp:148-198 with the full
+// depth-first search of randomWalk/Cuda/d_quadTree.cu:136-193): is the closed
+// box outside the root bounds, or does it intersect any rect (other than
+// `ignore`, by value)?
+template <bool kIgnore>
+__device__ inline bool q1_box(const DevTree& t, const LaneStack& st, double x0, double y0,
+                              double x1, double y1, int4 ignore, Counters* cnt)
+{
+    if (!in_bounds_box(t.bounds, x0, y0, x1, y1)) return true;
+    int sp = 0;
+    st.at(sp++) = 0;
+    while (sp > 0) {
+        const int ni = st.at(--sp);
+        const Node nd = t.nodes[ni];
+        if (cnt) cnt->nodes += nd.first_child >= 0 ? 5u : 1u;
+        for (int r = nd.rbeg; r < nd.rend; ++r) {
+            const int4 rc = t.rects[r];
+            if (cnt) cnt->rects++;
+            if (box_hits_rect(rc, x0, y0, x1, y1) && !(kIgnore && same_rect(rc, ignore)))
+                return true;
+        }
+        if (nd.first_child >= 0) {
+#pragma unroll
+            for (int c = 3; c >= 0; --c) {
+                const int4 cb = t.nodes[nd.first_child + c].box;
+                if (box_hits_rect(cb, x0, y0, x1, y1)) st.at(sp++) = nd.first_child + c;
+            }
+        }
+    }
+    return false;
+}
+
+// Nearest rect in the L-infinity metric: min over rects of linf(rect, p),
+// by branch and bound over the node boxes (children visited nearest first).
+// The minimum equals a brute-force scan bit for bit: the same per-rect
+// expression is evaluated, and a node is skipped only when its box distance
+// (a lower bound under monotone rounding) cannot beat the best.
+__device__ inline double nearest_rect(const DevTree& t, const LaneStack& st, double px, double py,
+                                      Counters* cnt)
+{
+    double best = __builtin_inf();
+    int sp = 0;
+    st.at(sp++) = 0;
+    while (sp > 0) {
+        const int ni = st.at(--sp);
+        const Node nd = t.nodes[ni];
+        if (cnt) cnt->nodes++;
+        if (!(linf(nd.box, px, py) < best)) continue;
+        if (cnt) cnt->rects += (uint32_t)(nd.rend - nd.rbeg);
+        if (cnt && nd.first_child >= 0) cnt->nodes += 4u;
+        for (int r = nd.rbeg; r < nd.rend; ++r) best = fmin(best, linf(t.rects[r], px, py));
+        if (nd.first_child >= 0) {
+            double lb[4];
+            int id[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                lb[c] = linf(t.nodes[nd.first_child + c].box, px, py);
+                id[c] = nd.first_child + c;
+            }
+            // sort descending so the nearest child is pushed last (popped first)
+#pragma unroll
+            for (int i = 1; i < 4; ++i)
+#pragma unroll
+                for (int j = i; j > 0; --j)
+                    if (lb[j] > lb[j - 1]) {
+                        const double tl = lb[j]; lb[j] = lb[j - 1]; lb[j - 1] = tl;
+                        const int ti = id[j]; id[j] = id[j - 1]; id[j - 1] = ti;
+                    }
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (lb[c] < best) st.at(sp++) = id[c];
+        }
+    }
+    return best;
+}
+
+// Q2 (checkCollisons(point), randomWalk/CPU/quadTree.cpp:240-262, answered
+// exactly): original index of the rect containing p (closed), deepest
+// containment first, then lowest index; -1 if none.
+__device__ inline int q2_point(const DevTree& t, const LaneStack& st, double px, double py,
+                               Counters* cnt)
+{
+    double best = 0.0;
+    int best_id = -1;
+    int sp = 0;
+    st.at(sp++) = 0;
+    while (sp > 0) {
+        const int ni = st.at(--sp);
+        const Node nd = t.nodes[ni];
+        if (cnt) {
+            cnt->nodes += nd.first_child >= 0 ? 5u : 1u;
+            cnt->rects += (uint32_t)(nd.rend - nd.rbeg);
+        }
+        for (int r = nd.rbeg; r < nd.rend; ++r) {
+            const double d = linf(t.rects[r], px, py);
+            if (d <= 0.0) {
+                const int id = t.ids[r];
+                if (best_id < 0 || d < best || (d == best && id < best_id)) {
+                    best = d;
+                    best_id = id;
+                }
+            }
+        }
+        if (nd.first_child >= 0) {
+#pragma unroll
+            for (int c = 3; c >= 0; --c)
+                if (linf(t.nodes[nd.first_child + c].box, px, py) <= 0.0)
+                    st.at(sp++) = nd.first_child + c;
+        }
+    }
+    return best_id;
+}
+
+// ---------------------------------------------------------------- Q3
+// drawBiggestSquareAtPoint (randomWalk/CPU/quadTree.cpp:264-312): returns the
+// p+-1 or p+-2 square when it collides, else runs the reference's grow/shrink
+// bisection.  The bisection's only input is the collision test of each
+// candidate box, so the schedule is replayed here on the four FP64
+// coordinates exactly as the reference updates them, and each test is
+// decided by the nearest-rect distance d*: every box side at least d*+margin
+// from p collides, every side below d*-margin does not, and only boxes inside
+// that band (rounding-ambiguous) fall back to an exact Q1.  kLiteral runs a
+// Q1 for every test instead (the reference algorithm as is).
+template <bool kLiteral>
+__device__ inline DBox biggest_square(const DevTree& t, const LaneStack& st, double px, double py,
+                                      double dstar, uint32_t* n_q1, Counters* cnt, int* err)
+{
+    const double MIN_DIST = (double)0.095f;
+    auto collides = [&](double a, double c, double b, double e) -> bool {
+        if (!in_bounds_box(t.bounds, a, c, b, e)) return true;
+        if (!kLiteral) {
+            const double lo = fmin(fmin(px - a, b - px), fmin(py - c, e - py));
+            const double hi = fmax(fmax(px - a, b - px), fmax(py - c, e - py));
+            if (lo >= dstar + t.margin) return true;
+            if (hi < dstar - t.margin) return false;
+        }
+        if (n_q1) (*n_q1)++;
+        return q1_box<false>(t, st, a, c, b, e, make_int4(0, 0, 0, 0), cnt);
+    };
+    DBox o = {px - 1, py - 1, px + 1, py + 1};
+    if (collides(o.x0, o.y0, o.x1, o.y1)) return o;
+    const DBox init = {px - 2, py - 2, px + 2, py + 2};
+    if (collides(init.x0, init.y0, init.x1, init.y1)) return init;
+    double dist = t.dist0;
+    bool max_reached = false;
+    for (int it = 0;; ++it) {
+        const bool c = collides(o.x0, o.y0, o.x1, o.y1);
+        if (!(c || !max_reached || dist > MIN_DIST)) break;
+        if (it >= kMaxBisect) {  // cannot happen for a finite layout; reported
+            if (err) atomicOr(err, kErrBisect);
+            break;
+        }
+        if (c) {
+            if (dist > MIN_DIST) dist /= 2;
+            max_reached = true;
+            o.x0 += dist; o.y0 += dist; o.x1 -= dist; o.y1 -= dist;
+        } else {
+            if (max_reached) dist /= 2;
+            o.x0 -= dist; o.y0 -= dist; o.x1 += dist; o.y1 += dist;
+        }
+    }
+    return o;
+}
+
+// Gaussian-surface factor search (getAdjustedGaussianFactor,
+// randomWalk/CPU/quadTree.cpp:315-349) and createGaussianSurface
+// (randomWalk/CPU/rectHost.cpp:84-100).
+__device__ inline DBox gauss_surface(const int4& r, double fx, double fy)
+{
+    const double mx = ((double)r.x + (double)r.z) / 2.;
+    const double my = ((double)r.y + (double)r.w) / 2.;
+    double vx = (double)r.x - mx;
+    double vy = (double)r.y - my;
+    vx *= fx;
+    vy *= fy;
+    return DBox{mx + vx, my + vy, mx - vx, my - vy};
+}
+
+__device__ inline double gauss_factor(const DevTree& t, const LaneStack& st, const int4& r,
+                                      double factor, bool axis_y)
+{
+    bool dividing = true, first = true;
+    double f = factor, lo = 1., hi = factor;
+    for (int i = 0; i < 10; i++) {
+        const DBox s = axis_y ? gauss_surface(r, 1., f) : gauss_surface(r, f, 1.);
+        const bool coll = !in_bounds_box(t.bounds, s.x0, s.y0, s.x1, s.y1) ||
+                          q1_box<true>(t, st, s.x0, s.y0, s.x1, s.y1, r, nullptr);
+        if (first && !coll) break;
+        if ((coll && !dividing) || (!coll && dividing)) dividing = !dividing;
+        if (dividing)
+            f = hi = (lo + hi) / 2.;
+        else
+            f = lo = (lo + hi) / 2.;
+        first = false;
+    }
+    return f;
+}
+
+}  // namespace frw

@@ -1,0 +1,289 @@
+// Host-side layout handling: the reference layout-file reader, padded bounds,
+// synthetic layout generators, the Green CDF table and the result writer.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "frw_internal.h"
+
+namespace frw {
+
+// Padded absorbing bounds, Parser::getLayerSize (randomWalk/CPU/Parser.cpp:51-87):
+// the bbox grown on all four sides by int(0.01 * width).
+int4 padded_bounds(const std::vector<int4>& r)
+{
+    if (r.empty()) return make_int4(0, 0, 0, 0);
+    double lx = r[0].x, ty = r[0].y, rx = r[0].z, by = r[0].w;
+    for (size_t i = 1; i < r.size(); ++i) {
+        lx = std::min(lx, (double)r[i].x);
+        ty = std::min(ty, (double)r[i].y);
+        rx = std::max(rx, (double)r[i].z);
+        by = std::max(by, (double)r[i].w);
+    }
+    const int pad = (int)((double)(rx - lx) * 0.01);
+    return make_int4((int)lx - pad, (int)ty - pad, (int)rx + pad, (int)by + pad);
+}
+
+// ---------------------------------------------------------------- reader
+// Same line rules as randomWalk/CPU/Parser.cpp:15-50,89-125: lines are read
+// into a 50-byte buffer (a longer line ends the read), the first line holding
+// "<<" is skipped and each later one closes a layer, a line holding "rect"
+// gives a rect from the four numbers after its first token.
+namespace {
+
+constexpr int kMaxLine = 50;
+
+bool is_space(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\n' || c == '\v' || c == '\f'; }
+
+int parse_rect(const char* s, int4& out)
+{
+    while (is_space(*s)) ++s;
+    while (*s && !is_space(*s)) ++s;  // header token
+    double c[4];
+    int k = 0;
+    for (;;) {
+        while (is_space(*s)) ++s;
+        if (!*s) break;
+        if (!((*s >= '0' && *s <= '9') || *s == '-' || *s == '+' || *s == '.')) break;
+        char* end = nullptr;
+        const double v = std::strtod(s, &end);
+        if (end == s) break;
+        if (k >= 4) return FRW_E_PARSE;
+        c[k++] = v;
+        s = end;
+    }
+    if (k != 4) return FRW_E_PARSE;
+    for (int i = 0; i < 4; ++i)
+        if (c[i] != std::floor(c[i]) || std::fabs(c[i]) > 1.0e9) return FRW_E_PARSE;
+    out = make_int4((int)c[0], (int)c[1], (int)c[2], (int)c[3]);
+    return FRW_OK;
+}
+
+}  // namespace
+
+int read_layout(const char* path, int layer, std::vector<int4>* rects, int* n_layers)
+{
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return set_error(FRW_E_IO, std::string("cannot open layout file ") + path);
+    char line[kMaxLine];
+    bool first = true;
+    int cur = 0, err = FRW_OK, line_no = 0;
+    std::vector<int4> acc;
+    for (;;) {
+        int len = 0, ch = 0;
+        bool got = false, fail = false;
+        for (;;) {
+            ch = std::fgetc(f);
+            if (ch == EOF) break;
+            got = true;
+            if (ch == '\n') break;
+            if (len == kMaxLine - 1) { fail = true; break; }
+            line[len++] = (char)ch;
+        }
+        line[len] = 0;
+        if (fail || (!got && ch == EOF)) break;
+        ++line_no;
+        if (std::strstr(line, "<<")) {
+            if (first) { first = false; continue; }
+            if (cur == layer && rects) *rects = acc;
+            acc.clear();
+            ++cur;
+            continue;
+        }
+        if (std::strstr(line, "rect")) {
+            int4 r;
+            if (parse_rect(line, r) != FRW_OK) {
+                err = set_error(FRW_E_PARSE, "malformed rect line " + std::to_string(line_no) +
+                                                 " in " + path);
+                break;
+            }
+            if (cur == layer) acc.push_back(r);
+        }
+        if (ch == EOF) break;
+    }
+    std::fclose(f);
+    if (err != FRW_OK) return err;
+    if (n_layers) *n_layers = cur;
+    if (layer < 0 || layer >= cur)
+        return set_error(FRW_E_LAYER, "layer " + std::to_string(layer) + " not in " + path);
+    return FRW_OK;
+}
+
+int validate_rects(const std::vector<int4>& r)
+{
+    for (size_t i = 0; i < r.size(); ++i) {
+        if (r[i].x > r[i].z || r[i].y > r[i].w)
+            return set_error(FRW_E_PARSE, "rect " + std::to_string(i) + " has x0 > x1 or y0 > y1");
+        const int lim = 1 << 28;
+        if (std::abs(r[i].x) > lim || std::abs(r[i].y) > lim || std::abs(r[i].z) > lim || std::abs(r[i].w) > lim)
+            return set_error(FRW_E_RANGE, "rect " + std::to_string(i) + " has a coordinate beyond 2^28");
+    }
+    return FRW_OK;
+}
+
+// ---------------------------------------------------------------- Green CDF
+// Harmonic measure on a side of the unit square from its centre:
+//   k(s) = sum_{n odd <= 19} sin(n pi / 2) sin(n pi s) / cosh(n pi / 2)
+// at 50 midpoint panels per side, four sides clockwise, cumulated and
+// normalised; thresholds T[i] = round-half-even(intg[i] * 2^32).
+void green_thresholds(uint32_t T[kNSample])
+{
+    const double pi = 3.14159265358979323846;
+    const int per_side = kNSample / 4;
+    double w[kNSample / 4];
+    for (int j = 0; j < per_side; ++j) {
+        const double s = (j + 0.5) / per_side;
+        double k = 0.0;
+        for (int n = 1; n <= 19; n += 2) {
+            const double sgn = ((n & 3) == 1) ? 1.0 : -1.0;
+            k += sgn * std::sin(n * pi * s) / std::cosh(n * pi / 2.0);
+        }
+        w[j] = k / per_side;
+    }
+    double intg[kNSample + 1];
+    intg[0] = 0.0;
+    for (int i = 0; i < kNSample; ++i) intg[i + 1] = intg[i] + w[i % per_side];
+    const double tot = intg[kNSample];
+    for (int i = 0; i < kNSample; ++i) T[i] = (uint32_t)std::nearbyint((intg[i] / tot) * 4294967296.0);
+}
+
+// ---------------------------------------------------------------- generator
+namespace {
+
+constexpr uint32_t kGenDomain = 0x47454E52u;  // 'GENR'
+
+void philox_host(uint32_t c[4], uint64_t seed)
+{
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// floor(u * range / 2^32): unbiased enough for layout synthesis, exact and
+// identical on every host.
+inline int64_t scale_u32(uint32_t u, int64_t range) { return (int64_t)(((unsigned __int128)u * (uint64_t)range) >> 32); }
+
+struct Grid {
+    int64_t cell;
+    std::unordered_map<uint64_t, std::vector<int32_t>> m;
+    static uint64_t key(int64_t cx, int64_t cy) { return ((uint64_t)(uint32_t)cx << 32) | (uint32_t)cy; }
+    bool hits(const std::vector<int4>& rs, const int4& r) const
+    {
+        for (int64_t cx = r.x / cell; cx <= r.z / cell; ++cx)
+            for (int64_t cy = r.y / cell; cy <= r.w / cell; ++cy) {
+                auto it = m.find(key(cx, cy));
+                if (it == m.end()) continue;
+                for (int32_t j : it->second) {
+                    const int4& q = rs[j];
+                    if (q.z >= r.x && r.z >= q.x && q.w >= r.y && r.w >= q.y) return true;
+                }
+            }
+        return false;
+    }
+    void add(const int4& r, int32_t id)
+    {
+        for (int64_t cx = r.x / cell; cx <= r.z / cell; ++cx)
+            for (int64_t cy = r.y / cell; cy <= r.w / cell; ++cy) m[key(cx, cy)].push_back(id);
+    }
+};
+
+}  // namespace
+
+int generate_layout(const frw_gen_params* p, std::vector<int4>* out)
+{
+    if (p->n < 0 || p->box <= 0 || p->side_min < 1 || p->side_max < p->side_min || p->side_max >= p->box)
+        return set_error(FRW_E_INVALID, "bad generator parameters");
+    if (p->kind == FRW_GEN_CLUSTERED && (p->n_clusters <= 0 || p->sigma <= 0))
+        return set_error(FRW_E_INVALID, "clustered layout needs n_clusters > 0 and sigma > 0");
+    std::vector<int4>& rs = *out;
+    rs.clear();
+    rs.reserve((size_t)p->n);
+    Grid g;
+    g.cell = std::max<int64_t>(p->side_max + 1, 1);
+    std::vector<double> cx, cy;
+    if (p->kind == FRW_GEN_CLUSTERED) {
+        for (int c = 0; c < p->n_clusters; ++c) {
+            uint32_t w[4] = {(uint32_t)c, 0u, 2u, kGenDomain};
+            philox_host(w, p->seed);
+            cx.push_back((double)scale_u32(w[0], (int64_t)p->box + 1));
+            cy.push_back((double)scale_u32(w[1], (int64_t)p->box + 1));
+        }
+    }
+    const double frac = p->kind == FRW_GEN_CLUSTERED ? p->cluster_frac : 0.0;
+    const int64_t span = (int64_t)p->side_max - p->side_min + 1;
+    const uint64_t max_attempts = (uint64_t)std::max<int64_t>(p->n, 1) * 1000ull;
+    uint64_t attempt = 0;
+    for (int64_t j = 0; j < p->n; ++j) {
+        const bool clustered = std::floor((j + 1) * frac) > std::floor(j * frac);
+        for (;;) {
+            if (attempt >= max_attempts)
+                return set_error(FRW_E_RANGE, "layout too dense: rejection sampling did not finish");
+            uint32_t w[4] = {(uint32_t)attempt, (uint32_t)(attempt >> 32), 0u, kGenDomain};
+            philox_host(w, p->seed);
+            const int64_t wd = p->side_min + scale_u32(w[0], span);
+            const int64_t ht = p->side_min + scale_u32(w[1], span);
+            int64_t x0, y0;
+            if (clustered) {
+                uint32_t z[4] = {(uint32_t)attempt, (uint32_t)(attempt >> 32), 1u, kGenDomain};
+                philox_host(z, p->seed);
+                const int c = (int)(w[2] % (uint32_t)p->n_clusters);
+                // Box-Muller on two uniforms in (0, 1]
+                const double u1 = ((double)z[0] + 1.0) / 4294967296.0;
+                const double u2 = ((double)z[1]) / 4294967296.0;
+                const double rad = std::sqrt(-2.0 * std::log(u1)) * p->sigma;
+                const double th = 2.0 * 3.14159265358979323846 * u2;
+                const double gx = cx[c] + rad * std::cos(th) - wd / 2.0;
+                const double gy = cy[c] + rad * std::sin(th) - ht / 2.0;
+                x0 = std::min<int64_t>(std::max<int64_t>((int64_t)std::llround(gx), 0), p->box - wd);
+                y0 = std::min<int64_t>(std::max<int64_t>((int64_t)std::llround(gy), 0), p->box - ht);
+            } else {
+                x0 = scale_u32(w[2], (int64_t)p->box - wd + 1);
+                y0 = scale_u32(w[3], (int64_t)p->box - ht + 1);
+            }
+            ++attempt;
+            const int4 r = make_int4((int)x0, (int)y0, (int)(x0 + wd), (int)(y0 + ht));
+            if (g.hits(rs, r)) continue;
+            g.add(r, (int32_t)rs.size());
+            rs.push_back(r);
+            break;
+        }
+    }
+    return FRW_OK;
+}
+
+int save_layout(const std::vector<int4>& rs, const char* path)
+{
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return set_error(FRW_E_IO, std::string("cannot write ") + path);
+    std::fprintf(f, "magic\ntech frw\ntimestamp 0\n<< metal3 >>\n");
+    for (const int4& r : rs) std::fprintf(f, "rect %d %d %d %d\n", r.x, r.y, r.z, r.w);
+    std::fprintf(f, "<< metal3 >>\n");
+    std::fclose(f);
+    return FRW_OK;
+}
+
+// Result file in the reference format (randomWalk/CPU/getAvgPathLen.cpp:13-33):
+// one "x0 y0 x1 y1: P%" line per rect that was hit, and the escaped share.
+// Percentages are float(count)/float(N)*100 printed like std::ostream (%g).
+int write_results(const std::vector<int4>& rs, const int64_t* counts, int64_t n_walks,
+                  const char* path)
+{
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return set_error(FRW_E_IO, std::string("cannot write ") + path);
+    std::fprintf(f, "[GPU]\n");
+    const int64_t n = (int64_t)rs.size();
+    if (counts[n] > 0)
+        std::fprintf(f, "Ile scieżek wyszło poza granice warstwy: %g%%\n",
+                     (double)((float)counts[n] / (float)n_walks * 100));
+    for (int64_t i = 0; i < n; ++i) {
+        if (counts[i] == 0) continue;
+        std::fprintf(f, "%g %g %g %g: %g%%\n", (double)rs[i].x, (double)rs[i].y, (double)rs[i].z,
+                     (double)rs[i].w, (double)(((float)counts[i] / (float)n_walks) * 100));
+    }
+    std::fclose(f);
+    return FRW_OK;
+}
+
+}  // namespace frw

@@ -1,0 +1,210 @@
+/*
+ * frw.h — C ABI of the MI355X (gfx950) floating-random-walk library.
+ *
+ * Drop-in boundary for the hot path of piogrzej/engineer-thesis: the layout
+ * reader, the quadtree build, the three tree queries and the random-walk
+ * driver that the reference exposes through randomWalk/randomwalk.h.  Each
+ * entry point names the reference interface it replaces (file:line, relative
+ * to the reference repository).
+ *
+ * Conventions
+ *   - Every function returns an int status (FRW_OK = 0, negative on error);
+ *     frw_last_error_string() describes the last error of the calling thread.
+ *     Nothing calls exit() or throws across this boundary.
+ *   - Rects are int32 {x0, y0, x1, y1}: top-left (x0,y0), bottom-right
+ *     (x1,y1), y grows downward, closed intervals (touching = collision).
+ *   - Pointers documented as "device" are HIP device pointers; functions
+ *     taking a `stream` (a hipStream_t, NULL = default stream) are
+ *     asynchronous on it.  Host pointers are read or written before return.
+ *   - Handles are owned by the caller and may be used from one host thread
+ *     at a time.
+ */
+#ifndef FRW_H
+#define FRW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FRW_OK = 0,
+    FRW_E_INVALID = -1, /* bad argument (null handle, negative size, ...) */
+    FRW_E_IO = -2,      /* layout file cannot be opened */
+    FRW_E_PARSE = -3,   /* malformed "rect" line, or non-integer coordinate */
+    FRW_E_LAYER = -4,   /* no such layer */
+    FRW_E_RANGE = -5,   /* master id / walk count / coordinates out of range */
+    FRW_E_DEVICE = -6,  /* HIP runtime error */
+    FRW_E_NOMEM = -7,   /* allocation failed */
+    FRW_E_KERNEL = -8   /* a kernel raised an internal error flag */
+};
+
+enum { FRW_MODE_EXACT = 0, FRW_MODE_BISECT = 1, FRW_MODE_LEARNED = 2 };
+enum { FRW_GEN_UNIFORM = 0, FRW_GEN_CLUSTERED = 1 };
+enum { FRW_NSAMPLE = 200 };
+
+typedef struct frw_layout frw_layout;
+typedef struct frw_tree frw_tree;
+
+const char* frw_last_error_string(void);
+const char* frw_version(void);
+
+/* ---------------------------------------------------------------- layouts */
+
+/* Parser("<<").parse(path) + getLayerAt(layer) + getLayerSize(layer)
+ * (randomWalk/CPU/Parser.h:13-30, randomWalk/CPU/Parser.cpp:15-125).  Same
+ * file format and line rules; errors are status codes instead of exit(0). */
+int frw_layout_load(const char* path, int32_t layer, frw_layout** out);
+/* Parser::getLayerCount (randomWalk/CPU/Parser.h:22). */
+int frw_layout_layer_count(const char* path, int32_t* n_layers);
+/* A layout from n rects (4 int32 each, host memory). */
+int frw_layout_from_rects(const int32_t* xyxy, int64_t n, frw_layout** out);
+
+typedef struct {
+    int32_t kind;         /* FRW_GEN_UNIFORM or FRW_GEN_CLUSTERED */
+    int32_t box;          /* coordinates in [0, box] */
+    int64_t n;            /* rect count */
+    int32_t side_min;     /* sides ~ U{side_min..side_max} */
+    int32_t side_max;
+    double cluster_frac;  /* fraction of rects drawn around Gaussian blobs */
+    double sigma;         /* blob standard deviation */
+    int32_t n_clusters;   /* blob centres, uniform in the box */
+    int32_t reserved;
+    uint64_t seed;        /* Philox key */
+} frw_gen_params;
+/* Non-overlapping, non-touching random layout (rejection sampling, the
+ * semantics of randomWalk/utils/TestGenerator.cpp:47-110 with Philox draws). */
+int frw_layout_generate(const frw_gen_params* p, frw_layout** out);
+/* Writes the reference's layout format (TestGenerator.cpp:65,103-106). */
+int frw_layout_save(const frw_layout* l, const char* path);
+int64_t frw_layout_size(const frw_layout* l);
+int frw_layout_rects(const frw_layout* l, int32_t* xyxy_out);
+/* Padded absorbing bounds = Parser::getLayerSize (Parser.cpp:51-87). */
+int frw_layout_bounds(const frw_layout* l, int32_t out[4]);
+void frw_layout_free(frw_layout* l);
+
+/* Panel thresholds of the Green CDF: T[i] = round(intg[i] * 2^32), the table
+ * precompute_unit_square_green feeds getIndex (getAvgPathLen.cpp:83-84,
+ * mainFunctions.cpp:40-46). */
+int frw_green_thresholds(uint32_t out[FRW_NSAMPLE]);
+
+/* ---------------------------------------------------------------- tree */
+
+typedef struct {
+    int32_t leaf_cap;     /* max rects per leaf (0 = default 16, <= 64) */
+    int32_t reserved[7];
+} frw_tree_params;
+
+typedef struct {
+    int64_t n_rects;
+    int64_t n_nodes;      /* including dummy quadrants */
+    int32_t depth;        /* levels */
+    int32_t leaf_cap;
+    int32_t bounds[4];
+    int32_t morton_bits;  /* K: the root cell is 2^K wide */
+    int32_t reserved;
+    double build_ms;      /* device time of the last build */
+} frw_tree_info;
+
+/* createQuadTree(layer, spaceSize, RECT_ID, doCheck)
+ * (randomWalk/Cuda/mainkernels.h:10, randomWalk/Cuda/createquadtree.cu:16-87):
+ * a level-synchronous Morton sort-and-split build on the device. */
+int frw_tree_build(const frw_layout* l, const frw_tree_params* p, void* stream,
+                   frw_tree** out);
+int frw_tree_info_get(const frw_tree* t, frw_tree_info* info);
+/* Copies the flat tree to host memory: nodes (8 int32 each: x0 y0 x1 y1
+ * first_child rbeg rend level), rects in tree order (4 int32 each) and their
+ * original ids.  Any pointer may be NULL.  (The structural check of
+ * checkQuadTree, randomWalk/Cuda/createquadtree.cu:373-425, runs on this.) */
+int frw_tree_export(const frw_tree* t, int32_t* nodes, int32_t* rects, int32_t* ids);
+/* freeQuadTreeManager (randomWalk/Cuda/freequadtreemanager.cu:3-11). */
+void frw_tree_free(frw_tree* t);
+
+/* ---------------------------------------------------------------- queries
+ * Batched parity hooks; inputs/outputs are device arrays of n items. */
+
+/* Q1 Tree::checkCollisions(rect) (randomWalk/CPU/quadTree.cpp:148-198,
+ * randomWalk/Cuda/d_quadTree.cu:136-193): boxes = n x {x0,y0,x1,y1} FP64;
+ * out[i] = 1 if box i leaves the bounds or touches any rect. */
+int frw_query_collide(const frw_tree* t, const double* boxes, int64_t n, uint8_t* out,
+                      void* stream);
+/* Q2 Tree::checkCollisons(point, rect) (quadTree.cpp:240-262,
+ * d_quadTree.cu:28-84): original id of the rect containing each point, or -1. */
+int frw_query_point(const frw_tree* t, const double* pts, int64_t n, int32_t* rect_id,
+                    void* stream);
+/* Signed L-infinity distance from each point to the nearest rect (<= 0 when
+ * the point is inside one): the quantity the biggest-square search bisects. */
+int frw_query_nearest(const frw_tree* t, const double* pts, int64_t n, double* dist,
+                      void* stream);
+/* Q3 Tree::drawBiggestSquareAtPoint (quadTree.cpp:264-312,
+ * d_quadTree.cu:86-134): boxes_out = n x {x0,y0,x1,y1}.  mode
+ * FRW_MODE_EXACT replays the bisection from one nearest-rect search,
+ * FRW_MODE_BISECT runs it literally (one Q1 per step). */
+int frw_query_maxsquare(const frw_tree* t, const double* pts, int64_t n, int32_t mode,
+                        double* boxes_out, void* stream);
+/* Tree::creatGaussianSurfFrom(rect(master), 1.5) (quadTree.cpp:352-364);
+ * out is host memory. */
+int frw_gauss_surface(const frw_tree* t, int32_t master, double out[4]);
+
+/* ---------------------------------------------------------------- walks */
+
+typedef struct {
+    int32_t master;      /* RECT_ID */
+    int32_t mode;        /* FRW_MODE_* */
+    int64_t n_walks;     /* ITER_NUM */
+    int64_t first_walk;  /* id of the first walk (Philox counter) */
+    uint64_t seed;       /* Philox key */
+    uint32_t job;        /* Philox counter word 1 (layout/master id in a batch) */
+    int32_t block;       /* threads per block, 0 = tuned default */
+    int32_t grid;        /* blocks, 0 = fill the device */
+    int32_t reserved;
+    int64_t max_hops;    /* per-walk hop cap (0 = 10^7); capped walks count as stuck */
+} frw_walk_params;
+
+typedef struct {
+    int64_t* counts;    /* device, n_rects + 2: hits per original rect id, then
+                           escaped, then stuck; accumulated (caller zeroes) */
+    int64_t* hops;      /* device, 1 element, accumulated */
+    int32_t* hit_out;   /* device, n_walks or NULL: rect id, -1 escaped, -2 stuck */
+    int32_t* hops_out;  /* device, n_walks or NULL */
+} frw_walk_outputs;
+
+/* randomWalkCudaWrapper(n, tree, output, rectOutput, seed)
+ * (randomWalk/Cuda/mainkernels.h:9, randomWalk/Cuda/randomwalk.cu:64-134):
+ * n_walks walks from the master's Gaussian surface, one persistent kernel. */
+int frw_walk_async(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o,
+                   void* stream);
+
+typedef struct {
+    int64_t n_walks;
+    int64_t hops;
+    int64_t n_master;       /* walks that ended on the master */
+    int64_t n_escaped;
+    int64_t n_stuck;
+    double fraction_other;  /* getAvgPathLen's return value: #(hit != master)/N */
+    double walk_ms;         /* device time of the walk kernel */
+} frw_walk_summary;
+
+/* getAvgPathLenCUDA's walk phase (randomWalk/Cuda/getAvgPathLenCUDA.cu:50-80)
+ * as one synchronous call: counts_host (n_rects + 2, may be NULL) and the
+ * summary are written on return. */
+int frw_walk(const frw_tree* t, const frw_walk_params* p, int64_t* counts_host,
+             frw_walk_summary* summary);
+
+/* Measurement hook (no reference counterpart): the same walks as frw_walk
+ * with per-hop traffic counters compiled in.  out = {node records read
+ * (32 B each), rect records read (16 B each), hops}.  Slower than frw_walk;
+ * used to state the algorithmic bytes per hop. */
+int frw_walk_traffic(const frw_tree* t, const frw_walk_params* p, int64_t out[3]);
+
+/* Writes <path> in the reference result format (getAvgPathLen.cpp:13-33):
+ * "x0 y0 x1 y1: P%" per hit rect and the escaped percentage. */
+int frw_write_results(const frw_layout* l, const int64_t* counts_host, int64_t n_walks,
+                      const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FRW_H */

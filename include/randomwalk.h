@@ -1,0 +1,80 @@
+// randomwalk.h — C++ drop-in for the entry points that the reference exports
+// through randomWalk/randomwalk.h (randomWalk/CPU/mainFunctions.h:15-18,
+// randomWalk/CPU/getAvgPathLen.h:5, randomWalk/CPU/Parser.h:13-30,
+// randomWalk/Cuda/mainkernels.h:9-15).  Same names, argument meanings and
+// result scalar; implemented on the gfx950 library declared in frw.h.
+//
+// Differences a caller can observe (kept on purpose):
+//   - errors that the reference answers with exit(0) print a message to
+//     stderr and make the function return (runRandomWalk) or return -1.0;
+//   - both engines run on the GPU: GPU_FLAG=true runs the exact-search walk,
+//     GPU_FLAG=false runs the reference's literal bisection (the algorithm of
+//     the reference CPU path) through the same device engine;
+//   - walk streams are Philox4x32-10 keyed by FRW_SEED (default 1), so runs
+//     are reproducible; the reference re-seeds from /dev/urandom.
+#ifndef FRW_RANDOMWALK_H
+#define FRW_RANDOMWALK_H
+
+#include <string>
+#include <vector>
+
+typedef double floatingPoint;
+
+struct point {
+    floatingPoint x, y;
+    point() : x(0), y(0) {}
+    point(floatingPoint px, floatingPoint py) : x(px), y(py) {}
+};
+
+struct RectHost {
+    point topLeft, bottomRight;
+    RectHost() : topLeft(-1, -1), bottomRight(-1, -1) {}  // "no rect" sentinel
+    RectHost(point tl, point br) : topLeft(tl), bottomRight(br) {}
+    bool operator==(const RectHost& o) const
+    {
+        return topLeft.x == o.topLeft.x && topLeft.y == o.topLeft.y &&
+               bottomRight.x == o.bottomRight.x && bottomRight.y == o.bottomRight.y;
+    }
+    bool operator!=(const RectHost& o) const { return !(*this == o); }
+};
+
+typedef std::vector<RectHost> Layer;
+typedef std::vector<Layer> Layers;
+
+// randomWalk/CPU/Parser.h:13-30
+class Parser {
+public:
+    explicit Parser(const char* delimiter);
+    void parse(std::string const& fileName);
+    RectHost getLayerSize(int layerIt);
+    Layer getLayerAt(int i) { return layers.at(i); }
+    Layers getAllLayers() { return layers; }
+    int getLayerCount() { return (int)layers.size(); }
+
+private:
+    Layers layers;
+    std::vector<RectHost> bounds;
+    std::string delimiter;
+};
+
+// Opaque device tree + master (randomWalk/Cuda/d_quadtree.h:36-47).
+struct QuadTreeManager;
+
+// randomWalk/CPU/mainFunctions.h:18
+void runRandomWalk(char* path, int ITER_NUM, int RECT_ID, bool GPU_FLAG, bool measure, int layer);
+// randomWalk/CPU/getAvgPathLen.h:5
+floatingPoint getAvgPathLen(char* path, int ITER_NUM, int RECT_ID, bool measure, int layerID);
+// randomWalk/Cuda/mainkernels.h:9-15
+floatingPoint getAvgPathLenCUDA(char* path, int ITER_NUM, int RECT_ID, bool measure, int layerID);
+QuadTreeManager* createQuadTree(const std::vector<RectHost>& layer, RectHost const& spaceSize,
+                                int RECT_ID, bool doCheck);
+QuadTreeManager* randomWalkCudaInit(char* path, bool measure, int RECT_ID, int layerID);
+// output[i] = 1 when walk i did not end on the master; rectOutput[i] = the
+// rect it ended on ((-1,-1,-1,-1) when it escaped).  Both are device arrays.
+void randomWalkCudaWrapper(int n, QuadTreeManager* quadTree, unsigned int* output,
+                           RectHost* rectOutput, unsigned long long randomSeed);
+bool initCuda(int argc, char** argv);
+void freeQuadTreeManager(QuadTreeManager* qtm);
+floatingPoint countAvg(unsigned int output[], int ITER_NUM);
+
+#endif  // FRW_RANDOMWALK_H
