@@ -39,6 +39,10 @@ DESCR = {
 }
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,9 +104,9 @@ def main():
     args = parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py runs on one GPU only: the walk shards by walk id across GPUs "
-              "(see DESIGN.md), but this build measures the single-GPU kernel; "
-              f"--gpus {args.gpus} is not supported.", file=sys.stderr)
+        print("bench.py measures one GPU only (walks would shard by walk id across GPUs, "
+              f"but no multi-GPU path is built); --gpus {args.gpus} is not supported.",
+              file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
@@ -122,6 +126,7 @@ def main():
     tree = frw.Tree(lay, leaf_cap=args.leaf_cap)
     info = tree.info()
     nr = info.n_rects
+    log(f"layout {n} rects, tree {info.n_nodes} nodes depth {info.depth}, build {info.build_ms:.2f} ms")
 
     stream = torch.cuda.Stream()
     sh = stream.cuda_stream
@@ -135,6 +140,7 @@ def main():
     for w in range(args.warmup):
         launch(w)
     torch.cuda.synchronize()
+    log("warmup done")
     counts.zero_()
     hops.zero_()
     torch.cuda.synchronize()
@@ -149,6 +155,7 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     launch_ms = [a.elapsed_time(b) for a, b in ev]
+    log(f"timed {args.steps} steps in {wall:.3f} s")
     total_hops = int(hops.item())
     c = counts.cpu().numpy()
     n_walks = walks * args.steps
@@ -164,6 +171,7 @@ def main():
     tr_nodes, tr_rects, tr_hops = tree.walk_traffic(master, sub, seed=1, mode=mode,
                                                     first_walk=args.warmup * walks)
     bytes_per_hop = (32.0 * tr_nodes + 16.0 * tr_rects) / max(tr_hops, 1)
+    log(f"traffic run: {tr_hops} hops, {bytes_per_hop:.1f} B/hop")
     avg_ms = sum(launch_ms) / len(launch_ms)
     bytes_per_launch = bytes_per_hop * total_hops / args.steps
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
@@ -212,6 +220,7 @@ def main():
         },
     }
     if not args.no_cpu:
+        log("cpu baseline ...")
         cb = cpu_baseline(lay.rects(), lay.bounds(), master, args.cpu_seconds)
         cb["time_to_1pct_s"] = n1 / cb.pop("walks_per_s")
         out["cpu_baseline"] = cb

@@ -52,6 +52,8 @@ int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 bounds, int lea
 
 DevTree dev_view(const frw_tree* t);
 int stack_bytes_per_lane(const frw_tree* t);
+int lds_per_block(const frw_tree* t);
+int block_for_stack(const frw_tree* t, int want, size_t extra_lds);
 
 // query launchers (frw_query.hip)
 int launch_q1(const frw_tree* t, const double* boxes, int64_t n, uint8_t* out, hipStream_t s);

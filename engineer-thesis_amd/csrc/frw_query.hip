@@ -69,7 +69,43 @@ __global__ void __launch_bounds__(64) gauss_kernel(DevTree t, int4 master, doubl
     out[3] = s.y1;
 }
 
-inline int grid_for(int64_t n) { return (int)((n + kQueryBlock - 1) / kQueryBlock); }
+}  // namespace
+
+// LDS budget of one block (bytes); the traversal stacks must fit in it.
+int lds_per_block(const frw_tree* t)
+{
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, t->device) != hipSuccess || v <= 0)
+        v = 64 * 1024;
+    return v;
+}
+
+// Largest block (<= want, multiple of 64) whose traversal stacks fit in LDS;
+// 0 if not even one wave fits.
+int block_for_stack(const frw_tree* t, int want, size_t extra_lds)
+{
+    const size_t budget = (size_t)lds_per_block(t);
+    int b = want;
+    while (b >= 64 && (size_t)b * stack_bytes_per_lane(t) + extra_lds > budget) b -= 64;
+    return b >= 64 ? b : 0;
+}
+
+namespace {
+
+struct QueryLaunch {
+    int block;
+    int grid;
+    size_t lds;
+};
+
+int plan_query(const frw_tree* t, int64_t n, QueryLaunch* q)
+{
+    q->block = block_for_stack(t, kQueryBlock, 0);
+    if (q->block == 0) return set_error(FRW_E_RANGE, "tree too deep for the LDS traversal stack");
+    q->grid = (int)((n + q->block - 1) / q->block);
+    q->lds = (size_t)q->block * stack_bytes_per_lane(t);
+    return FRW_OK;
+}
 
 }  // namespace
 
@@ -94,36 +130,40 @@ DevTree dev_view(const frw_tree* t)
 int launch_q1(const frw_tree* t, const double* boxes, int64_t n, uint8_t* out, hipStream_t s)
 {
     if (n == 0) return FRW_OK;
-    const size_t lds = (size_t)kQueryBlock * stack_bytes_per_lane(t);
-    q1_kernel<<<grid_for(n), kQueryBlock, lds, s>>>(dev_view(t), (const double4*)boxes, n, out);
+    QueryLaunch ql;
+    if (plan_query(t, n, &ql) != FRW_OK) return FRW_E_RANGE;
+    q1_kernel<<<ql.grid, ql.block, ql.lds, s>>>(dev_view(t), (const double4*)boxes, n, out);
     return check_hip(hipGetLastError(), "q1_kernel");
 }
 
 int launch_q2(const frw_tree* t, const double* pts, int64_t n, int32_t* ids, hipStream_t s)
 {
     if (n == 0) return FRW_OK;
-    const size_t lds = (size_t)kQueryBlock * stack_bytes_per_lane(t);
-    q2_kernel<<<grid_for(n), kQueryBlock, lds, s>>>(dev_view(t), (const double2*)pts, n, ids);
+    QueryLaunch ql;
+    if (plan_query(t, n, &ql) != FRW_OK) return FRW_E_RANGE;
+    q2_kernel<<<ql.grid, ql.block, ql.lds, s>>>(dev_view(t), (const double2*)pts, n, ids);
     return check_hip(hipGetLastError(), "q2_kernel");
 }
 
 int launch_nearest(const frw_tree* t, const double* pts, int64_t n, double* d, hipStream_t s)
 {
     if (n == 0) return FRW_OK;
-    const size_t lds = (size_t)kQueryBlock * stack_bytes_per_lane(t);
-    nearest_kernel<<<grid_for(n), kQueryBlock, lds, s>>>(dev_view(t), (const double2*)pts, n, d);
+    QueryLaunch ql;
+    if (plan_query(t, n, &ql) != FRW_OK) return FRW_E_RANGE;
+    nearest_kernel<<<ql.grid, ql.block, ql.lds, s>>>(dev_view(t), (const double2*)pts, n, d);
     return check_hip(hipGetLastError(), "nearest_kernel");
 }
 
 int launch_q3(const frw_tree* t, const double* pts, int64_t n, int mode, double* out, hipStream_t s)
 {
     if (n == 0) return FRW_OK;
-    const size_t lds = (size_t)kQueryBlock * stack_bytes_per_lane(t);
+    QueryLaunch ql;
+    if (plan_query(t, n, &ql) != FRW_OK) return FRW_E_RANGE;
     if (mode == kModeBisect)
-        q3_kernel<true><<<grid_for(n), kQueryBlock, lds, s>>>(dev_view(t), (const double2*)pts, n,
+        q3_kernel<true><<<ql.grid, ql.block, ql.lds, s>>>(dev_view(t), (const double2*)pts, n,
                                                                (double4*)out, t->d_err);
     else
-        q3_kernel<false><<<grid_for(n), kQueryBlock, lds, s>>>(dev_view(t), (const double2*)pts, n,
+        q3_kernel<false><<<ql.grid, ql.block, ql.lds, s>>>(dev_view(t), (const double2*)pts, n,
                                                                 (double4*)out, t->d_err);
     return check_hip(hipGetLastError(), "q3_kernel");
 }

@@ -152,17 +152,20 @@ int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outp
     zero_counter<<<1, 1, 0, s>>>(t->d_work);
     if ((rc = check_hip(hipGetLastError(), "zero_counter")) != FRW_OK) return rc;
 
-    int block = p->block > 0 ? p->block : 256;
-    if (block % kWaveSize != 0 || block > 256)
+    int block = p->block;
+    if (block < 0 || block % kWaveSize != 0 || block > 256)
         return set_error(FRW_E_INVALID, "block must be a multiple of 64 and <= 256");
-    const size_t lds = (size_t)block * stack_bytes_per_lane(t);
-    if (lds + kNSample * 4 > 160 * 1024)
+    if (block == 0) block = block_for_stack(t, 256, kNSample * 4);
+    if (block == 0 || (size_t)block * stack_bytes_per_lane(t) + kNSample * 4 > (size_t)lds_per_block(t))
         return set_error(FRW_E_RANGE, "tree too deep for the LDS traversal stack at this block size");
+    const size_t lds = (size_t)block * stack_bytes_per_lane(t);
     int grid = p->grid;
     if (grid <= 0) {
-        int cus = 0;
+        int cus = 0, lds_cu = 0;
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, t->device);
-        const int per_cu_lds = (int)((160 * 1024) / (lds + kNSample * 4));
+        hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, t->device);
+        if (lds_cu <= 0) lds_cu = 64 * 1024;
+        const int per_cu_lds = (int)((size_t)lds_cu / (lds + kNSample * 4));
         const int per_cu = std::max(1, std::min(per_cu_lds, 2048 / block));
         grid = std::max(1, cus * per_cu);
         const int64_t need = (p->n_walks + block - 1) / block;
