@@ -43,7 +43,7 @@ constexpr uint32_t kWalkDomain = 0x57414C4Bu;  // 'WALK'
 // ---------------------------------------------------------------- sampling
 // Panel index = largest i with T[i] <= u (T[0] = 0): an 8-step branch-free
 // binary search over the 200 thresholds of the Green CDF.
-__device__ inline int sample_panel(const uint32_t* T, uint32_t u)
+__host__ __device__ inline int sample_panel(const uint32_t* T, uint32_t u)
 {
     int lo = 0;
 #pragma unroll
@@ -97,26 +97,26 @@ struct DBox {
 };
 
 // Tree::isInBounds(point), strict (randomWalk/CPU/quadTree.cpp:49-59).
-__device__ inline bool in_bounds_strict(const int4& B, double px, double py)
+__host__ __device__ inline bool in_bounds_strict(const int4& B, double px, double py)
 {
     return px > (double)B.x && py > (double)B.y && px < (double)B.z && py < (double)B.w;
 }
 
 // Tree::isInBounds(rect), inclusive (randomWalk/CPU/quadTree.cpp:38-47).
-__device__ inline bool in_bounds_box(const int4& B, double x0, double y0, double x1, double y1)
+__host__ __device__ inline bool in_bounds_box(const int4& B, double x0, double y0, double x1, double y1)
 {
     return y0 >= (double)B.y && x0 >= (double)B.x && y1 <= (double)B.w && x1 <= (double)B.z;
 }
 
 // rectsCollision (randomWalk/CPU/rectHost.cpp:43-52): closed, touching collides.
-__device__ inline bool box_hits_rect(const int4& r, double x0, double y0, double x1, double y1)
+__host__ __device__ inline bool box_hits_rect(const int4& r, double x0, double y0, double x1, double y1)
 {
     return (double)r.z >= x0 && x1 >= (double)r.x && (double)r.w >= y0 && y1 >= (double)r.y;
 }
 
 // Signed L-infinity distance from p to a rect or node box; <= 0 exactly when
 // the closed box contains p.
-__device__ inline double linf(const int4& r, double px, double py)
+__host__ __device__ inline double linf(const int4& r, double px, double py)
 {
     const double dx = fmax((double)r.x - px, px - (double)r.z);
     const double dy = fmax((double)r.y - py, py - (double)r.w);
@@ -125,9 +125,20 @@ __device__ inline double linf(const int4& r, double px, double py)
 
 // Rect equality by value (the reference compares RectHost by value when it
 // ignores the master, randomWalk/CPU/quadTree.cpp:219-226).
-__device__ inline bool same_rect(const int4& a, const int4& b)
+__host__ __device__ inline bool same_rect(const int4& a, const int4& b)
 {
     return a.x == b.x && a.y == b.y && a.z == b.z && a.w == b.w;
+}
+
+// Error bits: an atomic OR on the device; a plain OR in a host build of these
+// functions (used by the CPU-side tests of this header).
+__host__ __device__ inline void raise_error(int* err, int bit)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    atomicOr(err, bit);
+#else
+    *err |= bit;
+#endif
 }
 
 // ---------------------------------------------------------------- tree view
@@ -155,7 +166,7 @@ struct Counters {
 struct LaneStack {
     int32_t* base;
     int stride;
-    __device__ int32_t& at(int slot) const { return base[slot * stride]; }
+    __host__ __device__ int32_t& at(int slot) const { return base[slot * stride]; }
 };
 
 // Q1 (checkCollisions, randomWalk/CPU/quadTree.cpp:148-198 with the full
@@ -163,7 +174,7 @@ struct LaneStack {
 // box outside the root bounds, or does it intersect any rect (other than
 // `ignore`, by value)?
 template <bool kIgnore>
-__device__ inline bool q1_box(const DevTree& t, const LaneStack& st, double x0, double y0,
+__host__ __device__ inline bool q1_box(const DevTree& t, const LaneStack& st, double x0, double y0,
                               double x1, double y1, int4 ignore, Counters* cnt)
 {
     if (!in_bounds_box(t.bounds, x0, y0, x1, y1)) return true;
@@ -195,7 +206,7 @@ __device__ inline bool q1_box(const DevTree& t, const LaneStack& st, double x0, 
 // The minimum equals a brute-force scan bit for bit: the same per-rect
 // expression is evaluated, and a node is skipped only when its box distance
 // (a lower bound under monotone rounding) cannot beat the best.
-__device__ inline double nearest_rect(const DevTree& t, const LaneStack& st, double px, double py,
+__host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack& st, double px, double py,
                                       Counters* cnt)
 {
     double best = __builtin_inf();
@@ -237,7 +248,7 @@ __device__ inline double nearest_rect(const DevTree& t, const LaneStack& st, dou
 // Q2 (checkCollisons(point), randomWalk/CPU/quadTree.cpp:240-262, answered
 // exactly): original index of the rect containing p (closed), deepest
 // containment first, then lowest index; -1 if none.
-__device__ inline int q2_point(const DevTree& t, const LaneStack& st, double px, double py,
+__host__ __device__ inline int q2_point(const DevTree& t, const LaneStack& st, double px, double py,
                                Counters* cnt)
 {
     double best = 0.0;
@@ -282,7 +293,7 @@ __device__ inline int q2_point(const DevTree& t, const LaneStack& st, double px,
 // that band (rounding-ambiguous) fall back to an exact Q1.  kLiteral runs a
 // Q1 for every test instead (the reference algorithm as is).
 template <bool kLiteral>
-__device__ inline DBox biggest_square(const DevTree& t, const LaneStack& st, double px, double py,
+__host__ __device__ inline DBox biggest_square(const DevTree& t, const LaneStack& st, double px, double py,
                                       double dstar, uint32_t* n_q1, Counters* cnt, int* err)
 {
     const double MIN_DIST = (double)0.095f;
@@ -307,7 +318,7 @@ __device__ inline DBox biggest_square(const DevTree& t, const LaneStack& st, dou
         const bool c = collides(o.x0, o.y0, o.x1, o.y1);
         if (!(c || !max_reached || dist > MIN_DIST)) break;
         if (it >= kMaxBisect) {  // cannot happen for a finite layout; reported
-            if (err) atomicOr(err, kErrBisect);
+            if (err) raise_error(err, kErrBisect);
             break;
         }
         if (c) {
@@ -325,7 +336,7 @@ __device__ inline DBox biggest_square(const DevTree& t, const LaneStack& st, dou
 // Gaussian-surface factor search (getAdjustedGaussianFactor,
 // randomWalk/CPU/quadTree.cpp:315-349) and createGaussianSurface
 // (randomWalk/CPU/rectHost.cpp:84-100).
-__device__ inline DBox gauss_surface(const int4& r, double fx, double fy)
+__host__ __device__ inline DBox gauss_surface(const int4& r, double fx, double fy)
 {
     const double mx = ((double)r.x + (double)r.z) / 2.;
     const double my = ((double)r.y + (double)r.w) / 2.;
@@ -336,7 +347,7 @@ __device__ inline DBox gauss_surface(const int4& r, double fx, double fy)
     return DBox{mx + vx, my + vy, mx - vx, my - vy};
 }
 
-__device__ inline double gauss_factor(const DevTree& t, const LaneStack& st, const int4& r,
+__host__ __device__ inline double gauss_factor(const DevTree& t, const LaneStack& st, const int4& r,
                                       double factor, bool axis_y)
 {
     bool dividing = true, first = true;

@@ -91,6 +91,19 @@ def check_tree(lay, tree, leaf_cap):
 
 
 @pytest.mark.parametrize("name", [f for f, _ in FIXTURES])
+@pytest.mark.parametrize("cap", [1, 16])
+def test_tree_equals_host_build(name, cap):
+    """The device build equals the sequential host restatement node for node."""
+    from test_hostsim import HostTree
+    lay, _, _ = load_fixture(name)
+    torch_dev()
+    g_nodes, g_rects, g_ids = frw.Tree(lay, leaf_cap=cap).export()
+    h_nodes, h_rects, h_ids, info = HostTree(lay, cap).export()
+    assert np.array_equal(g_nodes, h_nodes)
+    assert np.array_equal(g_rects, h_rects) and np.array_equal(g_ids, h_ids)
+
+
+@pytest.mark.parametrize("name", [f for f, _ in FIXTURES])
 @pytest.mark.parametrize("cap", [4, 16, 64])
 def test_tree_invariants_fixtures(name, cap):
     lay, _, _ = load_fixture(name)
@@ -116,6 +129,9 @@ def test_tree_large_deterministic():
     na, ra, ia = a.export()
     nb, rb_, ib = b.export()
     assert np.array_equal(na, nb) and np.array_equal(ia, ib)
+    from test_hostsim import HostTree
+    hn, hr, hi, _ = HostTree(lay, 16).export()
+    assert np.array_equal(na, hn) and np.array_equal(ia, hi)
     owned = np.zeros(len(ia), dtype=np.int64)
     np.add.at(owned, np.concatenate([np.arange(s, e) for s, e in zip(na[:, 5], na[:, 6]) if e > s]), 1)
     assert np.all(owned == 1)
