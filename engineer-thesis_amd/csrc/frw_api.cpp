@@ -153,6 +153,7 @@ void frw_tree_free(frw_tree* t)
     if (t->d_work) hipFree(t->d_work);
     if (t->d_err) hipFree(t->d_err);
     if (t->d_surface) hipFree(t->d_surface);
+    if (t->d_job) hipFree(t->d_job);
     delete t;
 }
 
@@ -192,6 +193,7 @@ int frw_tree_build(const frw_layout* l, const frw_tree_params* p, void* stream, 
         if (rc == FRW_OK) rc = check_hip(hipMalloc(&t->d_err, sizeof(int)), "hipMalloc err");
         if (rc == FRW_OK) rc = check_hip(hipMemset(t->d_err, 0, sizeof(int)), "memset err");
         if (rc == FRW_OK) rc = check_hip(hipMalloc(&t->d_surface, 4 * sizeof(double)), "hipMalloc surface");
+        if (rc == FRW_OK) rc = check_hip(hipMalloc(&t->d_job, sizeof(WalkJob)), "hipMalloc job");
     }
     if (rc != FRW_OK) {
         frw_tree_free(t);

@@ -1,0 +1,277 @@
+// Batched walks: many layouts and masters, each with its own walk count, in
+// one persistent kernel launch (one job table, one work queue), optionally
+// captured once as a HIP graph and replayed; plus the launch-shape autotuner.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "frw_internal.h"
+
+using namespace frw;
+
+struct frw_batch {
+    int device = 0;
+    std::vector<const frw_tree*> trees;
+    std::vector<frw_batch_job> jobs;
+    std::vector<int64_t> count_off;  // offset of each job's bins in the counts array
+    int64_t counts_size = 0;
+    uint64_t total_walks = 0;
+    int max_depth = 1;
+    WalkJob* d_jobs = nullptr;
+    unsigned long long* d_work = nullptr;
+    int* d_err = nullptr;
+    uint32_t* d_thresholds = nullptr;
+    // table currently uploaded / graph currently captured
+    int64_t* up_counts = nullptr;
+    int64_t* up_hops = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    frw_batch_params graph_params{};
+    void* graph_stream = nullptr;
+};
+
+namespace {
+
+void drop_graph(frw_batch* b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+int upload_jobs(frw_batch* b, int64_t* counts, int64_t* hops, hipStream_t s)
+{
+    if (b->up_counts == counts && b->up_hops == hops) return FRW_OK;
+    std::vector<WalkJob> tab(b->jobs.size());
+    uint64_t begin = 0;
+    for (size_t j = 0; j < b->jobs.size(); ++j) {
+        const frw_batch_job& q = b->jobs[j];
+        const frw_tree* t = b->trees[q.tree];
+        WalkJob& w = tab[j];
+        std::memset(&w, 0, sizeof(w));
+        w.tree = dev_view(t);
+        w.master = t->host_rects[q.master];
+        w.counts = (unsigned long long*)(counts + b->count_off[j]);
+        w.hops = (unsigned long long*)(hops + j);
+        w.walk_begin = begin;
+        w.n_walks = (uint64_t)q.n_walks;
+        w.first_walk = (uint32_t)q.first_walk;
+        w.job = q.job;
+        begin += (uint64_t)q.n_walks;
+    }
+    int rc = check_hip(hipMemcpyAsync(b->d_jobs, tab.data(), sizeof(WalkJob) * tab.size(),
+                                      hipMemcpyHostToDevice, s), "upload job table");
+    if (rc == FRW_OK) rc = check_hip(hipStreamSynchronize(s), "upload job table");
+    if (rc != FRW_OK) return rc;
+    b->up_counts = counts;
+    b->up_hops = hops;
+    drop_graph(b);
+    return FRW_OK;
+}
+
+WalkLaunch launch_of(const frw_batch* b, const frw_batch_params* p)
+{
+    WalkLaunch L;
+    L.device = b->device;
+    L.n_jobs = (int)b->jobs.size();
+    L.total_walks = b->total_walks;
+    L.max_depth = b->max_depth;
+    L.mode = p->mode;
+    L.block = p->block;
+    L.grid = p->grid;
+    L.seed = p->seed;
+    L.max_hops = (uint32_t)(p->max_hops > 0 ? p->max_hops : 10000000);
+    L.thresholds = b->d_thresholds;
+    L.work = b->d_work;
+    L.err = b->d_err;
+    return L;
+}
+
+bool same_params(const frw_batch_params& a, const frw_batch_params& b)
+{
+    return a.mode == b.mode && a.block == b.block && a.grid == b.grid && a.seed == b.seed &&
+           a.max_hops == b.max_hops;
+}
+
+}  // namespace
+
+extern "C" {
+
+int frw_batch_create(const frw_tree* const* trees, int32_t n_trees, const frw_batch_job* jobs,
+                     int32_t n_jobs, frw_batch** out)
+{
+    if (!trees || n_trees <= 0 || !jobs || n_jobs <= 0 || !out)
+        return set_error(FRW_E_INVALID, "bad batch arguments");
+    try {
+        frw_batch* b = new frw_batch;
+        hipGetDevice(&b->device);
+        b->trees.assign(trees, trees + n_trees);
+        b->jobs.assign(jobs, jobs + n_jobs);
+        for (int32_t j = 0; j < n_jobs; ++j) {
+            const frw_batch_job& q = jobs[j];
+            if (q.tree < 0 || q.tree >= n_trees || !trees[q.tree]) {
+                delete b;
+                return set_error(FRW_E_RANGE, "job " + std::to_string(j) + ": bad tree index");
+            }
+            const frw_tree* t = trees[q.tree];
+            if (t->device != b->device) {
+                delete b;
+                return set_error(FRW_E_INVALID, "all trees of a batch must live on the current device");
+            }
+            if (q.master < 0 || q.master >= t->n_rects || q.n_walks <= 0 || q.first_walk < 0 ||
+                q.first_walk + q.n_walks > ((int64_t)1 << 32)) {
+                delete b;
+                return set_error(FRW_E_RANGE, "job " + std::to_string(j) + ": bad master or walk range");
+            }
+            b->count_off.push_back(b->counts_size);
+            b->counts_size += t->n_rects + 2;
+            b->total_walks += (uint64_t)q.n_walks;
+            b->max_depth = std::max(b->max_depth, (int)t->depth);
+        }
+        uint32_t T[kNSample];
+        green_thresholds(T);
+        int rc = check_hip(hipMalloc(&b->d_jobs, sizeof(WalkJob) * n_jobs), "hipMalloc jobs");
+        if (rc == FRW_OK) rc = check_hip(hipMalloc(&b->d_work, sizeof(unsigned long long)), "hipMalloc work");
+        if (rc == FRW_OK) rc = check_hip(hipMalloc(&b->d_err, sizeof(int)), "hipMalloc err");
+        if (rc == FRW_OK) rc = check_hip(hipMemset(b->d_err, 0, sizeof(int)), "memset err");
+        if (rc == FRW_OK) rc = check_hip(hipMalloc(&b->d_thresholds, sizeof(T)), "hipMalloc thresholds");
+        if (rc == FRW_OK) rc = check_hip(hipMemcpy(b->d_thresholds, T, sizeof(T), hipMemcpyHostToDevice), "copy thresholds");
+        if (rc != FRW_OK) {
+            frw_batch_free(b);
+            return rc;
+        }
+        *out = b;
+        return FRW_OK;
+    } catch (...) {
+        return set_error(FRW_E_NOMEM, "host allocation failed");
+    }
+}
+
+int64_t frw_batch_counts_size(const frw_batch* b) { return b ? b->counts_size : -1; }
+
+int frw_batch_counts_offset(const frw_batch* b, int32_t job, int64_t* offset)
+{
+    if (!b || !offset || job < 0 || job >= (int32_t)b->jobs.size())
+        return set_error(FRW_E_INVALID, "bad job index");
+    *offset = b->count_off[job];
+    return FRW_OK;
+}
+
+int frw_batch_walk(frw_batch* b, const frw_batch_params* p, int64_t* counts, int64_t* hops,
+                   void* stream)
+{
+    if (!b || !p || !counts || !hops) return set_error(FRW_E_INVALID, "null argument");
+    if (p->mode != FRW_MODE_EXACT && p->mode != FRW_MODE_BISECT)
+        return set_error(FRW_E_INVALID, "unsupported walk mode");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = upload_jobs(b, counts, hops, s);
+    if (rc != FRW_OK) return rc;
+    const WalkLaunch L = launch_of(b, p);
+    if (!p->use_graph) return launch_walk_jobs(b->d_jobs, L, s);
+    if (!b->exec || !same_params(b->graph_params, *p) || b->graph_stream != stream) {
+        drop_graph(b);
+        int block = 0, grid = 0;
+        size_t lds = 0;
+        if ((rc = plan_walk(L, &block, &grid, &lds)) != FRW_OK) return rc;
+        if ((rc = check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture")) != FRW_OK)
+            return rc;
+        const int lrc = launch_walk_jobs(b->d_jobs, L, s);
+        hipGraph_t g = nullptr;
+        rc = check_hip(hipStreamEndCapture(s, &g), "end capture");
+        if (lrc != FRW_OK) {
+            if (g) hipGraphDestroy(g);
+            return lrc;
+        }
+        if (rc != FRW_OK) return rc;
+        b->graph = g;
+        if ((rc = check_hip(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "instantiate")) != FRW_OK) {
+            drop_graph(b);
+            return rc;
+        }
+        b->graph_params = *p;
+        b->graph_stream = stream;
+    }
+    return check_hip(hipGraphLaunch(b->exec, s), "graph launch");
+}
+
+int frw_batch_autotune(frw_batch* b, const frw_batch_params* base, frw_batch_params* best,
+                       double* best_hops_per_s)
+{
+    if (!b || !base || !best) return set_error(FRW_E_INVALID, "null argument");
+    int64_t *d_counts = nullptr, *d_hops = nullptr;
+    int rc = check_hip(hipMalloc(&d_counts, sizeof(int64_t) * b->counts_size), "hipMalloc counts");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hops, sizeof(int64_t) * b->jobs.size()), "hipMalloc hops");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    double best_rate = -1.0;
+    frw_batch_params cand = *base;
+    cand.use_graph = 0;
+    int cus = 0;
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
+    const int blocks[] = {64, 128, 192, 256};
+    const int waves_per_cu[] = {4, 8, 16, 32};
+    for (int bi = 0; bi < 4 && rc == FRW_OK; ++bi) {
+        for (int wi = 0; wi < 4 && rc == FRW_OK; ++wi) {
+            cand.block = blocks[bi];
+            cand.grid = std::max(1, cus * waves_per_cu[wi] / (blocks[bi] / kWaveSize));
+            WalkLaunch L = launch_of(b, &cand);
+            int blk = 0, grd = 0;
+            size_t lds = 0;
+            if (plan_walk(L, &blk, &grd, &lds) != FRW_OK) continue;  // does not fit: skip
+            hipMemsetAsync(d_counts, 0, sizeof(int64_t) * b->counts_size, nullptr);
+            hipMemsetAsync(d_hops, 0, sizeof(int64_t) * b->jobs.size(), nullptr);
+            hipEventRecord(e0, nullptr);
+            rc = frw_batch_walk(b, &cand, d_counts, d_hops, nullptr);
+            hipEventRecord(e1, nullptr);
+            if (rc != FRW_OK) break;
+            if ((rc = check_hip(hipDeviceSynchronize(), "autotune run")) != FRW_OK) break;
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            std::vector<int64_t> h(b->jobs.size());
+            hipMemcpy(h.data(), d_hops, sizeof(int64_t) * h.size(), hipMemcpyDeviceToHost);
+            int64_t tot = 0;
+            for (int64_t v : h) tot += v;
+            const double rate = tot / (ms * 1e-3);
+            if (rate > best_rate) {
+                best_rate = rate;
+                *best = cand;
+            }
+        }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (d_counts) hipFree(d_counts);
+    if (d_hops) hipFree(d_hops);
+    b->up_counts = b->up_hops = nullptr;  // the tuning buffers are gone
+    if (rc != FRW_OK) return rc;
+    if (best_rate < 0) return set_error(FRW_E_RANGE, "no launch shape fits");
+    best->use_graph = base->use_graph;
+    if (best_hops_per_s) *best_hops_per_s = best_rate;
+    return FRW_OK;
+}
+
+int frw_batch_check(const frw_batch* b)
+{
+    if (!b) return set_error(FRW_E_INVALID, "null batch");
+    int err = 0;
+    int rc = check_hip(hipMemcpy(&err, b->d_err, sizeof(int), hipMemcpyDeviceToHost), "read err flag");
+    if (rc != FRW_OK) return rc;
+    if (err) return set_error(FRW_E_KERNEL, "kernel error flag " + std::to_string(err));
+    return FRW_OK;
+}
+
+void frw_batch_free(frw_batch* b)
+{
+    if (!b) return;
+    drop_graph(b);
+    if (b->d_jobs) hipFree(b->d_jobs);
+    if (b->d_work) hipFree(b->d_work);
+    if (b->d_err) hipFree(b->d_err);
+    if (b->d_thresholds) hipFree(b->d_thresholds);
+    delete b;
+}
+
+}  // extern "C"

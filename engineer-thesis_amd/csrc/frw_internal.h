@@ -10,6 +10,41 @@
 #include "frw_device.h"
 #include "frw_types.h"
 
+namespace frw {
+
+// One row of a walk launch: a layout tree, its master and a range of walks.
+struct WalkJob {
+    DevTree tree;
+    int4 master;
+    double surface[4];             // Gaussian surface, written by the prologue
+    unsigned long long* counts;    // n_rects + 2 bins
+    unsigned long long* hops;      // 1 element
+    int32_t* hit_out;              // optional, indexed by walk within the job
+    int32_t* hops_out;
+    uint64_t walk_begin;           // global index of the job's first walk
+    uint64_t n_walks;
+    uint32_t first_walk;           // Philox walk id of the job's first walk
+    uint32_t job;                  // Philox counter word 1
+};
+
+struct WalkLaunch {
+    int device = 0;
+    int n_jobs = 1;
+    uint64_t total_walks = 0;
+    int max_depth = 1;
+    int mode = 0;
+    int block = 0;
+    int grid = 0;
+    uint64_t seed = 1;
+    uint32_t max_hops = 10000000;
+    const uint32_t* thresholds = nullptr;
+    unsigned long long* work = nullptr;
+    int* err = nullptr;
+    unsigned long long* traffic = nullptr;
+};
+
+}  // namespace frw
+
 struct frw_layout {
     std::vector<int4> rects;  // file order = original ids
     int4 bounds;              // padded absorbing bounds
@@ -35,6 +70,7 @@ struct frw_tree {
     unsigned long long* d_work = nullptr;  // walk work-queue counter
     int* d_err = nullptr;                  // kernel error bits
     double* d_surface = nullptr;           // Gaussian surface of the master
+    frw::WalkJob* d_job = nullptr;         // job slot of a plain walk
 };
 
 namespace frw {
@@ -63,7 +99,9 @@ int launch_q3(const frw_tree* t, const double* pts, int64_t n, int mode, double*
               hipStream_t s);
 int launch_gauss(const frw_tree* t, int4 master, double* d_out, hipStream_t s);
 
-// walk launcher (frw_walk.hip)
+// walk launchers (frw_walk.hip)
+int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds);
+int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s);
 int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o,
                 hipStream_t s, unsigned long long* traffic = nullptr);
 

@@ -2,9 +2,12 @@
 // randomWalk/Cuda/randomwalk.cu:64-134).
 //
 // Every lane owns one walker.  A lane whose walk has ended takes the next walk
-// id from a device work counter (one atomic per wave per refill), so uneven
-// walk lengths never idle a wave while work remains.  Per hop (reference
-// RandomWalk, randomWalk/CPU/mainFunctions.cpp:48-90):
+// index from a device work counter (one atomic per wave per refill), so uneven
+// walk lengths never idle a wave while work remains.  A launch runs a table of
+// jobs (layout tree, master surface, walk range); one job is the plain
+// single-layout walk, many jobs are the batched form, where the global walk
+// index is mapped to its job by a binary search over the jobs' walk offsets.
+// Per hop (reference RandomWalk, randomWalk/CPU/mainFunctions.cpp:48-90):
 //   u     = Philox4x32-10(walk, job, hop/4, 'WALK'; seed)[hop % 4]
 //   panel = largest i with T[i] <= u          (Green CDF thresholds in LDS)
 //   p     = getPointFromNindex(square, panel) (FP64, reference op order)
@@ -12,50 +15,80 @@
 //   exact mode:  d* = nearest-rect distance; d* <= 0 -> hit (Q2 for the id),
 //                else square = biggest square replayed from d*
 //   bisect mode: square = literal bisection (one Q1 per step), hit = Q2(p)
-// Tallies: int64 hit counts per original rect id + escaped + stuck bins, and
-// the total hop count; optional per-walk outputs for replay checks.
+// Tallies: int64 hit counts per original rect id + escaped + stuck bins and
+// hop counts per job; optional per-walk outputs for replay checks.
+#include <algorithm>
+
 #include "frw_internal.h"
 
 namespace frw {
 namespace {
 
 struct WalkArgs {
-    const double* surface;    // Gaussian surface of the master (4 doubles)
     const uint32_t* thresholds;
     unsigned long long* work;
     int* err;
-    unsigned long long* counts;
-    unsigned long long* hops_total;
-    int32_t* hit_out;
-    int32_t* hops_out;
     unsigned long long* traffic;  // kStats: node records, rect records
     uint64_t seed;
-    uint64_t n_walks;
-    uint32_t first_walk;
-    uint32_t job;
+    uint64_t total_walks;
     uint32_t max_hops;
+    int32_t n_jobs;
 };
 
-__global__ void zero_counter(unsigned long long* work) { *work = 0ull; }
+constexpr int kLdsJobs = 128;  // job walk offsets cached in LDS
+
+__device__ inline int find_job(const WalkJob* jobs, const uint64_t* s_begin, int n_jobs, uint64_t g)
+{
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const uint64_t b = mid < kLdsJobs ? s_begin[mid] : jobs[mid].walk_begin;
+        if (b <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Prologue: Gaussian surface of every job's master (one lane per job) and a
+// zeroed work counter.  Part of the launch so the whole walk can be captured
+// in a graph and replayed.
+__global__ void __launch_bounds__(64) walk_prologue(WalkJob* jobs, int n_jobs, unsigned long long* work)
+{
+    extern __shared__ int32_t stack_mem[];
+    const LaneStack st{stack_mem + threadIdx.x, (int)blockDim.x};
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j == 0) *work = 0ull;
+    if (j >= n_jobs) return;
+    WalkJob& jb = jobs[j];
+    const double fx = gauss_factor(jb.tree, st, jb.master, 1.5, false);
+    const double fy = gauss_factor(jb.tree, st, jb.master, 1.5, true);
+    const DBox s = gauss_surface(jb.master, fx, fy);
+    jb.surface[0] = s.x0;
+    jb.surface[1] = s.y0;
+    jb.surface[2] = s.x1;
+    jb.surface[3] = s.y1;
+}
 
 template <int kMode, bool kStats>
-__global__ void __launch_bounds__(256) walk_kernel(DevTree t, WalkArgs a)
+__global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a)
 {
     extern __shared__ int32_t stack_mem[];
     __shared__ uint32_t sT[kNSample];
+    __shared__ uint64_t s_begin[kLdsJobs];
     for (int i = threadIdx.x; i < kNSample; i += blockDim.x) sT[i] = a.thresholds[i];
+    for (int i = threadIdx.x; i < min(a.n_jobs, kLdsJobs); i += blockDim.x) s_begin[i] = jobs[i].walk_begin;
     __syncthreads();
     const LaneStack st{stack_mem + threadIdx.x, (int)blockDim.x};
     const int lane = threadIdx.x & (kWaveSize - 1);
-    const double s0x = a.surface[0], s0y = a.surface[1], s1x = a.surface[2], s1y = a.surface[3];
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
 
     bool active = false, done = false;
-    uint64_t w = 0;
+    int j = 0;
+    uint64_t w = 0;  // walk index within the job
     uint32_t hop = 0;
+    DevTree t = jobs[0].tree;
     DBox sq{0, 0, 0, 0};
     uint32_t rnd[4] = {0, 0, 0, 0};
-    unsigned long long my_hops = 0;
+    unsigned long long my_hops = 0;  // hops accumulated for job j
     Counters cnt{0u, 0u};
     Counters* cp = kStats ? &cnt : nullptr;
     unsigned long long my_nodes = 0, my_rects = 0;
@@ -70,11 +103,20 @@ __global__ void __launch_bounds__(256) walk_kernel(DevTree t, WalkArgs a)
             base = __shfl(base, leader);
             if (need) {
                 const unsigned long long below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
-                w = base + (unsigned long long)__popcll(below);
-                if (w < a.n_walks) {
+                const uint64_t g = base + (unsigned long long)__popcll(below);
+                if (g < a.total_walks) {
                     active = true;
                     hop = 0;
-                    sq = DBox{s0x, s0y, s1x, s1y};
+                    const int nj = a.n_jobs == 1 ? 0 : find_job(jobs, s_begin, a.n_jobs, g);
+                    if (nj != j) {
+                        if (my_hops) atomicAdd(jobs[j].hops, my_hops);
+                        my_hops = 0;
+                        j = nj;
+                        t = jobs[j].tree;
+                    }
+                    w = g - jobs[j].walk_begin;
+                    sq = DBox{jobs[j].surface[0], jobs[j].surface[1], jobs[j].surface[2],
+                              jobs[j].surface[3]};
                 } else {
                     done = true;
                 }
@@ -84,8 +126,8 @@ __global__ void __launch_bounds__(256) walk_kernel(DevTree t, WalkArgs a)
         if (!active) continue;
 
         if ((hop & 3u) == 0u) {
-            rnd[0] = a.first_walk + (uint32_t)w;
-            rnd[1] = a.job;
+            rnd[0] = jobs[j].first_walk + (uint32_t)w;
+            rnd[1] = jobs[j].job;
             rnd[2] = hop >> 2;
             rnd[3] = kWalkDomain;
             philox4x32_10(rnd, k0, k1);
@@ -119,15 +161,19 @@ __global__ void __launch_bounds__(256) walk_kernel(DevTree t, WalkArgs a)
         }
         if (result != -3) {
             const int bin = result >= 0 ? result : (result == -1 ? t.n_rects : t.n_rects + 1);
-            atomicAdd(&a.counts[bin], 1ull);
+            atomicAdd(&jobs[j].counts[bin], 1ull);
             my_hops += hop;
-            if (a.hit_out) a.hit_out[w] = result;
-            if (a.hops_out) a.hops_out[w] = (int32_t)hop;
+            if (jobs[j].hit_out) jobs[j].hit_out[w] = result;
+            if (jobs[j].hops_out) jobs[j].hops_out[w] = (int32_t)hop;
             active = false;
         }
     }
-    for (int off = 32; off >= 1; off >>= 1) my_hops += __shfl_xor(my_hops, off);
-    if (lane == 0 && my_hops) atomicAdd(a.hops_total, my_hops);
+    if (a.n_jobs == 1) {
+        for (int off = 32; off >= 1; off >>= 1) my_hops += __shfl_xor(my_hops, off);
+        if (lane == 0 && my_hops) atomicAdd(jobs[0].hops, my_hops);
+    } else if (my_hops) {
+        atomicAdd(jobs[j].hops, my_hops);
+    }
     if (kStats) {
         for (int off = 32; off >= 1; off >>= 1) {
             my_nodes += __shfl_xor(my_nodes, off);
@@ -140,65 +186,110 @@ __global__ void __launch_bounds__(256) walk_kernel(DevTree t, WalkArgs a)
     }
 }
 
+// Writes the single job of a plain walk into the tree's job slot.
+__global__ void set_job(WalkJob* slot, WalkJob jb) { *slot = jb; }
+
 }  // namespace
+
+int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds)
+{
+    int b = L.block;
+    if (b < 0 || b % kWaveSize != 0 || b > 256)
+        return set_error(FRW_E_INVALID, "block must be a multiple of 64 and <= 256");
+    const size_t per_lane = (size_t)(3 * L.max_depth + 2) * sizeof(int32_t);
+    const size_t fixed = kNSample * 4 + kLdsJobs * 8;
+    int lds_block = 0, lds_cu = 0, cus = 0;
+    hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, L.device);
+    hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, L.device);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, L.device);
+    if (lds_block <= 0) lds_block = 64 * 1024;
+    if (lds_cu <= 0) lds_cu = lds_block;
+    if (b == 0) {
+        b = 256;
+        while (b >= 64 && (size_t)b * per_lane + fixed > (size_t)lds_block) b -= 64;
+    }
+    if (b < 64 || (size_t)b * per_lane + fixed > (size_t)lds_block)
+        return set_error(FRW_E_RANGE, "tree too deep for the LDS traversal stack at this block size");
+    *block = b;
+    *lds = (size_t)b * per_lane;
+    int g = L.grid;
+    if (g <= 0) {
+        const int per_cu_lds = (int)((size_t)lds_cu / (*lds + fixed));
+        const int per_cu = std::max(1, std::min(per_cu_lds, 2048 / b));
+        g = std::max(1, cus * per_cu);
+        const int64_t need = ((int64_t)L.total_walks + b - 1) / b;
+        if (need < g) g = (int)need;
+    }
+    *grid = g;
+    return FRW_OK;
+}
+
+int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
+{
+    if (L.total_walks == 0) return FRW_OK;
+    int block = 0, grid = 0;
+    size_t lds = 0;
+    int rc = plan_walk(L, &block, &grid, &lds);
+    if (rc != FRW_OK) return rc;
+    const size_t plds = (size_t)64 * (3 * L.max_depth + 2) * sizeof(int32_t);
+    walk_prologue<<<(L.n_jobs + 63) / 64, 64, plds, s>>>(d_jobs, L.n_jobs, L.work);
+    if ((rc = check_hip(hipGetLastError(), "walk_prologue")) != FRW_OK) return rc;
+    WalkArgs a;
+    a.thresholds = L.thresholds;
+    a.work = L.work;
+    a.err = L.err;
+    a.traffic = L.traffic;
+    a.seed = L.seed;
+    a.total_walks = L.total_walks;
+    a.max_hops = L.max_hops;
+    a.n_jobs = L.n_jobs;
+    if (L.traffic) {
+        if (L.mode == kModeBisect)
+            walk_kernel<kModeBisect, true><<<grid, block, lds, s>>>(d_jobs, a);
+        else
+            walk_kernel<kModeExact, true><<<grid, block, lds, s>>>(d_jobs, a);
+    } else {
+        if (L.mode == kModeBisect)
+            walk_kernel<kModeBisect, false><<<grid, block, lds, s>>>(d_jobs, a);
+        else
+            walk_kernel<kModeExact, false><<<grid, block, lds, s>>>(d_jobs, a);
+    }
+    return check_hip(hipGetLastError(), "walk_kernel");
+}
 
 int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o,
                 hipStream_t s, unsigned long long* traffic)
 {
     if (p->n_walks == 0) return FRW_OK;
-    const int4 m = t->host_rects[p->master];
-    int rc = launch_gauss(t, m, t->d_surface, s);
+    WalkJob jb{};
+    jb.tree = dev_view(t);
+    jb.counts = (unsigned long long*)o->counts;
+    jb.hops = (unsigned long long*)o->hops;
+    jb.hit_out = o->hit_out;
+    jb.hops_out = o->hops_out;
+    jb.walk_begin = 0;
+    jb.n_walks = (uint64_t)p->n_walks;
+    jb.first_walk = (uint32_t)p->first_walk;
+    jb.job = p->job;
+    jb.master = t->host_rects[p->master];
+    set_job<<<1, 1, 0, s>>>(t->d_job, jb);
+    int rc = check_hip(hipGetLastError(), "set_job");
     if (rc != FRW_OK) return rc;
-    zero_counter<<<1, 1, 0, s>>>(t->d_work);
-    if ((rc = check_hip(hipGetLastError(), "zero_counter")) != FRW_OK) return rc;
-
-    int block = p->block;
-    if (block < 0 || block % kWaveSize != 0 || block > 256)
-        return set_error(FRW_E_INVALID, "block must be a multiple of 64 and <= 256");
-    if (block == 0) block = block_for_stack(t, 256, kNSample * 4);
-    if (block == 0 || (size_t)block * stack_bytes_per_lane(t) + kNSample * 4 > (size_t)lds_per_block(t))
-        return set_error(FRW_E_RANGE, "tree too deep for the LDS traversal stack at this block size");
-    const size_t lds = (size_t)block * stack_bytes_per_lane(t);
-    int grid = p->grid;
-    if (grid <= 0) {
-        int cus = 0, lds_cu = 0;
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, t->device);
-        hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, t->device);
-        if (lds_cu <= 0) lds_cu = 64 * 1024;
-        const int per_cu_lds = (int)((size_t)lds_cu / (lds + kNSample * 4));
-        const int per_cu = std::max(1, std::min(per_cu_lds, 2048 / block));
-        grid = std::max(1, cus * per_cu);
-        const int64_t need = (p->n_walks + block - 1) / block;
-        if (need < grid) grid = (int)need;
-    }
-    WalkArgs a;
-    a.surface = t->d_surface;
-    a.thresholds = t->d_thresholds;
-    a.work = t->d_work;
-    a.err = t->d_err;
-    a.counts = (unsigned long long*)o->counts;
-    a.hops_total = (unsigned long long*)o->hops;
-    a.hit_out = o->hit_out;
-    a.hops_out = o->hops_out;
-    a.traffic = traffic;
-    a.seed = p->seed;
-    a.n_walks = (uint64_t)p->n_walks;
-    a.first_walk = (uint32_t)p->first_walk;
-    a.job = p->job;
-    a.max_hops = (uint32_t)(p->max_hops > 0 ? p->max_hops : 10000000);
-    const DevTree v = dev_view(t);
-    if (traffic) {
-        if (p->mode == kModeBisect)
-            walk_kernel<kModeBisect, true><<<grid, block, lds, s>>>(v, a);
-        else
-            walk_kernel<kModeExact, true><<<grid, block, lds, s>>>(v, a);
-    } else {
-        if (p->mode == kModeBisect)
-            walk_kernel<kModeBisect, false><<<grid, block, lds, s>>>(v, a);
-        else
-            walk_kernel<kModeExact, false><<<grid, block, lds, s>>>(v, a);
-    }
-    return check_hip(hipGetLastError(), "walk_kernel");
+    WalkLaunch L;
+    L.device = t->device;
+    L.n_jobs = 1;
+    L.total_walks = (uint64_t)p->n_walks;
+    L.max_depth = t->depth;
+    L.mode = p->mode;
+    L.block = p->block;
+    L.grid = p->grid;
+    L.seed = p->seed;
+    L.max_hops = (uint32_t)(p->max_hops > 0 ? p->max_hops : 10000000);
+    L.thresholds = t->d_thresholds;
+    L.work = t->d_work;
+    L.err = t->d_err;
+    L.traffic = traffic;
+    return launch_walk_jobs(t->d_job, L, s);
 }
 
 }  // namespace frw

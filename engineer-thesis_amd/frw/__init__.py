@@ -56,6 +56,16 @@ class WalkOutputs(C.Structure):
                 ("hops_out", C.c_void_p)]
 
 
+class BatchJob(C.Structure):
+    _fields_ = [("tree", C.c_int32), ("master", C.c_int32), ("n_walks", C.c_int64),
+                ("first_walk", C.c_int64), ("job", C.c_uint32), ("reserved", C.c_int32)]
+
+
+class BatchParams(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("block", C.c_int32), ("grid", C.c_int32),
+                ("use_graph", C.c_int32), ("seed", C.c_uint64), ("max_hops", C.c_int64)]
+
+
 class WalkSummary(C.Structure):
     _fields_ = [("n_walks", C.c_int64), ("hops", C.c_int64), ("n_master", C.c_int64),
                 ("n_escaped", C.c_int64), ("n_stuck", C.c_int64), ("fraction_other", C.c_double),
@@ -70,6 +80,8 @@ EXPORTS = [
     "frw_tree_build", "frw_tree_info_get", "frw_tree_export", "frw_tree_free",
     "frw_query_collide", "frw_query_point", "frw_query_nearest", "frw_query_maxsquare",
     "frw_gauss_surface", "frw_walk_async", "frw_walk", "frw_walk_traffic", "frw_write_results",
+    "frw_batch_create", "frw_batch_counts_size", "frw_batch_counts_offset", "frw_batch_walk",
+    "frw_batch_autotune", "frw_batch_check", "frw_batch_free",
 ]
 
 _lib = None
@@ -112,6 +124,16 @@ def lib():
     L.frw_walk.argtypes = [vp, C.POINTER(WalkParams), vp, C.POINTER(WalkSummary)]
     L.frw_walk_traffic.argtypes = [vp, C.POINTER(WalkParams), vp]
     L.frw_write_results.argtypes = [vp, vp, i64, C.c_char_p]
+    L.frw_batch_create.argtypes = [vp, i32, vp, i32, C.POINTER(vp)]
+    L.frw_batch_counts_size.restype = i64
+    L.frw_batch_counts_size.argtypes = [vp]
+    L.frw_batch_counts_offset.argtypes = [vp, i32, C.POINTER(i64)]
+    L.frw_batch_walk.argtypes = [vp, C.POINTER(BatchParams), vp, vp, vp]
+    L.frw_batch_autotune.argtypes = [vp, C.POINTER(BatchParams), C.POINTER(BatchParams),
+                                     C.POINTER(C.c_double)]
+    L.frw_batch_check.argtypes = [vp]
+    L.frw_batch_free.argtypes = [vp]
+    L.frw_batch_free.restype = None
     _lib = L
     return L
 
@@ -268,6 +290,57 @@ class Tree:
         try:
             if self.h:
                 lib().frw_tree_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many (tree, master, walk count) jobs in one persistent walk launch."""
+
+    def __init__(self, trees, jobs):
+        """trees: list of Tree; jobs: list of dicts with keys tree, master,
+        n_walks and optionally first_walk, job."""
+        self.trees = list(trees)
+        arr = (C.c_void_p * len(self.trees))(*[t.h.value for t in self.trees])
+        js = (BatchJob * len(jobs))()
+        for i, j in enumerate(jobs):
+            js[i] = BatchJob(tree=j["tree"], master=j["master"], n_walks=j["n_walks"],
+                             first_walk=j.get("first_walk", 0), job=j.get("job", i), reserved=0)
+        self.n_jobs = len(jobs)
+        h = C.c_void_p()
+        _check(lib().frw_batch_create(arr, len(self.trees), js, len(jobs), C.byref(h)))
+        self.h = h
+
+    def counts_size(self):
+        return lib().frw_batch_counts_size(self.h)
+
+    def counts_offset(self, job):
+        o = C.c_int64()
+        _check(lib().frw_batch_counts_offset(self.h, job, C.byref(o)))
+        return o.value
+
+    def walk(self, counts, hops, seed=1, mode=MODE_EXACT, block=0, grid=0, use_graph=False,
+             max_hops=0, stream=None):
+        p = BatchParams(mode=mode, block=block, grid=grid, use_graph=int(use_graph), seed=seed,
+                        max_hops=max_hops)
+        _check(lib().frw_batch_walk(self.h, C.byref(p), counts.data_ptr(), hops.data_ptr(), stream))
+
+    def autotune(self, seed=1, mode=MODE_EXACT, use_graph=False):
+        base = BatchParams(mode=mode, block=0, grid=0, use_graph=int(use_graph), seed=seed,
+                           max_hops=0)
+        best = BatchParams()
+        rate = C.c_double()
+        _check(lib().frw_batch_autotune(self.h, C.byref(base), C.byref(best), C.byref(rate)))
+        return {"block": best.block, "grid": best.grid}, rate.value
+
+    def check(self):
+        _check(lib().frw_batch_check(self.h))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().frw_batch_free(self.h)
                 self.h = None
         except Exception:
             pass

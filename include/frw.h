@@ -193,6 +193,50 @@ typedef struct {
 int frw_walk(const frw_tree* t, const frw_walk_params* p, int64_t* counts_host,
              frw_walk_summary* summary);
 
+/* ---------------------------------------------------------------- batches
+ * Many layouts and masters, each with its own walk count, in one persistent
+ * kernel launch.  No reference counterpart: the reference runs one layout and
+ * one master per process (randomWalk/Cuda/getAvgPathLenCUDA.cu:50-80). */
+
+typedef struct frw_batch frw_batch;
+
+typedef struct {
+    int32_t tree;        /* index into the trees array */
+    int32_t master;      /* RECT_ID in that layout */
+    int64_t n_walks;
+    int64_t first_walk;  /* Philox walk id of the job's first walk */
+    uint32_t job;        /* Philox counter word 1 */
+    int32_t reserved;
+} frw_batch_job;
+
+typedef struct {
+    int32_t mode;        /* FRW_MODE_EXACT or FRW_MODE_BISECT */
+    int32_t block;       /* 0 = default */
+    int32_t grid;        /* 0 = fill the device */
+    int32_t use_graph;   /* capture the launch as a HIP graph once, then replay */
+    uint64_t seed;
+    int64_t max_hops;    /* 0 = 10^7 */
+} frw_batch_params;
+
+/* The trees must outlive the batch. */
+int frw_batch_create(const frw_tree* const* trees, int32_t n_trees, const frw_batch_job* jobs,
+                     int32_t n_jobs, frw_batch** out);
+/* Size of the counts array: sum over jobs of (n_rects + 2). */
+int64_t frw_batch_counts_size(const frw_batch* b);
+/* Offset of job j's bins (n_rects hits, escaped, stuck) in the counts array. */
+int frw_batch_counts_offset(const frw_batch* b, int32_t job, int64_t* offset);
+/* counts: device int64[frw_batch_counts_size], hops: device int64[n_jobs];
+ * both accumulated (caller zeroes).  Asynchronous on `stream`. */
+int frw_batch_walk(frw_batch* b, const frw_batch_params* p, int64_t* counts, int64_t* hops,
+                   void* stream);
+/* Times each launch shape (block x waves per CU) on the whole batch and
+ * returns the fastest in *best (walk steps/s in *best_hops_per_s). */
+int frw_batch_autotune(frw_batch* b, const frw_batch_params* base, frw_batch_params* best,
+                       double* best_hops_per_s);
+/* FRW_E_KERNEL if a kernel of this batch raised an error flag. */
+int frw_batch_check(const frw_batch* b);
+void frw_batch_free(frw_batch* b);
+
 /* Measurement hook (no reference counterpart): the same walks as frw_walk
  * with per-hop traffic counters compiled in.  out = {node records read
  * (32 B each), rect records read (16 B each), hops}.  Slower than frw_walk;
