@@ -94,6 +94,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run `make -C engineer-thesis_amd`")
+    # One HIP runtime per process: when torch is present, let it load its
+    # libamdhip64 first so libfrw.so binds to the same instance (same SONAME)
+    # instead of pulling in a second copy from /opt/rocm.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, i32, i64, u64 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64
     L.frw_last_error_string.restype = C.c_char_p
