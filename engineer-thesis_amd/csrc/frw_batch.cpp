@@ -28,7 +28,7 @@ struct frw_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     frw_batch_params graph_params{};
-    void* graph_stream = nullptr;
+    hipStream_t capture = nullptr;  // private stream the graph is captured on
 };
 
 namespace {
@@ -170,16 +170,21 @@ int frw_batch_walk(frw_batch* b, const frw_batch_params* p, int64_t* counts, int
     if (rc != FRW_OK) return rc;
     const WalkLaunch L = launch_of(b, p);
     if (!p->use_graph) return launch_walk_jobs(b->d_jobs, L, s);
-    if (!b->exec || !same_params(b->graph_params, *p) || b->graph_stream != stream) {
+    if (!b->exec || !same_params(b->graph_params, *p)) {
         drop_graph(b);
         int block = 0, grid = 0;
         size_t lds = 0;
         if ((rc = plan_walk(L, &block, &grid, &lds)) != FRW_OK) return rc;
-        if ((rc = check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture")) != FRW_OK)
+        // capture on a private stream (the legacy default stream cannot be
+        // captured); the graph is then launched on the caller's stream
+        if (!b->capture &&
+            (rc = check_hip(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking), "capture stream")) != FRW_OK)
             return rc;
-        const int lrc = launch_walk_jobs(b->d_jobs, L, s);
+        if ((rc = check_hip(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal), "begin capture")) != FRW_OK)
+            return rc;
+        const int lrc = launch_walk_jobs(b->d_jobs, L, b->capture);
         hipGraph_t g = nullptr;
-        rc = check_hip(hipStreamEndCapture(s, &g), "end capture");
+        rc = check_hip(hipStreamEndCapture(b->capture, &g), "end capture");
         if (lrc != FRW_OK) {
             if (g) hipGraphDestroy(g);
             return lrc;
@@ -191,7 +196,6 @@ int frw_batch_walk(frw_batch* b, const frw_batch_params* p, int64_t* counts, int
             return rc;
         }
         b->graph_params = *p;
-        b->graph_stream = stream;
     }
     return check_hip(hipGraphLaunch(b->exec, s), "graph launch");
 }
@@ -267,6 +271,7 @@ void frw_batch_free(frw_batch* b)
 {
     if (!b) return;
     drop_graph(b);
+    if (b->capture) hipStreamDestroy(b->capture);
     if (b->d_jobs) hipFree(b->d_jobs);
     if (b->d_work) hipFree(b->d_work);
     if (b->d_err) hipFree(b->d_err);

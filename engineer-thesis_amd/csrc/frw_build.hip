@@ -2,18 +2,19 @@
 //
 // Replaces the reference's recursive dynamic-parallelism build
 // (randomWalk/Cuda/createquadtree.cu:97-371, one block per node):
-//   1. key per rect: the deepest quadtree cell that holds the whole rect is
-//      the longest common 2-bit prefix of the Morton codes of its two corners;
-//      key = (cell code padded with zeros) << 6 | cell level.
+//   1. key per rect: Morton code of its centre, so each rect belongs to the
+//      quadtree cell holding its centre at every level (loose quadtree).  The
+//      key layout (code << 6 | level) also admits rects that stop at an inner
+//      cell; with centre keys every rect sits at the deepest level.
 //   2. one device radix sort of (key, rect id).  Sorted by key, the rects of
-//      any cell form one contiguous run, and the rects that straddle the
-//      cell's centre lines (they stop at that cell) come first in the run.
-//   3. level by level: every frontier node finds its own rects and its four
-//      child runs by binary search over the sorted keys; a node becomes a leaf
-//      when its run holds <= leaf_cap rects (or it is a unit cell).  Child
-//      groups are allocated by an exclusive scan, so the flat array is in
-//      breadth-first order and deterministic.
-//   4. each node's box = tight bounding box of its run (block reduction).
+//      any cell form one contiguous run.
+//   3. level by level: every frontier node splits its run into its four
+//      quadrant runs by binary search over the sorted keys; a node becomes a
+//      leaf when its run holds <= leaf_cap rects (or it is a unit cell).
+//      Child groups are allocated by an exclusive scan, so the flat array is
+//      in breadth-first order and deterministic.
+//   4. each node's box = tight bounding box of its run (block reduction); the
+//      boxes of siblings may overlap.
 // Query answers depend only on the rect set, never on this structure.
 #include <hipcub/hipcub.hpp>
 
@@ -51,23 +52,20 @@ __device__ inline uint64_t morton(uint32_t ux, uint32_t uy)
     return spread_bits(ux) | (spread_bits(uy) << 1);
 }
 
+// Key of a rect: Morton code of its doubled centre (x0 + x1, y0 + y1)
+// relative to the bounds corner (K bits per axis), tagged with the deepest
+// level K.  A rect belongs to the cell that holds its centre: no rect stays
+// behind in an internal node, and node boxes are the tight bounds of their
+// subtrees (a loose quadtree).
 __global__ void rect_keys(const int4* __restrict__ rects, int n, int4 B, int K,
                           uint64_t* __restrict__ keys, int32_t* __restrict__ vals)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int4 r = rects[i];
-    const uint64_t m0 = morton((uint32_t)(r.x - B.x), (uint32_t)(r.y - B.y));
-    const uint64_t m1 = morton((uint32_t)(r.z - B.x), (uint32_t)(r.w - B.y));
-    const uint64_t d = m0 ^ m1;
-    int level = K;
-    if (d) {
-        const int msb = 63 - __clzll((long long)d);
-        level = (2 * K - 1 - msb) / 2;
-    }
-    const int low = 2 * (K - level);
-    const uint64_t code = low >= 64 ? 0ull : (m0 >> low) << low;
-    keys[i] = (code << 6) | (uint64_t)level;
+    const uint32_t cx = (uint32_t)((long long)r.x - B.x + (long long)r.z - B.x);
+    const uint32_t cy = (uint32_t)((long long)r.y - B.y + (long long)r.w - B.y);
+    keys[i] = (morton(cx, cy) << 6) | (uint64_t)K;
     vals[i] = i;
 }
 
@@ -232,8 +230,9 @@ int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 B, int leaf_cap
     const int64_t W = (int64_t)B.z - B.x, H = (int64_t)B.w - B.y;
     int K = 0;
     while (K < 30 && ((int64_t)1 << K) <= std::max(W, H)) ++K;
-    if (((int64_t)1 << K) <= std::max(W, H) || K > 29)
-        return set_error(FRW_E_RANGE, "layout bounds wider than 2^29");
+    if (((int64_t)1 << K) <= std::max(W, H) || K > 28)
+        return set_error(FRW_E_RANGE, "layout bounds wider than 2^28");
+    K += 1;  // doubled centre coordinates take one more bit per axis
     t->morton_bits = K;
 
     hipEvent_t e0, e1;

@@ -41,22 +41,15 @@ void build(HostTree& t, const int32_t* xyxy, int n, const int32_t* B, int cap)
     const int64_t W = (int64_t)B[2] - B[0], H = (int64_t)B[3] - B[1];
     int K = 0;
     while (K < 30 && ((int64_t)1 << K) <= std::max(W, H)) ++K;
+    K += 1;  // doubled centre coordinates need one more bit
     t.K = K;
     std::vector<std::pair<uint64_t, int>> kv(n);
     for (int i = 0; i < n; ++i) {
         const int32_t* r = xyxy + 4 * i;
-        const uint64_t m0 = spread((uint32_t)(r[0] - B[0])) | (spread((uint32_t)(r[1] - B[1])) << 1);
-        const uint64_t m1 = spread((uint32_t)(r[2] - B[0])) | (spread((uint32_t)(r[3] - B[1])) << 1);
-        int level = K;
-        const uint64_t d = m0 ^ m1;
-        if (d) {
-            int msb = 63;
-            while (!((d >> msb) & 1ull)) --msb;
-            level = (2 * K - 1 - msb) / 2;
-        }
-        const int low = 2 * (K - level);
-        const uint64_t code = low >= 64 ? 0 : (m0 >> low) << low;
-        kv[i] = {(code << 6) | (uint64_t)level, i};
+        const uint32_t cx = (uint32_t)((int64_t)r[0] - B[0] + (int64_t)r[2] - B[0]);
+        const uint32_t cy = (uint32_t)((int64_t)r[1] - B[1] + (int64_t)r[3] - B[1]);
+        const uint64_t code = spread(cx) | (spread(cy) << 1);
+        kv[i] = {(code << 6) | (uint64_t)K, i};
     }
     std::stable_sort(kv.begin(), kv.end(),
                      [](const std::pair<uint64_t, int>& a, const std::pair<uint64_t, int>& b) { return a.first < b.first; });

@@ -58,10 +58,10 @@ def check_tree(lay, tree, leaf_cap):
             for c in range(4):
                 assert lvl[fc[ni] + c] == lvl[ni] + 1
                 stack.append((fc[ni] + c, bx))
-    assert seen == len(nodes)
-    # Morton cell containment: own rects lie in the node's cell and straddle
-    # its centre lines unless the node is a leaf.
-    B = lay.bounds()
+    assert seen == len(nodes) - 3   # the root's three empty sibling slots
+    # Morton cell containment (loose quadtree): every rect's doubled centre
+    # (x0 + x1, y0 + y1) lies in its node's cell; internal nodes own no rects.
+    B = lay.bounds().astype(np.int64)
     size = 1 << K
     code_x = np.zeros(len(nodes), dtype=np.int64)
     code_y = np.zeros(len(nodes), dtype=np.int64)
@@ -72,21 +72,18 @@ def check_tree(lay, tree, leaf_cap):
                 code_x[fc[ni] + c] = code_x[ni] + (half if c & 1 else 0)
                 code_y[fc[ni] + c] = code_y[ni] + (half if c & 2 else 0)
     for ni in range(len(nodes)):
+        if fc[ni] >= 0:
+            assert rb[ni] == re[ni], "internal node owns rects"
+            continue
         if rb[ni] == re[ni]:
             continue
         cs = size >> lvl[ni]
         r = trects[rb[ni]:re[ni]].astype(np.int64)
-        ux0, uy0, ux1, uy1 = r[:, 0] - B[0], r[:, 1] - B[1], r[:, 2] - B[0], r[:, 3] - B[1]
-        assert np.all(ux0 >= code_x[ni]) and np.all(ux1 < code_x[ni] + cs)
-        assert np.all(uy0 >= code_y[ni]) and np.all(uy1 < code_y[ni] + cs)
-        if fc[ni] >= 0:
-            mx, my = code_x[ni] + cs // 2, code_y[ni] + cs // 2
-            straddle = ((ux0 < mx) & (ux1 >= mx)) | ((uy0 < my) & (uy1 >= my))
-            assert np.all(straddle)
-        else:
-            assert re[ni] - rb[ni] <= leaf_cap or lvl[ni] >= K or np.all(
-                ((ux0 < code_x[ni] + cs // 2) & (ux1 >= code_x[ni] + cs // 2)) |
-                ((uy0 < code_y[ni] + cs // 2) & (uy1 >= code_y[ni] + cs // 2)))
+        cx = r[:, 0] + r[:, 2] - 2 * B[0]
+        cy = r[:, 1] + r[:, 3] - 2 * B[1]
+        assert np.all(cx >= code_x[ni]) and np.all(cx < code_x[ni] + cs)
+        assert np.all(cy >= code_y[ni]) and np.all(cy < code_y[ni] + cs)
+        assert re[ni] - rb[ni] <= leaf_cap or lvl[ni] >= K
     return info
 
 
