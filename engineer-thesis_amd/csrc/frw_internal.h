@@ -8,6 +8,7 @@
 
 #include "../../include/frw.h"
 #include "frw_device.h"
+#include "frw_mlp.h"
 #include "frw_types.h"
 
 namespace frw {
@@ -41,6 +42,26 @@ struct WalkLaunch {
     unsigned long long* work = nullptr;
     int* err = nullptr;
     unsigned long long* traffic = nullptr;
+    // exact-mode training-set harvest
+    double* h_pts = nullptr;
+    double* h_prev = nullptr;
+    double* h_dstar = nullptr;
+    unsigned long long* h_count = nullptr;
+    uint64_t h_cap = 0;
+    // learned mode
+    const MlpView* mlp = nullptr;
+    unsigned long long* l_stats = nullptr;
+};
+
+// Optional extras of a single-layout walk launch.
+struct WalkExtras {
+    double* h_pts = nullptr;
+    double* h_prev = nullptr;
+    double* h_dstar = nullptr;
+    unsigned long long* h_count = nullptr;
+    uint64_t h_cap = 0;
+    const MlpView* mlp = nullptr;
+    unsigned long long* l_stats = nullptr;
 };
 
 }  // namespace frw
@@ -103,6 +124,6 @@ int launch_gauss(const frw_tree* t, int4 master, double* d_out, hipStream_t s);
 int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds);
 int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s);
 int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o,
-                hipStream_t s, unsigned long long* traffic = nullptr);
+                hipStream_t s, unsigned long long* traffic = nullptr, const WalkExtras* extra = nullptr);
 
 }  // namespace frw

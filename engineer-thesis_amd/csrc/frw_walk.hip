@@ -12,14 +12,18 @@
 //   panel = largest i with T[i] <= u          (Green CDF thresholds in LDS)
 //   p     = getPointFromNindex(square, panel) (FP64, reference op order)
 //   p outside the strict bounds            -> walk escaped (hop not counted)
-//   exact mode:  d* = nearest-rect distance; d* <= 0 -> hit (Q2 for the id),
-//                else square = biggest square replayed from d*
-//   bisect mode: square = literal bisection (one Q1 per step), hit = Q2(p)
+//   exact mode:   d* = nearest-rect distance; d* <= 0 -> hit (Q2 for the id),
+//                 else square = biggest square replayed from d*
+//   bisect mode:  square = literal bisection (one Q1 per step), hit = Q2(p)
+//   learned mode: the wave's 64 points go through the proposal network
+//                 (bf16 MFMA); a collision-free Q1 probe of the proposed
+//                 square accepts it, otherwise the exact-mode step runs
 // Tallies: int64 hit counts per original rect id + escaped + stuck bins and
 // hop counts per job; optional per-walk outputs for replay checks.
 #include <algorithm>
 
 #include "frw_internal.h"
+#include "frw_mlp.h"
 
 namespace frw {
 namespace {
@@ -33,9 +37,19 @@ struct WalkArgs {
     uint64_t total_walks;
     uint32_t max_hops;
     int32_t n_jobs;
+    int32_t stack_cap;  // LDS stack entries per lane
+    // harvest: (point, previous half-size, d*) of non-final exact hops
+    double* h_pts;
+    double* h_prev;
+    double* h_dstar;
+    unsigned long long* h_count;
+    uint64_t h_cap;
+    // learned proposal
+    unsigned long long* l_stats;  // accepted, fallback
 };
 
-constexpr int kLdsJobs = 128;  // job walk offsets cached in LDS
+constexpr int kLdsJobs = 128;                                   // job walk offsets cached in LDS
+constexpr int kMlpLdsPerWave = 2 * 64 * kMlpStride * 2;         // two bf16 activation tiles
 
 __device__ inline int find_job(const WalkJob* jobs, const uint64_t* s_begin, int n_jobs, uint64_t g)
 {
@@ -68,8 +82,8 @@ __global__ void __launch_bounds__(64) walk_prologue(WalkJob* jobs, int n_jobs, u
     jb.surface[3] = s.y1;
 }
 
-template <int kMode, bool kStats>
-__global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a)
+template <int kMode, bool kStats, bool kHarvest>
+__global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpView mlp)
 {
     extern __shared__ int32_t stack_mem[];
     __shared__ uint32_t sT[kNSample];
@@ -77,8 +91,15 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
     for (int i = threadIdx.x; i < kNSample; i += blockDim.x) sT[i] = a.thresholds[i];
     for (int i = threadIdx.x; i < min(a.n_jobs, kLdsJobs); i += blockDim.x) s_begin[i] = jobs[i].walk_begin;
     __syncthreads();
-    const LaneStack st{stack_mem + threadIdx.x, (int)blockDim.x};
     const int lane = threadIdx.x & (kWaveSize - 1);
+    const LaneStack st{stack_mem + threadIdx.x, (int)blockDim.x};
+    uint16_t* mlp_xs = nullptr;
+    uint16_t* mlp_hs = nullptr;
+    if (kMode == kModeLearned) {
+        uint16_t* base = (uint16_t*)(stack_mem + (size_t)blockDim.x * a.stack_cap);
+        mlp_xs = base + (size_t)(threadIdx.x / kWaveSize) * (2 * 64 * kMlpStride);
+        mlp_hs = mlp_xs + 64 * kMlpStride;
+    }
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
 
     bool active = false, done = false;
@@ -91,7 +112,7 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
     unsigned long long my_hops = 0;  // hops accumulated for job j
     Counters cnt{0u, 0u};
     Counters* cp = kStats ? &cnt : nullptr;
-    unsigned long long my_nodes = 0, my_rects = 0;
+    unsigned long long my_nodes = 0, my_rects = 0, my_acc = 0, my_fb = 0;
 
     for (;;) {
         const bool need = !active && !done;
@@ -123,33 +144,77 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
             }
         }
         if (__ballot(active) == 0ull) break;
+
+        double px = 0.0, py = 0.0;
+        bool escaped = false;
+        if (active) {
+            if ((hop & 3u) == 0u) {
+                rnd[0] = jobs[j].first_walk + (uint32_t)w;
+                rnd[1] = jobs[j].job;
+                rnd[2] = hop >> 2;
+                rnd[3] = kWalkDomain;
+                philox4x32_10(rnd, k0, k1);
+            }
+            const int panel = sample_panel(sT, rnd[hop & 3u]);
+            point_from_panel(sq.x0, sq.y0, sq.x1, sq.y1, panel, px, py);
+            escaped = !in_bounds_strict(t.bounds, px, py);
+        }
+        // half-size of the square the point was sampled on
+        const double hprev = fmax(sq.x1 - sq.x0, sq.y1 - sq.y0) / 2;
+
+        float y_mlp = 0.0f;
+        double cell = 1.0;
+        if (kMode == kModeLearned) {
+            // all 64 lanes take part in the MFMA forward pass
+            uint64_t bits = 0;
+            int level = 0;
+            if (active && !escaped) {
+                level = mlp_level(mlp, hprev, &cell);
+                bits = mlp_window(mlp, level, cell, px, py);
+            }
+            mlp_feature_row(mlp_xs + lane * kMlpStride, bits, level);
+            y_mlp = mlp_forward_wave(mlp, mlp_xs, mlp_hs, lane);
+        }
         if (!active) continue;
 
-        if ((hop & 3u) == 0u) {
-            rnd[0] = jobs[j].first_walk + (uint32_t)w;
-            rnd[1] = jobs[j].job;
-            rnd[2] = hop >> 2;
-            rnd[3] = kWalkDomain;
-            philox4x32_10(rnd, k0, k1);
-        }
-        const uint32_t u = rnd[hop & 3u];
-        const int panel = sample_panel(sT, u);
-        double px, py;
-        point_from_panel(sq.x0, sq.y0, sq.x1, sq.y1, panel, px, py);
         int result = -3;  // walk continues
-        if (!in_bounds_strict(t.bounds, px, py)) {
+        if (escaped) {
             result = -1;
         } else {
-            if (kMode == kModeExact) {
-                const double ds = nearest_rect(t, st, px, py, cp);
-                if (ds <= 0.0)
-                    result = q2_point(t, st, px, py, cp);
-                else
-                    sq = biggest_square<false>(t, st, px, py, ds, nullptr, cp, a.err);
-            } else {
-                sq = biggest_square<true>(t, st, px, py, 0.0, nullptr, cp, a.err);
-                const int h = q2_point(t, st, px, py, cp);
-                if (h >= 0) result = h;
+            bool proposed = false;
+            if (kMode == kModeLearned) {
+                const double d = cell * exp2((double)y_mlp) * (double)mlp.alpha;
+                if (d > 2.0 && d < 1e300 &&
+                    !q1_box<false>(t, st, px - d, py - d, px + d, py + d, make_int4(0, 0, 0, 0), cp)) {
+                    sq = DBox{px - d, py - d, px + d, py + d};
+                    proposed = true;
+                    my_acc++;
+                } else {
+                    my_fb++;
+                }
+            }
+            if (!proposed) {
+                if (kMode != kModeBisect) {
+                    const double ds = nearest_rect(t, st, px, py, cp);
+                    if (ds <= 0.0) {
+                        result = q2_point(t, st, px, py, cp);
+                    } else {
+                        if (kHarvest) {
+                            const unsigned long long slot = atomicAdd(a.h_count, 1ull);
+                            if (slot < a.h_cap) {
+                                a.h_pts[2 * slot] = px;
+                                a.h_pts[2 * slot + 1] = py;
+                                a.h_prev[slot] = hprev;
+                                a.h_dstar[slot] = ds;
+                            }
+                        }
+                        sq = biggest_square<false>(t, st, px, py, ds, nullptr, cp, a.err);
+                    }
+                } else {
+                    sq = biggest_square<true>(t, st, px, py, 0.0, nullptr, cp, a.err);
+                    const int h = q2_point(t, st, px, py, cp);
+                    if (h >= 0) result = h;
+                }
             }
             if (kStats) {
                 my_nodes += cnt.nodes;
@@ -184,10 +249,27 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
             atomicAdd(&a.traffic[1], my_rects);
         }
     }
+    if (kMode == kModeLearned) {
+        for (int off = 32; off >= 1; off >>= 1) {
+            my_acc += __shfl_xor(my_acc, off);
+            my_fb += __shfl_xor(my_fb, off);
+        }
+        if (lane == 0 && a.l_stats) {
+            atomicAdd(&a.l_stats[0], my_acc);
+            atomicAdd(&a.l_stats[1], my_fb);
+        }
+    }
 }
 
 // Writes the single job of a plain walk into the tree's job slot.
 __global__ void set_job(WalkJob* slot, WalkJob jb) { *slot = jb; }
+
+template <int kMode, bool kStats, bool kHarvest>
+void launch_kernel(int grid, int block, size_t lds, hipStream_t s, const WalkJob* jobs, const WalkArgs& a,
+                   const MlpView& m)
+{
+    walk_kernel<kMode, kStats, kHarvest><<<grid, block, lds, s>>>(jobs, a, m);
+}
 
 }  // namespace
 
@@ -197,6 +279,7 @@ int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds)
     if (b < 0 || b % kWaveSize != 0 || b > 256)
         return set_error(FRW_E_INVALID, "block must be a multiple of 64 and <= 256");
     const size_t per_lane = (size_t)(3 * L.max_depth + 2) * sizeof(int32_t);
+    const size_t per_wave = L.mode == kModeLearned ? (size_t)kMlpLdsPerWave : 0;
     const size_t fixed = kNSample * 4 + kLdsJobs * 8;
     int lds_block = 0, lds_cu = 0, cus = 0;
     hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, L.device);
@@ -204,21 +287,22 @@ int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds)
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, L.device);
     if (lds_block <= 0) lds_block = 64 * 1024;
     if (lds_cu <= 0) lds_cu = lds_block;
+    auto need = [&](int bb) { return (size_t)bb * per_lane + (size_t)(bb / kWaveSize) * per_wave; };
     if (b == 0) {
         b = 256;
-        while (b >= 64 && (size_t)b * per_lane + fixed > (size_t)lds_block) b -= 64;
+        while (b >= 64 && need(b) + fixed > (size_t)lds_block) b -= 64;
     }
-    if (b < 64 || (size_t)b * per_lane + fixed > (size_t)lds_block)
+    if (b < 64 || need(b) + fixed > (size_t)lds_block)
         return set_error(FRW_E_RANGE, "tree too deep for the LDS traversal stack at this block size");
     *block = b;
-    *lds = (size_t)b * per_lane;
+    *lds = need(b);
     int g = L.grid;
     if (g <= 0) {
         const int per_cu_lds = (int)((size_t)lds_cu / (*lds + fixed));
         const int per_cu = std::max(1, std::min(per_cu_lds, 2048 / b));
         g = std::max(1, cus * per_cu);
-        const int64_t need = ((int64_t)L.total_walks + b - 1) / b;
-        if (need < g) g = (int)need;
+        const int64_t want = ((int64_t)L.total_walks + b - 1) / b;
+        if (want < g) g = (int)want;
     }
     *grid = g;
     return FRW_OK;
@@ -234,7 +318,7 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
     const size_t plds = (size_t)64 * (3 * L.max_depth + 2) * sizeof(int32_t);
     walk_prologue<<<(L.n_jobs + 63) / 64, 64, plds, s>>>(d_jobs, L.n_jobs, L.work);
     if ((rc = check_hip(hipGetLastError(), "walk_prologue")) != FRW_OK) return rc;
-    WalkArgs a;
+    WalkArgs a{};
     a.thresholds = L.thresholds;
     a.work = L.work;
     a.err = L.err;
@@ -243,22 +327,34 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
     a.total_walks = L.total_walks;
     a.max_hops = L.max_hops;
     a.n_jobs = L.n_jobs;
-    if (L.traffic) {
-        if (L.mode == kModeBisect)
-            walk_kernel<kModeBisect, true><<<grid, block, lds, s>>>(d_jobs, a);
-        else
-            walk_kernel<kModeExact, true><<<grid, block, lds, s>>>(d_jobs, a);
+    a.stack_cap = 3 * L.max_depth + 2;
+    a.h_pts = L.h_pts;
+    a.h_prev = L.h_prev;
+    a.h_dstar = L.h_dstar;
+    a.h_count = L.h_count;
+    a.h_cap = L.h_cap;
+    a.l_stats = L.l_stats;
+    MlpView m{};
+    if (L.mlp) m = *L.mlp;
+    const bool st = L.traffic != nullptr;
+    if (L.mode == kModeLearned) {
+        if (!L.mlp) return set_error(FRW_E_INVALID, "learned mode needs a network");
+        if (st) launch_kernel<kModeLearned, true, false>(grid, block, lds, s, d_jobs, a, m);
+        else launch_kernel<kModeLearned, false, false>(grid, block, lds, s, d_jobs, a, m);
+    } else if (L.mode == kModeBisect) {
+        if (st) launch_kernel<kModeBisect, true, false>(grid, block, lds, s, d_jobs, a, m);
+        else launch_kernel<kModeBisect, false, false>(grid, block, lds, s, d_jobs, a, m);
+    } else if (L.h_count) {
+        launch_kernel<kModeExact, false, true>(grid, block, lds, s, d_jobs, a, m);
     } else {
-        if (L.mode == kModeBisect)
-            walk_kernel<kModeBisect, false><<<grid, block, lds, s>>>(d_jobs, a);
-        else
-            walk_kernel<kModeExact, false><<<grid, block, lds, s>>>(d_jobs, a);
+        if (st) launch_kernel<kModeExact, true, false>(grid, block, lds, s, d_jobs, a, m);
+        else launch_kernel<kModeExact, false, false>(grid, block, lds, s, d_jobs, a, m);
     }
     return check_hip(hipGetLastError(), "walk_kernel");
 }
 
 int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o,
-                hipStream_t s, unsigned long long* traffic)
+                hipStream_t s, unsigned long long* traffic, const WalkExtras* extra)
 {
     if (p->n_walks == 0) return FRW_OK;
     WalkJob jb{};
@@ -289,6 +385,15 @@ int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outp
     L.work = t->d_work;
     L.err = t->d_err;
     L.traffic = traffic;
+    if (extra) {
+        L.h_pts = extra->h_pts;
+        L.h_prev = extra->h_prev;
+        L.h_dstar = extra->h_dstar;
+        L.h_count = extra->h_count;
+        L.h_cap = extra->h_cap;
+        L.mlp = extra->mlp;
+        L.l_stats = extra->l_stats;
+    }
     return launch_walk_jobs(t->d_job, L, s);
 }
 

@@ -82,6 +82,10 @@ EXPORTS = [
     "frw_gauss_surface", "frw_walk_async", "frw_walk", "frw_walk_traffic", "frw_write_results",
     "frw_batch_create", "frw_batch_counts_size", "frw_batch_counts_offset", "frw_batch_walk",
     "frw_batch_autotune", "frw_batch_check", "frw_batch_free",
+    "frw_mlp_create", "frw_mlp_free", "frw_mlp_param_count", "frw_mlp_get_params",
+    "frw_mlp_set_params", "frw_mlp_occupancy", "frw_mlp_features", "frw_mlp_forward",
+    "frw_mlp_gradient", "frw_mlp_set_samples", "frw_mlp_train", "frw_walk_harvest",
+    "frw_walk_learned",
 ]
 
 _lib = None
@@ -141,6 +145,20 @@ def lib():
     L.frw_batch_check.argtypes = [vp]
     L.frw_batch_free.argtypes = [vp]
     L.frw_batch_free.restype = None
+    L.frw_mlp_create.argtypes = [vp, u64, C.POINTER(vp)]
+    L.frw_mlp_free.argtypes = [vp]
+    L.frw_mlp_free.restype = None
+    L.frw_mlp_get_params.argtypes = [vp, vp, i32]
+    L.frw_mlp_set_params.argtypes = [vp, vp, i32]
+    L.frw_mlp_occupancy.argtypes = [vp, i32, vp, i64]
+    L.frw_mlp_features.argtypes = [vp, vp, vp, i64, vp, vp]
+    L.frw_mlp_forward.argtypes = [vp, vp, i64, vp, vp]
+    L.frw_mlp_gradient.argtypes = [vp, vp, vp, i64, vp, C.POINTER(C.c_float)]
+    L.frw_mlp_set_samples.argtypes = [vp, vp, vp, vp, i64]
+    L.frw_mlp_train.argtypes = [vp, i32, i32, C.c_float, u64, C.POINTER(C.c_float)]
+    L.frw_walk_harvest.argtypes = [vp, C.POINTER(WalkParams), vp, vp, vp, i64, C.POINTER(i64)]
+    L.frw_walk_learned.argtypes = [vp, vp, C.POINTER(WalkParams), C.POINTER(WalkOutputs),
+                                   C.c_float, vp]
     _lib = L
     return L
 
@@ -351,3 +369,86 @@ class Batch:
                 self.h = None
         except Exception:
             pass
+
+
+class Mlp:
+    """Learned square-size proposal bound to one tree (occupancy pyramid +
+    65-64-64-64-1 network)."""
+
+    def __init__(self, tree, seed=1):
+        self.tree = tree
+        h = C.c_void_p()
+        _check(lib().frw_mlp_create(tree.h, seed, C.byref(h)))
+        self.h = h
+        self.n_params = lib().frw_mlp_param_count()
+
+    def params(self):
+        p = np.zeros(self.n_params, dtype=np.float32)
+        _check(lib().frw_mlp_get_params(self.h, p.ctypes.data, self.n_params))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().frw_mlp_set_params(self.h, p.ctypes.data, self.n_params))
+
+    def occupancy(self, level):
+        n = (1 << level) * (1 << level)
+        words = np.zeros((n + 31) // 32, dtype=np.uint32)
+        _check(lib().frw_mlp_occupancy(self.h, level, words.ctypes.data, len(words)))
+        bits = np.unpackbits(words.view(np.uint8), bitorder="little")[:n]
+        return bits.reshape(1 << level, 1 << level)   # [j][i]
+
+    def features(self, pts, h, out, stream=None):
+        _check(lib().frw_mlp_features(self.h, pts.data_ptr(), h.data_ptr(), pts.shape[0],
+                                      out.data_ptr(), stream))
+
+    def forward(self, X, y, stream=None):
+        _check(lib().frw_mlp_forward(self.h, X.data_ptr(), X.shape[0], y.data_ptr(), stream))
+
+    def gradient(self, X, t):
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().frw_mlp_gradient(self.h, X.data_ptr(), t.data_ptr(), X.shape[0], g.ctypes.data,
+                                      C.byref(loss)))
+        return g, loss.value
+
+    def set_samples(self, pts, h, dstar):
+        _check(lib().frw_mlp_set_samples(self.h, pts.data_ptr(), h.data_ptr(), dstar.data_ptr(),
+                                         pts.shape[0]))
+
+    def train(self, epochs=2, batch=1024, lr=1e-3, seed=1):
+        loss = C.c_float()
+        _check(lib().frw_mlp_train(self.h, epochs, batch, lr, seed, C.byref(loss)))
+        return loss.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().frw_mlp_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def walk_harvest(tree, master, n_walks, pts, h, dstar, seed=1, first_walk=0, job=0):
+    """Exact walks that record (point, previous half-size, d*) of non-final hops
+    into device tensors; returns the number of records."""
+    p = WalkParams(master=master, mode=MODE_EXACT, n_walks=n_walks, first_walk=first_walk, seed=seed,
+                   job=job, block=0, grid=0, reserved=0, max_hops=0)
+    n = C.c_int64()
+    _check(lib().frw_walk_harvest(tree.h, C.byref(p), pts.data_ptr(), h.data_ptr(), dstar.data_ptr(),
+                                  pts.shape[0], C.byref(n)))
+    return n.value
+
+
+def walk_learned(tree, mlp, master, n_walks, counts, hops, hit_out=None, hops_out=None, seed=1,
+                 first_walk=0, job=0, alpha=0.8, block=0, grid=0):
+    """Walks with the learned proposal; returns (accepted, fallback) counts."""
+    p = WalkParams(master=master, mode=MODE_LEARNED, n_walks=n_walks, first_walk=first_walk,
+                   seed=seed, job=job, block=block, grid=grid, reserved=0, max_hops=0)
+    o = WalkOutputs(counts.data_ptr(), hops.data_ptr(),
+                    hit_out.data_ptr() if hit_out is not None else None,
+                    hops_out.data_ptr() if hops_out is not None else None)
+    st = np.zeros(2, dtype=np.int64)
+    _check(lib().frw_walk_learned(tree.h, mlp.h, C.byref(p), C.byref(o), alpha, st.ctypes.data))
+    return int(st[0]), int(st[1])

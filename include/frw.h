@@ -237,6 +237,51 @@ int frw_batch_autotune(frw_batch* b, const frw_batch_params* base, frw_batch_par
 int frw_batch_check(const frw_batch* b);
 void frw_batch_free(frw_batch* b);
 
+/* ---------------------------------------------------------------- learned proposal
+ * A small network that proposes the half-size of each hop's square; one Q1
+ * probe verifies the proposal and a failed probe falls back to the exact
+ * search.  No reference counterpart (the reference always bisects,
+ * randomWalk/CPU/quadTree.cpp:264-312).  Features: the 8x8 occupancy window
+ * around the point at a scale set by the previous square, plus the level;
+ * network 65-64-64-64-1, GELU, bf16 weights, FP32 accumulation (MFMA). */
+
+typedef struct frw_mlp frw_mlp;
+
+/* Occupancy pyramid of the tree's layout and Philox-initialised weights.
+ * The tree must outlive the network. */
+int frw_mlp_create(const frw_tree* t, uint64_t seed, frw_mlp** out);
+void frw_mlp_free(frw_mlp* m);
+/* Number of FP32 parameters: W1[64][72] b1 W2[64][64] b2 W3[64][64] b3 w4[64] b4. */
+int frw_mlp_param_count(void);
+int frw_mlp_get_params(const frw_mlp* m, float* out, int32_t n);
+int frw_mlp_set_params(frw_mlp* m, const float* in, int32_t n);
+/* Copies one occupancy level (2^l x 2^l bits, row-major, 32-bit words). */
+int frw_mlp_occupancy(const frw_mlp* m, int32_t level, uint32_t* out_words, int64_t n_words);
+/* Device arrays: pts n x 2, h n (previous half-size) -> feats n x 72 (FP32). */
+int frw_mlp_features(const frw_mlp* m, const double* pts, const double* h, int64_t n, float* feats,
+                     void* stream);
+/* Device arrays: X n x 72 -> y n, the MFMA inference path of the walk. */
+int frw_mlp_forward(const frw_mlp* m, const float* X, int64_t n, float* y, void* stream);
+/* One forward + backward pass over device X (n x 72) and targets t (n, a
+ * multiple of 32); grad (host, param_count) = gradient of mean((y-t)^2)/2. */
+int frw_mlp_gradient(frw_mlp* m, const float* X, const float* t, int64_t n, float* grad, float* loss);
+/* Training set from device arrays (pts n x 2, h n, d* n): features and
+ * targets log2(d* over cell) are computed on the device. */
+int frw_mlp_set_samples(frw_mlp* m, const double* pts, const double* h, const double* dstar, int64_t n);
+/* Adam over the training set (batch a multiple of 32). */
+int frw_mlp_train(frw_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* last_loss);
+/* Runs exact walks and records, for up to `capacity` hops that did not end
+ * the walk, the point, the previous half-size and d* (device arrays pts
+ * capacity x 2, h capacity, dstar capacity); *n_out = records written. */
+int frw_walk_harvest(const frw_tree* t, const frw_walk_params* p, double* pts, double* h, double* dstar,
+                     int64_t capacity, int64_t* n_out);
+/* Walks with the learned proposal (FRW_MODE_LEARNED semantics): per hop the
+ * network proposes alpha * s * 2^y; a collision-free Q1 probe accepts it,
+ * otherwise the exact search decides the square.  stats (host, may be NULL)
+ * = {accepted proposals, fallbacks}.  Synchronous. */
+int frw_walk_learned(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p,
+                     const frw_walk_outputs* o, float alpha, int64_t stats[2]);
+
 /* Measurement hook (no reference counterpart): the same walks as frw_walk
  * with per-hop traffic counters compiled in.  out = {node records read
  * (32 B each), rect records read (16 B each), hops}.  Slower than frw_walk;

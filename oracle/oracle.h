@@ -116,6 +116,15 @@ int64_t orc_walk_many(const orc_index* ix, int master, uint64_t seed, uint32_t j
                       int64_t* counts, int32_t* hit_out, int32_t* hops_out,
                       int nthreads);
 
+/* ---- learned square-size proposal (orc_mlp.c) -------------------------- */
+float orc_bf16(float x);
+int   orc_mlp_level(const orc_box* bounds, double h, double* cell);
+void  orc_mlp_features(const double* xyxy, int n, const orc_box* bounds, double px, double py,
+                       double h, float out[65]);
+int   orc_mlp_param_count(void);
+void  orc_mlp_forward(const float* params, const float* X, int B, float* y);
+float orc_mlp_backward(const float* params, const float* X, const float* t, int B, float* grad);
+
 #ifdef __cplusplus
 }
 #endif

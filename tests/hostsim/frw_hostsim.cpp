@@ -196,6 +196,26 @@ void hs_queries(void* h, const double* pts, int n, double* dist, int32_t* q2, do
     }
 }
 
+// Traffic of the nearest-rect search and of a Q1 probe of the box p +- f*d*
+// (per point: nodes/rects of each), for sizing a proposal-and-verify step.
+void hs_probe_cost(void* h, const double* pts, int n, double f, int64_t* out)
+{
+    const HostTree* t = static_cast<HostTree*>(h);
+    const DevTree v = view(*t);
+    std::vector<int32_t> stack(v.stack_cap + 8);
+    LaneStack st{stack.data(), 1};
+    int64_t nn = 0, nr = 0, pn = 0, pr = 0, ok = 0;
+    for (int i = 0; i < n; ++i) {
+        const double px = pts[2 * i], py = pts[2 * i + 1];
+        Counters a{0, 0}, b{0, 0};
+        const double d = nearest_rect(v, st, px, py, &a);
+        const double hh = f * d;
+        const bool hit = q1_box<false>(v, st, px - hh, py - hh, px + hh, py + hh, make_int4(0, 0, 0, 0), &b);
+        nn += a.nodes; nr += a.rects; pn += b.nodes; pr += b.rects; ok += hit ? 0 : 1;
+    }
+    out[0] = nn; out[1] = nr; out[2] = pn; out[3] = pr; out[4] = ok;
+}
+
 void hs_gauss(void* h, int master, double* out)
 {
     const HostTree* t = static_cast<HostTree*>(h);
