@@ -1,16 +1,30 @@
-// Learned square-size proposal: occupancy-window features and a 65-64-64-64-1
+// Learned square-size proposal: occupancy-window features and a 67-64-64-64-1
 // GELU network evaluated with bf16 MFMA (v_mfma_f32_32x32x8_bf16) and FP32
-// accumulation.  No counterpart in the reference: the reference sizes every
-// square by bisection (randomWalk/CPU/quadTree.cpp:264-312).
+// accumulation, and the occupancy-pyramid probe that verifies a proposal.
+// No counterpart in the reference: the reference sizes every square by
+// bisection (randomWalk/CPU/quadTree.cpp:264-312).
+//
+// Occupancy pyramid: level l is a 2^l x 2^l grid of cells of side
+// s_l = 2^(K0-l) from the bounds corner (K0 = smallest k with 2^k > max(W, H));
+// a cell is occupied when some closed rect meets the half-open cell.  Levels
+// are stored as row-major bitmaps (features) and, up to level Vmax, as
+// summed-area tables of (2^l + 1)^2 uint32 counts with a zero first row and
+// column (verification).
 //
 // Features of a point p whose previous square had half-size h:
-//   K0 = smallest k with 2^k > max(W, H) of the bounds, Lmax = min(K0, 12),
-//   level l = clamp(K0 - floor(log2 max(h, 1)) + 1, 0, Lmax), cell s = 2^(K0-l),
-//   (ci, cj) = floor((p - bounds corner) / s);
+//   Lmax = min(K0, 12), level l = clamp(K0 - floor(log2 max(h, 1)) + 1, 0, Lmax),
+//   cell s = s_l, (ci, cj) = floor((p - bounds corner) / s);
 //   f[v*8+u] = occupancy of cell (ci-4+u, cj-4+v) at level l (1 outside the
-//   grid), f[64] = l / 16, f[65..71] = 0.
-// Occupancy bit of a cell: some closed rect meets the half-open cell.
-// Output y estimates log2(d* / s); the proposed half-size is s * 2^y * alpha.
+//   grid), f[64] = l / 16, f[65], f[66] = fractional position of p in its
+//   cell (x, y), f[67..71] = 0; every feature is rounded to bf16.
+// Output y estimates log2(d* / s); the proposed half-size is d = s 2^y alpha.
+//
+// Verification of the square [p - d, p + d]^2 (closed, as the walk builds it):
+// level v = clamp(K0 - floor(log2 d) + 3, 0, Vmax), so cells are at most d/8;
+// the square is certified empty when it lies inside the bounds and the cell
+// block covering it, floor((corner - bounds corner) / s_v) per axis, holds no
+// occupied cell (four loads from the summed-area table).  Any rect point in
+// the square lies in a cell of that block, so a certified square is free.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,11 +34,13 @@
 
 namespace frw {
 
-constexpr int kMlpIn = 65;
+constexpr int kMlpIn = 67;   // features (window 64, level, position in cell x and y)
 constexpr int kMlpInPad = 72;   // K of the first layer, a multiple of 8
 constexpr int kMlpHid = 64;
 constexpr int kMlpStride = 72;  // LDS row stride (bf16 elements) of activations
-constexpr int kMlpMaxLevel = 12;
+constexpr int kMlpMaxLevel = 12;  // feature-window levels
+constexpr int kPyrMaxLevel = 15;  // bitmap / summed-area-table levels
+constexpr int kCertShift = 3;     // verification cells are at most d / 2^3
 
 // FP32 parameter block layout (row-major [out][in]).
 constexpr int kOffW1 = 0;
@@ -50,11 +66,14 @@ struct MlpView {
     const float* b3;
     float b4;
     float alpha;          // safety factor on the proposal
-    const uint32_t* occ;  // occupancy pyramid, level l at word occ_off[l]
-    int64_t occ_off[kMlpMaxLevel + 1];
+    const uint32_t* occ;  // occupancy bitmaps, level l at word occ_off[l]
+    int64_t occ_off[kPyrMaxLevel + 1];
+    const uint32_t* sat;  // summed-area tables, level l at entry sat_off[l]
+    int64_t sat_off[kPyrMaxLevel + 1];
     int4 bounds;
     int K0;
     int Lmax;
+    int Vmax;
 };
 
 __host__ __device__ inline uint16_t f2bf(float x)
@@ -85,36 +104,76 @@ __device__ inline int mlp_level(const MlpView& m, double h, double* cell)
     return l;
 }
 
-// 64 occupancy bits of the 8x8 window (bit v*8+u) at level l around p.
-__device__ inline uint64_t mlp_window(const MlpView& m, int l, double cell, double px, double py)
+// 64 occupancy bits of the 8x8 window (bit v*8+u) at level l around p.  Each
+// window row is one unaligned 8-bit field of the level's row-major bitmap,
+// read as two words (every level carries one padding word at its end).
+__device__ inline uint64_t mlp_window(const MlpView& m, int l, double cell, double px, double py, float* frac_x,
+                                      float* frac_y)
 {
     const int64_t n = (int64_t)1 << l;
-    const int64_t ci = (int64_t)floor((px - (double)m.bounds.x) / cell);
-    const int64_t cj = (int64_t)floor((py - (double)m.bounds.y) / cell);
+    const double qx = (px - (double)m.bounds.x) / cell;
+    const double qy = (py - (double)m.bounds.y) / cell;
+    const double fx = floor(qx);
+    const double fy = floor(qy);
+    *frac_x = 0.0f;
+    *frac_y = 0.0f;
+    // far outside the grid (or not a number): every window cell reads as occupied
+    if (!(fx > -16.0 && fy > -16.0 && fx < (double)(n + 16) && fy < (double)(n + 16))) return ~0ull;
+    *frac_x = (float)(qx - fx);
+    *frac_y = (float)(qy - fy);
+    const int64_t ci = (int64_t)fx;
+    const int64_t cj = (int64_t)fy;
     const uint32_t* base = m.occ + m.occ_off[l];
+    const int64_t i0 = ci - 4;
+    const int64_t lo = i0 > 0 ? i0 : 0;
+    const int64_t hi = i0 + 8 < n ? i0 + 8 : n;
+    uint32_t inside = 0;  // window columns inside the grid
+    if (lo < hi) inside = ((1u << (hi - lo)) - 1u) << (lo - i0);
     uint64_t bits = 0;
+#pragma unroll
     for (int v = 0; v < 8; ++v) {
         const int64_t j = cj - 4 + v;
-        for (int u = 0; u < 8; ++u) {
-            const int64_t i = ci - 4 + u;
-            uint64_t b = 1;
-            if (i >= 0 && j >= 0 && i < n && j < n) {
-                const int64_t idx = j * n + i;
-                b = (base[idx >> 5] >> (idx & 31)) & 1u;
-            }
-            bits |= b << (v * 8 + u);
+        uint32_t row = 0xFFu;
+        if (inside && j >= 0 && j < n) {
+            const int64_t idx = j * n + lo;
+            const uint64_t w = ((uint64_t)base[(idx >> 5) + 1] << 32) | base[idx >> 5];
+            const uint32_t field = (uint32_t)(w >> (idx & 31)) & ((1u << (hi - lo)) - 1u);
+            row = (field << (lo - i0)) | (~inside & 0xFFu);
         }
+        bits |= (uint64_t)row << (v * 8);
     }
     return bits;
 }
 
 // Writes one feature row (72 bf16) of this lane's point into LDS.
-__device__ inline void mlp_feature_row(uint16_t* row, uint64_t bits, int level)
+__device__ inline void mlp_feature_row(uint16_t* row, uint64_t bits, int level, float frac_x, float frac_y)
 {
     const uint16_t one = 0x3F80u;  // bf16(1.0)
     for (int k = 0; k < 64; ++k) row[k] = ((bits >> k) & 1u) ? one : (uint16_t)0;
     row[64] = f2bf((float)level / 16.0f);
-    for (int k = 65; k < kMlpInPad; ++k) row[k] = 0;
+    row[65] = f2bf(frac_x);
+    row[66] = f2bf(frac_y);
+    for (int k = 67; k < kMlpInPad; ++k) row[k] = 0;
+}
+
+// Verification probe: true when the closed square (x0, y0, x1, y1) of
+// half-size d (> 0) lies in the bounds and its covering cell block at level
+// clamp(K0 - floor(log2 d) + 3, 0, Vmax) holds no occupied cell.
+__device__ inline bool mlp_certify(const MlpView& m, double x0, double y0, double x1, double y1, double d)
+{
+    if (!in_bounds_box(m.bounds, x0, y0, x1, y1)) return false;
+    int v = m.K0 - ilogb(d) + kCertShift;
+    v = v < 0 ? 0 : (v > m.Vmax ? m.Vmax : v);
+    const double s = ldexp(1.0, m.K0 - v);
+    const int64_t n = (int64_t)1 << v;
+    const double a0 = floor((x0 - (double)m.bounds.x) / s), a1 = floor((x1 - (double)m.bounds.x) / s);
+    const double b0 = floor((y0 - (double)m.bounds.y) / s), b1 = floor((y1 - (double)m.bounds.y) / s);
+    if (!(a0 >= 0.0 && b0 >= 0.0 && a1 < (double)n && b1 < (double)n)) return false;
+    const int64_t i0 = (int64_t)a0, i1 = (int64_t)a1 + 1, j0 = (int64_t)b0, j1 = (int64_t)b1 + 1;
+    const uint32_t* S = m.sat + m.sat_off[v];
+    const int64_t w = n + 1;
+    const uint32_t occ = S[j1 * w + i1] - S[j0 * w + i1] - S[j1 * w + i0] + S[j0 * w + i0];
+    return occ == 0u;
 }
 
 // Orders this wave's LDS accesses (the other lanes' rows) without a block barrier.
@@ -126,32 +185,32 @@ __device__ inline void wave_sync()
 }
 
 // One hidden layer for the wave's 64 rows: out = bf16(gelu(in * W^T + b)).
-// 2x2 tiles of 32x32, K/8 MFMA steps each.  Lane l feeds row/column l%32 and
-// k-quad l/32; its 16 results sit at rows 8*(r/4) + 4*(l/32) + r%4.
+// Four 32x32 output tiles, one at a time (one 16-register accumulator live),
+// K/8 MFMA steps each.  Lane l feeds row/column l%32 and k-quad l/32; its 16
+// results sit at rows 8*(r/4) + 4*(l/32) + r%4 of column l%32.
 template <int K>
 __device__ inline void mlp_layer(const uint16_t* in, uint16_t* out, const uint16_t* W, const float* bias,
                                  int lane)
 {
     const int r32 = lane & 31, q = lane >> 5;
+#pragma unroll 1
+    for (int t = 0; t < 4; ++t) {
+        const int mi = t >> 1, nj = t & 1;
+        const uint16_t* arow = in + (mi * 32 + r32) * kMlpStride + 4 * q;
+        const uint16_t* brow = W + (nj * 32 + r32) * K + 4 * q;
+        f32x16 acc = {0};
 #pragma unroll
-    for (int mi = 0; mi < 2; ++mi) {
+        for (int kb = 0; kb < K / 8; ++kb) {
+            const bf16x4 a = *(const bf16x4*)(arow + kb * 8);
+            const bf16x4 b = *(const bf16x4*)(brow + kb * 8);
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        }
+        const int col = nj * 32 + r32;
+        const float bb = bias[col];
 #pragma unroll
-        for (int nj = 0; nj < 2; ++nj) {
-            f32x16 acc = {0};
-#pragma unroll
-            for (int kb = 0; kb < K / 8; ++kb) {
-                const int kk = kb * 8 + 4 * q;
-                const bf16x4 a = *(const bf16x4*)(in + (mi * 32 + r32) * kMlpStride + kk);
-                const bf16x4 b = *(const bf16x4*)(W + (nj * 32 + r32) * K + kk);
-                acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
-            }
-            const int col = nj * 32 + r32;
-            const float bb = bias[col];
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = mi * 32 + 8 * (r >> 2) + 4 * q + (r & 3);
-                out[row * kMlpStride + col] = f2bf(gelu_f(acc[r] + bb));
-            }
+        for (int r = 0; r < 16; ++r) {
+            const int row = mi * 32 + 8 * (r >> 2) + 4 * q + (r & 3);
+            out[row * kMlpStride + col] = f2bf(gelu_f(acc[r] + bb));
         }
     }
 }

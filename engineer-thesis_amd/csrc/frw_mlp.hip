@@ -14,9 +14,11 @@
 struct frw_mlp {
     const frw_tree* tree = nullptr;
     int device = 0;
-    int K0 = 0, Lmax = 0;
-    std::vector<int64_t> occ_off;
+    int K0 = 0, Lmax = 0, Vmax = 0, Pmax = 0;  // Pmax = max(Lmax, Vmax)
+    std::vector<int64_t> occ_off;  // bitmap word offsets, levels 0..Pmax
+    std::vector<int64_t> sat_off;  // summed-area-table entry offsets, levels 0..Vmax
     uint32_t* d_occ = nullptr;
+    uint32_t* d_sat = nullptr;
     float* d_params = nullptr;     // FP32 master parameters
     uint16_t* d_bf = nullptr;      // bf16 copy of W1 | W2 | W3 | w4
     float* d_adam_m = nullptr;
@@ -24,6 +26,7 @@ struct frw_mlp {
     float* d_grad = nullptr;
     float* d_gs = nullptr;         // weight-gradient scratch, 64 x 72
     int64_t adam_step = 0;
+    float h_b4 = 0.0f;             // host copy of the output bias (kept in sync)
     // training set
     int64_t n_samples = 0;
     uint16_t* d_X = nullptr;       // [n][72] bf16
@@ -47,6 +50,8 @@ constexpr int kBfW4 = kBfW3 + kMlpHid * kMlpHid;
 constexpr int kBfCount = kBfW4 + kMlpHid;
 
 // ---------------------------------------------------------------- occupancy
+// One lane per rect: sets the bits of every cell the closed rect meets, one
+// word mask per bitmap row segment.
 __global__ void occ_mark(const int4* __restrict__ rects, int n, int4 B, int K0, int Lmax,
                          const int64_t* __restrict__ off, uint32_t* __restrict__ occ)
 {
@@ -61,12 +66,79 @@ __global__ void occ_mark(const int4* __restrict__ rects, int n, int4 B, int K0, 
         const int64_t i0 = x0 >> sh, i1 = min(x1 >> sh, nl - 1);
         const int64_t j0 = y0 >> sh, j1 = min(y1 >> sh, nl - 1);
         uint32_t* base = occ + off[l];
-        for (int64_t j = j0; j <= j1; ++j)
-            for (int64_t i = i0; i <= i1; ++i) {
-                const int64_t idx = j * nl + i;
-                atomicOr(base + (idx >> 5), 1u << (idx & 31));
+        for (int64_t j = j0; j <= j1; ++j) {
+            int64_t a = j * nl + i0;
+            const int64_t e = j * nl + i1;  // inclusive
+            while (a <= e) {
+                const int64_t wd = a >> 5;
+                const int b0 = (int)(a & 31);
+                const int b1 = (int)min<int64_t>(31, b0 + (e - a));
+                const uint32_t mask = (b1 == 31 ? 0xFFFFFFFFu : ((1u << (b1 + 1)) - 1u)) & ~((1u << b0) - 1u);
+                atomicOr(base + wd, mask);
+                a += b1 - b0 + 1;
             }
+        }
     }
+}
+
+// ---------------------------------------------------------------- summed-area tables
+// Row pass, one block of 256 lanes per grid row j: S[(j+1)(n+1) + i + 1] =
+// occupied cells (a, j) with a <= i; S[(j+1)(n+1)] = 0.  Row 0 is zeroed by
+// the caller.
+__global__ void __launch_bounds__(256) sat_rows(const uint32_t* __restrict__ bits, int64_t n,
+                                                uint32_t* __restrict__ S)
+{
+    __shared__ uint32_t part[256];
+    const int64_t j = blockIdx.x;
+    const int t = threadIdx.x;
+    const int64_t c = (n + 255) / 256;
+    const int64_t a = min(n, (int64_t)t * c), b = min(n, a + c);
+    auto bit = [&](int64_t i) -> uint32_t {
+        const int64_t idx = j * n + i;
+        return (bits[idx >> 5] >> (idx & 31)) & 1u;
+    };
+    uint32_t cnt = 0;
+    for (int64_t i = a; i < b; ++i) cnt += bit(i);
+    part[t] = cnt;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - cnt;
+    uint32_t* row = S + (j + 1) * (n + 1);
+    if (t == 0) row[0] = 0u;
+    for (int64_t i = a; i < b; ++i) {
+        run += bit(i);
+        row[i + 1] = run;
+    }
+}
+
+// Column pass: one lane per column, running sums down the rows.
+__global__ void sat_cols(int64_t n, uint32_t* __restrict__ S)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    uint32_t run = 0;
+    for (int64_t j = 1; j <= n; ++j) {
+        uint32_t* e = S + j * (n + 1) + i;
+        run += *e;
+        *e = run;
+    }
+}
+
+// Verification probe for host-supplied squares (test hook).
+__global__ void certify_kernel(MlpView m, const double* __restrict__ pts, const double* __restrict__ d,
+                               int64_t n, uint8_t* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double px = pts[2 * i], py = pts[2 * i + 1], h = d[i];
+    bool ok = false;
+    if (h > 0.0 && h < 1e300) ok = mlp_certify(m, px - h, py - h, px + h, py + h, h);
+    out[i] = ok ? 1 : 0;
 }
 
 // ---------------------------------------------------------------- features
@@ -80,11 +152,13 @@ __global__ void features_kernel(MlpView m, const double* __restrict__ pts, const
     const double px = pts[2 * i], py = pts[2 * i + 1];
     double cell;
     const int l = mlp_level(m, hs[i], &cell);
-    const uint64_t bits = mlp_window(m, l, cell, px, py);
-    mlp_feature_row(X + i * kMlpInPad, bits, l);
+    float fx, fy;
+    const uint64_t bits = mlp_window(m, l, cell, px, py, &fx, &fy);
+    mlp_feature_row(X + i * kMlpInPad, bits, l, fx, fy);
     if (target) {
-        float t = (float)log2(dstar[i] / cell);
-        target[i] = fminf(fmaxf(t, -8.0f), 8.0f);
+        // log2(d* / s), with d* below s/16 (no proposal could be used) lumped together
+        const float t = (float)log2(fmax(dstar[i], cell / 16.0) / cell);
+        target[i] = fminf(fmaxf(t, -4.0f), 4.0f);
     }
     if (cell_out) cell_out[i] = cell;
 }
@@ -133,7 +207,8 @@ __global__ void bias_gelu(float* __restrict__ Z, const float* __restrict__ bias,
 // Output layer, loss and its gradients: y = w4.a3 + b4, r = y - t,
 // D = bf16(r/B * w4 * gelu'(z3)); dw4, db4 and the loss accumulate atomically.
 __global__ void output_layer(const uint16_t* __restrict__ A3, const float* __restrict__ Z3,
-                             const uint16_t* __restrict__ w4, float b4, const float* __restrict__ t,
+                             const uint16_t* __restrict__ w4, const float* __restrict__ b4p,
+                             const float* __restrict__ t,
                              int B, float* __restrict__ y_out, uint16_t* __restrict__ D,
                              float* __restrict__ grad, float* __restrict__ loss)
 {
@@ -141,7 +216,7 @@ __global__ void output_layer(const uint16_t* __restrict__ A3, const float* __res
     if (s >= B) return;
     float y = 0.0f;
     for (int j = 0; j < kMlpHid; ++j) y += bf2f(w4[j]) * bf2f(A3[s * kMlpHid + j]);
-    y += b4;
+    y += *b4p;
     if (y_out) y_out[s] = y;
     if (!t) return;
     const float r = y - t[s];
@@ -258,14 +333,18 @@ MlpView mlp_view(const frw_mlp* m, float alpha)
     v.b1 = m->d_params + kOffB1;
     v.b2 = m->d_params + kOffB2;
     v.b3 = m->d_params + kOffB3;
-    v.b4 = 0.0f;
-    hipMemcpy(&v.b4, m->d_params + kOffB4, sizeof(float), hipMemcpyDeviceToHost);
+    v.b4 = m->h_b4;
     v.alpha = alpha;
     v.occ = m->d_occ;
-    for (int l = 0; l <= kMlpMaxLevel; ++l) v.occ_off[l] = l <= m->Lmax ? m->occ_off[l] : 0;
+    for (int l = 0; l <= kPyrMaxLevel; ++l) {
+        v.occ_off[l] = l <= m->Pmax ? m->occ_off[l] : 0;
+        v.sat_off[l] = l <= m->Vmax ? m->sat_off[l] : 0;
+    }
+    v.sat = m->d_sat;
     v.bounds = m->tree->bounds;
     v.K0 = m->K0;
     v.Lmax = m->Lmax;
+    v.Vmax = m->Vmax;
     return v;
 }
 
@@ -315,9 +394,6 @@ int gradient(frw_mlp* m, const uint16_t* Xb, const float* tb, int B, hipStream_t
     float* GS = m->d_gs;  // 64 x 72 weight-gradient scratch
     const int64_t h = (int64_t)B * kMlpHid;
     const int H = kMlpHid, IN = kMlpInPad;
-    float b4 = 0.f;
-    int rc = check_hip(hipMemcpy(&b4, P + kOffB4, 4, hipMemcpyDeviceToHost), "read b4");
-    if (rc != FRW_OK) return rc;
     // forward: Z = A_prev * W^T + b, A = bf16(gelu(Z))
     gemm(Xb, IN, 1, W1, 1, IN, m->d_Z1, H, B, H, IN, s);
     bias_gelu<<<blocks(h, 256), 256, 0, s>>>(m->d_Z1, P + kOffB1, m->d_A1, h);
@@ -326,7 +402,7 @@ int gradient(frw_mlp* m, const uint16_t* Xb, const float* tb, int B, hipStream_t
     gemm(m->d_A2, H, 1, W3, 1, H, m->d_Z3, H, B, H, H, s);
     bias_gelu<<<blocks(h, 256), 256, 0, s>>>(m->d_Z3, P + kOffB3, m->d_A3, h);
     // output layer: loss, dw4, db4 and D3
-    output_layer<<<blocks(B, 256), 256, 0, s>>>(m->d_A3, m->d_Z3, w4, b4, tb, B, nullptr, m->d_D, G,
+    output_layer<<<blocks(B, 256), 256, 0, s>>>(m->d_A3, m->d_Z3, w4, P + kOffB4, tb, B, nullptr, m->d_D, G,
                                                 m->d_loss);
     // layer 3: dW3 = D3^T A2, db3 = colsum D3, F = D3 W3, D2 = bf16(F gelu'(Z2))
     gemm(m->d_D, 1, H, m->d_A2, H, 1, GS, H, H, H, B, s);
@@ -363,13 +439,28 @@ int frw_mlp_create(const frw_tree* t, uint64_t seed, frw_mlp** out)
     const int64_t W = (int64_t)t->bounds.z - t->bounds.x, H = (int64_t)t->bounds.w - t->bounds.y;
     while (((int64_t)1 << m->K0) <= std::max(W, H)) ++m->K0;
     m->Lmax = std::min(m->K0, kMlpMaxLevel);
-    int64_t words = 0;
-    for (int l = 0; l <= m->Lmax; ++l) {
+    // verification levels: finest cell about 1/64 of the mean rect spacing
+    // sqrt(W H / n), at most level 15 and at least the window levels
+    {
+        const double spacing = std::sqrt((double)std::max<int64_t>(W, 1) * (double)std::max<int64_t>(H, 1) /
+                                         (double)std::max<int64_t>(t->n_rects, 1));
+        const double target = std::max(1.0, spacing / 64.0);
+        const int v = m->K0 - (int)std::floor(std::log2(target));
+        m->Vmax = std::max(0, std::min({v, m->K0, kPyrMaxLevel}));
+    }
+    m->Pmax = std::max(m->Lmax, m->Vmax);
+    int64_t words = 0, entries = 0;
+    for (int l = 0; l <= m->Pmax; ++l) {
         m->occ_off.push_back(words);
         const int64_t bits = ((int64_t)1 << l) * ((int64_t)1 << l);
-        words += (bits + 31) / 32;
+        words += (bits + 31) / 32 + 1;  // +1: the window reader loads word pairs
+    }
+    for (int l = 0; l <= m->Vmax; ++l) {
+        m->sat_off.push_back(entries);
+        entries += (((int64_t)1 << l) + 1) * (((int64_t)1 << l) + 1);
     }
     int rc = check_hip(hipMalloc(&m->d_occ, words * 4), "hipMalloc occupancy");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_sat, entries * 4), "hipMalloc summed-area tables");
     int64_t* d_off = nullptr;
     if (rc == FRW_OK) rc = check_hip(hipMemset(m->d_occ, 0, words * 4), "memset occupancy");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_off, sizeof(int64_t) * m->occ_off.size()), "hipMalloc off");
@@ -377,10 +468,20 @@ int frw_mlp_create(const frw_tree* t, uint64_t seed, frw_mlp** out)
         rc = check_hip(hipMemcpy(d_off, m->occ_off.data(), sizeof(int64_t) * m->occ_off.size(),
                                  hipMemcpyHostToDevice), "copy off");
     if (rc == FRW_OK) {
-        occ_mark<<<blocks(t->n_rects, 256), 256>>>(t->d_rects, (int)t->n_rects, t->bounds, m->K0, m->Lmax,
+        occ_mark<<<blocks(t->n_rects, 256), 256>>>(t->d_rects, (int)t->n_rects, t->bounds, m->K0, m->Pmax,
                                                    d_off, m->d_occ);
-        rc = check_hip(hipDeviceSynchronize(), "occ_mark");
+        rc = check_hip(hipGetLastError(), "occ_mark");
     }
+    for (int l = 0; l <= m->Vmax && rc == FRW_OK; ++l) {
+        const int64_t n = (int64_t)1 << l;
+        uint32_t* S = m->d_sat + m->sat_off[l];
+        rc = check_hip(hipMemset(S, 0, (size_t)(n + 1) * 4), "memset table row 0");
+        if (rc != FRW_OK) break;
+        sat_rows<<<(unsigned)n, 256>>>(m->d_occ + m->occ_off[l], n, S);
+        sat_cols<<<blocks(n + 1, 256), 256>>>(n, S);
+        rc = check_hip(hipGetLastError(), "summed-area tables");
+    }
+    if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "occupancy pyramid");
     if (d_off) hipFree(d_off);
     // Xavier-uniform initial weights from Philox, zero biases
     std::vector<float> P(kMlpParams, 0.0f);
@@ -423,7 +524,7 @@ int frw_mlp_create(const frw_tree* t, uint64_t seed, frw_mlp** out)
 void frw_mlp_free(frw_mlp* m)
 {
     if (!m) return;
-    for (void* p : {(void*)m->d_occ, (void*)m->d_params, (void*)m->d_bf, (void*)m->d_adam_m,
+    for (void* p : {(void*)m->d_occ, (void*)m->d_sat, (void*)m->d_params, (void*)m->d_bf, (void*)m->d_adam_m,
                     (void*)m->d_adam_v, (void*)m->d_grad, (void*)m->d_gs, (void*)m->d_X, (void*)m->d_t, (void*)m->d_Z1,
                     (void*)m->d_Z2, (void*)m->d_Z3, (void*)m->d_F, (void*)m->d_A1, (void*)m->d_A2,
                     (void*)m->d_A3, (void*)m->d_D, (void*)m->d_Xb, (void*)m->d_tb, (void*)m->d_loss,
@@ -445,18 +546,38 @@ int frw_mlp_set_params(frw_mlp* m, const float* in, int32_t n)
     if (!m || !in || n != kMlpParams) return set_error(FRW_E_INVALID, "bad parameter buffer");
     int rc = check_hip(hipMemcpy(m->d_params, in, kMlpParams * 4, hipMemcpyHostToDevice), "copy params");
     if (rc != FRW_OK) return rc;
+    m->h_b4 = in[kOffB4];
     to_bf16<<<blocks(kMlpHid * kMlpInPad, 256), 256>>>(m->d_params, m->d_bf);
     return check_hip(hipDeviceSynchronize(), "to_bf16");
 }
 
 int frw_mlp_occupancy(const frw_mlp* m, int32_t level, uint32_t* out_words, int64_t n_words)
 {
-    if (!m || !out_words || level < 0 || level > m->Lmax) return set_error(FRW_E_INVALID, "bad level");
+    if (!m || !out_words || level < 0 || level > m->Pmax) return set_error(FRW_E_INVALID, "bad level");
     const int64_t bits = ((int64_t)1 << level) * ((int64_t)1 << level);
     const int64_t need = (bits + 31) / 32;
     if (n_words < need) return set_error(FRW_E_INVALID, "output too small");
     return check_hip(hipMemcpy(out_words, m->d_occ + m->occ_off[level], need * 4, hipMemcpyDeviceToHost),
                      "copy occupancy");
+}
+
+int frw_mlp_levels(const frw_mlp* m, int32_t out[4])
+{
+    if (!m || !out) return set_error(FRW_E_INVALID, "null argument");
+    out[0] = m->K0;
+    out[1] = m->Lmax;
+    out[2] = m->Vmax;
+    out[3] = m->Pmax;
+    return FRW_OK;
+}
+
+int frw_mlp_certify(const frw_mlp* m, const double* pts, const double* d, int64_t n, uint8_t* out, void* stream)
+{
+    if (!m || n < 0 || (n > 0 && (!pts || !d || !out))) return set_error(FRW_E_INVALID, "bad arguments");
+    if (n == 0) return FRW_OK;
+    hipStream_t s = (hipStream_t)stream;
+    certify_kernel<<<blocks(n, 256), 256, 0, s>>>(mlp_view(m, 1.0f), pts, d, n, out);
+    return check_hip(hipStreamSynchronize(s), "certify");
 }
 
 int frw_mlp_features(const frw_mlp* m, const double* pts, const double* h, int64_t n, float* feats,
@@ -573,6 +694,7 @@ int frw_mlp_train(frw_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t 
     }
     if (d_perm) hipFree(d_perm);
     if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "train");
+    if (rc == FRW_OK) rc = check_hip(hipMemcpy(&m->h_b4, m->d_params + kOffB4, 4, hipMemcpyDeviceToHost), "read b4");
     if (rc == FRW_OK && last_loss) *last_loss = loss_n ? loss_sum / loss_n : 0.f;
     return rc;
 }
@@ -618,15 +740,22 @@ int frw_walk_harvest(const frw_tree* t, const frw_walk_params* p, double* pts, d
     return rc;
 }
 
-int frw_walk_learned(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p, const frw_walk_outputs* o,
-                     float alpha, int64_t stats[2])
+namespace {
+
+int check_learned(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p, float alpha)
 {
-    if (!t || !m || !p || !o || !o->counts || !o->hops) return set_error(FRW_E_INVALID, "bad arguments");
+    if (!t || !m || !p) return set_error(FRW_E_INVALID, "bad arguments");
     if (m->tree != t) return set_error(FRW_E_INVALID, "network built for another tree");
     if (p->master < 0 || p->master >= t->n_rects || p->n_walks <= 0 || p->first_walk < 0 ||
         p->first_walk + p->n_walks > ((int64_t)1 << 32))
         return set_error(FRW_E_RANGE, "bad master or walk range");
     if (!(alpha > 0.0f)) return set_error(FRW_E_INVALID, "alpha must be positive");
+    return FRW_OK;
+}
+
+int learned_sync(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p, const frw_walk_outputs* o,
+                 float alpha, unsigned long long* d_traffic, int64_t stats[2])
+{
     frw_walk_params q = *p;
     q.mode = FRW_MODE_LEARNED;
     unsigned long long* d_st = nullptr;
@@ -637,7 +766,7 @@ int frw_walk_learned(const frw_tree* t, const frw_mlp* m, const frw_walk_params*
         WalkExtras ex;
         ex.mlp = &v;
         ex.l_stats = d_st;
-        rc = launch_walk(t, &q, o, nullptr, nullptr, &ex);
+        rc = launch_walk(t, &q, o, nullptr, d_traffic, &ex);
     }
     if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "learned walk");
     unsigned long long st[2] = {0, 0};
@@ -646,10 +775,62 @@ int frw_walk_learned(const frw_tree* t, const frw_mlp* m, const frw_walk_params*
         stats[0] = (int64_t)st[0];
         stats[1] = (int64_t)st[1];
     }
-    hipFree(d_st);
+    if (d_st) hipFree(d_st);
     int err = 0;
     if (rc == FRW_OK) rc = check_hip(hipMemcpy(&err, t->d_err, sizeof(int), hipMemcpyDeviceToHost), "err flag");
     if (rc == FRW_OK && err) rc = set_error(FRW_E_KERNEL, "kernel error flag " + std::to_string(err));
+    return rc;
+}
+
+}  // namespace
+
+int frw_walk_learned_async(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p,
+                           const frw_walk_outputs* o, float alpha, uint64_t* stats_dev, void* stream)
+{
+    int rc = check_learned(t, m, p, alpha);
+    if (rc != FRW_OK) return rc;
+    if (!o || !o->counts || !o->hops) return set_error(FRW_E_INVALID, "null outputs");
+    frw_walk_params q = *p;
+    q.mode = FRW_MODE_LEARNED;
+    const MlpView v = mlp_view(m, alpha);
+    WalkExtras ex;
+    ex.mlp = &v;
+    ex.l_stats = (unsigned long long*)stats_dev;
+    return launch_walk(t, &q, o, (hipStream_t)stream, nullptr, &ex);
+}
+
+int frw_walk_learned(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p, const frw_walk_outputs* o,
+                     float alpha, int64_t stats[2])
+{
+    int rc = check_learned(t, m, p, alpha);
+    if (rc != FRW_OK) return rc;
+    if (!o || !o->counts || !o->hops) return set_error(FRW_E_INVALID, "null outputs");
+    return learned_sync(t, m, p, o, alpha, nullptr, stats);
+}
+
+int frw_walk_learned_traffic(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p, float alpha,
+                             int64_t out[5])
+{
+    int rc = check_learned(t, m, p, alpha);
+    if (rc != FRW_OK) return rc;
+    if (!out) return set_error(FRW_E_INVALID, "null output");
+    const int64_t nb = t->n_rects + 2;
+    int64_t *d_counts = nullptr, *d_hops = nullptr, *d_tr = nullptr;
+    rc = check_hip(hipMalloc(&d_counts, sizeof(int64_t) * nb), "hipMalloc counts");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hops, sizeof(int64_t)), "hipMalloc hops");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_tr, 2 * sizeof(int64_t)), "hipMalloc traffic");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_counts, 0, sizeof(int64_t) * nb), "memset");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_hops, 0, sizeof(int64_t)), "memset");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_tr, 0, 2 * sizeof(int64_t)), "memset");
+    if (rc == FRW_OK) {
+        frw_walk_outputs o{d_counts, d_hops, nullptr, nullptr};
+        rc = learned_sync(t, m, p, &o, alpha, (unsigned long long*)d_tr, out + 3);
+    }
+    if (rc == FRW_OK) rc = check_hip(hipMemcpy(out, d_tr, 2 * sizeof(int64_t), hipMemcpyDeviceToHost), "copy");
+    if (rc == FRW_OK) rc = check_hip(hipMemcpy(out + 2, d_hops, sizeof(int64_t), hipMemcpyDeviceToHost), "copy");
+    if (d_counts) hipFree(d_counts);
+    if (d_hops) hipFree(d_hops);
+    if (d_tr) hipFree(d_tr);
     return rc;
 }
 

@@ -16,8 +16,9 @@
 //                 else square = biggest square replayed from d*
 //   bisect mode:  square = literal bisection (one Q1 per step), hit = Q2(p)
 //   learned mode: the wave's 64 points go through the proposal network
-//                 (bf16 MFMA); a collision-free Q1 probe of the proposed
-//                 square accepts it, otherwise the exact-mode step runs
+//                 (bf16 MFMA); an occupancy-pyramid probe that certifies the
+//                 proposed square free accepts it, otherwise the exact-mode
+//                 step runs
 // Tallies: int64 hit counts per original rect id + escaped + stuck bins and
 // hop counts per job; optional per-walk outputs for replay checks.
 #include <algorithm>
@@ -168,11 +169,12 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
             // all 64 lanes take part in the MFMA forward pass
             uint64_t bits = 0;
             int level = 0;
+            float fx = 0.0f, fy = 0.0f;
             if (active && !escaped) {
                 level = mlp_level(mlp, hprev, &cell);
-                bits = mlp_window(mlp, level, cell, px, py);
+                bits = mlp_window(mlp, level, cell, px, py, &fx, &fy);
             }
-            mlp_feature_row(mlp_xs + lane * kMlpStride, bits, level);
+            mlp_feature_row(mlp_xs + lane * kMlpStride, bits, level, fx, fy);
             y_mlp = mlp_forward_wave(mlp, mlp_xs, mlp_hs, lane);
         }
         if (!active) continue;
@@ -184,14 +186,14 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
             bool proposed = false;
             if (kMode == kModeLearned) {
                 const double d = cell * exp2((double)y_mlp) * (double)mlp.alpha;
-                if (d > 2.0 && d < 1e300 &&
-                    !q1_box<false>(t, st, px - d, py - d, px + d, py + d, make_int4(0, 0, 0, 0), cp)) {
-                    sq = DBox{px - d, py - d, px + d, py + d};
-                    proposed = true;
-                    my_acc++;
-                } else {
-                    my_fb++;
+                if (d > 2.0 && d < 1e300) {
+                    const DBox c{px - d, py - d, px + d, py + d};
+                    if (mlp_certify(mlp, c.x0, c.y0, c.x1, c.y1, d)) {
+                        sq = c;
+                        proposed = true;
+                    }
                 }
+                if (proposed) my_acc++; else my_fb++;
             }
             if (!proposed) {
                 if (kMode != kModeBisect) {

@@ -85,7 +85,8 @@ EXPORTS = [
     "frw_mlp_create", "frw_mlp_free", "frw_mlp_param_count", "frw_mlp_get_params",
     "frw_mlp_set_params", "frw_mlp_occupancy", "frw_mlp_features", "frw_mlp_forward",
     "frw_mlp_gradient", "frw_mlp_set_samples", "frw_mlp_train", "frw_walk_harvest",
-    "frw_walk_learned",
+    "frw_walk_learned", "frw_walk_learned_async", "frw_walk_learned_traffic",
+    "frw_mlp_levels", "frw_mlp_certify",
 ]
 
 _lib = None
@@ -151,6 +152,8 @@ def lib():
     L.frw_mlp_get_params.argtypes = [vp, vp, i32]
     L.frw_mlp_set_params.argtypes = [vp, vp, i32]
     L.frw_mlp_occupancy.argtypes = [vp, i32, vp, i64]
+    L.frw_mlp_levels.argtypes = [vp, vp]
+    L.frw_mlp_certify.argtypes = [vp, vp, vp, i64, vp, vp]
     L.frw_mlp_features.argtypes = [vp, vp, vp, i64, vp, vp]
     L.frw_mlp_forward.argtypes = [vp, vp, i64, vp, vp]
     L.frw_mlp_gradient.argtypes = [vp, vp, vp, i64, vp, C.POINTER(C.c_float)]
@@ -159,6 +162,9 @@ def lib():
     L.frw_walk_harvest.argtypes = [vp, C.POINTER(WalkParams), vp, vp, vp, i64, C.POINTER(i64)]
     L.frw_walk_learned.argtypes = [vp, vp, C.POINTER(WalkParams), C.POINTER(WalkOutputs),
                                    C.c_float, vp]
+    L.frw_walk_learned_async.argtypes = [vp, vp, C.POINTER(WalkParams), C.POINTER(WalkOutputs),
+                                         C.c_float, vp, vp]
+    L.frw_walk_learned_traffic.argtypes = [vp, vp, C.POINTER(WalkParams), C.c_float, vp]
     _lib = L
     return L
 
@@ -373,7 +379,7 @@ class Batch:
 
 class Mlp:
     """Learned square-size proposal bound to one tree (occupancy pyramid +
-    65-64-64-64-1 network)."""
+    67-64-64-64-1 network)."""
 
     def __init__(self, tree, seed=1):
         self.tree = tree
@@ -397,6 +403,17 @@ class Mlp:
         _check(lib().frw_mlp_occupancy(self.h, level, words.ctypes.data, len(words)))
         bits = np.unpackbits(words.view(np.uint8), bitorder="little")[:n]
         return bits.reshape(1 << level, 1 << level)   # [j][i]
+
+    def levels(self):
+        """(K0, finest window level, finest verification level, finest bitmap level)."""
+        out = np.zeros(4, dtype=np.int32)
+        _check(lib().frw_mlp_levels(self.h, out.ctypes.data))
+        return tuple(int(v) for v in out)
+
+    def certify(self, pts, d, out, stream=None):
+        """Verification probe on device tensors: out uint8[n]."""
+        _check(lib().frw_mlp_certify(self.h, pts.data_ptr(), d.data_ptr(), pts.shape[0], out.data_ptr(),
+                                     stream))
 
     def features(self, pts, h, out, stream=None):
         _check(lib().frw_mlp_features(self.h, pts.data_ptr(), h.data_ptr(), pts.shape[0],
@@ -452,3 +469,26 @@ def walk_learned(tree, mlp, master, n_walks, counts, hops, hit_out=None, hops_ou
     st = np.zeros(2, dtype=np.int64)
     _check(lib().frw_walk_learned(tree.h, mlp.h, C.byref(p), C.byref(o), alpha, st.ctypes.data))
     return int(st[0]), int(st[1])
+
+
+def walk_learned_async(tree, mlp, master, n_walks, counts, hops, stats=None, hit_out=None,
+                       hops_out=None, seed=1, first_walk=0, job=0, alpha=0.8, block=0, grid=0,
+                       stream=None):
+    """Asynchronous learned walks into device tensors; `stats` (device uint64[2]
+    or int64[2]) accumulates (accepted, fallback) when given."""
+    p = WalkParams(master=master, mode=MODE_LEARNED, n_walks=n_walks, first_walk=first_walk,
+                   seed=seed, job=job, block=block, grid=grid, reserved=0, max_hops=0)
+    o = WalkOutputs(counts.data_ptr(), hops.data_ptr(),
+                    hit_out.data_ptr() if hit_out is not None else None,
+                    hops_out.data_ptr() if hops_out is not None else None)
+    _check(lib().frw_walk_learned_async(tree.h, mlp.h, C.byref(p), C.byref(o), alpha,
+                                        stats.data_ptr() if stats is not None else None, stream))
+
+
+def walk_learned_traffic(tree, mlp, master, n_walks, seed=1, alpha=0.8, first_walk=0, job=0):
+    """Instrumented learned walks: (node records, rect records, hops, accepted, fallback)."""
+    p = WalkParams(master=master, mode=MODE_LEARNED, n_walks=n_walks, first_walk=first_walk,
+                   seed=seed, job=job, block=0, grid=0, reserved=0, max_hops=0)
+    out = np.zeros(5, dtype=np.int64)
+    _check(lib().frw_walk_learned_traffic(tree.h, mlp.h, C.byref(p), alpha, out.ctypes.data))
+    return tuple(int(v) for v in out)

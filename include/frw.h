@@ -238,12 +238,14 @@ int frw_batch_check(const frw_batch* b);
 void frw_batch_free(frw_batch* b);
 
 /* ---------------------------------------------------------------- learned proposal
- * A small network that proposes the half-size of each hop's square; one Q1
- * probe verifies the proposal and a failed probe falls back to the exact
- * search.  No reference counterpart (the reference always bisects,
- * randomWalk/CPU/quadTree.cpp:264-312).  Features: the 8x8 occupancy window
- * around the point at a scale set by the previous square, plus the level;
- * network 65-64-64-64-1, GELU, bf16 weights, FP32 accumulation (MFMA). */
+ * A small network that proposes the half-size of each hop's square; one probe
+ * of the layout's occupancy pyramid (a complete quadtree of cell occupancy,
+ * stored per level as a summed-area table) verifies the proposal, and a
+ * failed probe falls back to the exact search.  No reference counterpart (the
+ * reference always bisects, randomWalk/CPU/quadTree.cpp:264-312).  Features:
+ * the 8x8 occupancy window around the point at a scale set by the previous
+ * square, the level and the point's position in its cell; network
+ * 67-64-64-64-1 (input padded to 72), GELU, bf16 weights, FP32 accumulation (MFMA). */
 
 typedef struct frw_mlp frw_mlp;
 
@@ -257,6 +259,14 @@ int frw_mlp_get_params(const frw_mlp* m, float* out, int32_t n);
 int frw_mlp_set_params(frw_mlp* m, const float* in, int32_t n);
 /* Copies one occupancy level (2^l x 2^l bits, row-major, 32-bit words). */
 int frw_mlp_occupancy(const frw_mlp* m, int32_t level, uint32_t* out_words, int64_t n_words);
+/* out = {K0 (2^K0 > bounds extent), Lmax (finest window level), Vmax (finest
+ * verification level), finest bitmap level}. */
+int frw_mlp_levels(const frw_mlp* m, int32_t out[4]);
+/* Verification probe on device arrays: out[i] = 1 when the closed square of
+ * half-size d[i] centred at pts[i] is certified free (inside the bounds and
+ * no occupied cell in its covering block at the verification level). */
+int frw_mlp_certify(const frw_mlp* m, const double* pts, const double* d, int64_t n, uint8_t* out,
+                    void* stream);
 /* Device arrays: pts n x 2, h n (previous half-size) -> feats n x 72 (FP32). */
 int frw_mlp_features(const frw_mlp* m, const double* pts, const double* h, int64_t n, float* feats,
                      void* stream);
@@ -281,6 +291,14 @@ int frw_walk_harvest(const frw_tree* t, const frw_walk_params* p, double* pts, d
  * = {accepted proposals, fallbacks}.  Synchronous. */
 int frw_walk_learned(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p,
                      const frw_walk_outputs* o, float alpha, int64_t stats[2]);
+/* Asynchronous form on `stream`: no allocation, no synchronisation; stats_dev
+ * (device, 2 x uint64, may be NULL) accumulates {accepted, fallbacks}. */
+int frw_walk_learned_async(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p,
+                           const frw_walk_outputs* o, float alpha, uint64_t* stats_dev, void* stream);
+/* Measurement hook: learned walks with traffic counters compiled in.
+ * out = {node records read, rect records read, hops, accepted, fallbacks}. */
+int frw_walk_learned_traffic(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p, float alpha,
+                             int64_t out[5]);
 
 /* Measurement hook (no reference counterpart): the same walks as frw_walk
  * with per-hop traffic counters compiled in.  out = {node records read

@@ -118,9 +118,12 @@ int64_t orc_walk_many(const orc_index* ix, int master, uint64_t seed, uint32_t j
 
 /* ---- learned square-size proposal (orc_mlp.c) -------------------------- */
 float orc_bf16(float x);
+void  orc_mlp_set_rounding(int on);
 int   orc_mlp_level(const orc_box* bounds, double h, double* cell);
 void  orc_mlp_features(const double* xyxy, int n, const orc_box* bounds, double px, double py,
-                       double h, float out[65]);
+                       double h, float out[67]);
+int   orc_mlp_vmax(const orc_box* bounds, int n_rects);
+int   orc_mlp_certify(const double* xyxy, int n, const orc_box* bounds, double px, double py, double d);
 int   orc_mlp_param_count(void);
 void  orc_mlp_forward(const float* params, const float* X, int B, float* y);
 float orc_mlp_backward(const float* params, const float* X, const float* t, int B, float* grad);

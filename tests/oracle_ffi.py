@@ -45,6 +45,7 @@ _D = C.POINTER(C.c_double)
 _I32 = C.POINTER(C.c_int32)
 _I64 = C.POINTER(C.c_int64)
 _U32 = C.POINTER(C.c_uint32)
+_F32 = C.POINTER(C.c_float)
 
 
 def _ptr(a, t):
@@ -82,6 +83,17 @@ def oracle():
     L.orc_q3.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_int)]
     L.orc_gauss_from.restype = Box
     L.orc_gauss_from.argtypes = [C.c_void_p, C.c_int]
+    L.orc_bf16.restype = C.c_float
+    L.orc_bf16.argtypes = [C.c_float]
+    L.orc_mlp_set_rounding.argtypes = [C.c_int]
+    L.orc_mlp_level.argtypes = [C.POINTER(Box), C.c_double, _D]
+    L.orc_mlp_features.argtypes = [_D, C.c_int, C.POINTER(Box), C.c_double, C.c_double,
+                                   C.c_double, _F32]
+    L.orc_mlp_forward.argtypes = [_F32, _F32, C.c_int, _F32]
+    L.orc_mlp_vmax.argtypes = [C.POINTER(Box), C.c_int]
+    L.orc_mlp_certify.argtypes = [_D, C.c_int, C.POINTER(Box), C.c_double, C.c_double, C.c_double]
+    L.orc_mlp_backward.restype = C.c_float
+    L.orc_mlp_backward.argtypes = [_F32, _F32, _F32, C.c_int, _F32]
     L.orc_walk_many.restype = C.c_int64
     L.orc_walk_many.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_uint32, C.c_int64,
                                 C.c_int64, C.c_int64, _I64, _I32, _I32, C.c_int]
@@ -276,3 +288,78 @@ def ref_parse(path, layer=0, cap=10_000_000):
     buf = np.zeros((max(n, 1), 4), dtype=np.float64)
     L.ref_parse(path.encode(), layer, _ptr(buf, _D), n, _ptr(b, _D), C.byref(nl))
     return n, buf[:n], tuple(b), nl.value
+
+
+# ---------------------------------------------------------------- learned proposal
+
+MLP_IN_PAD = 72
+
+
+def bf16(x: float) -> float:
+    return oracle().orc_bf16(x)
+
+
+def mlp_param_count() -> int:
+    return oracle().orc_mlp_param_count()
+
+
+def mlp_level(bnds, h):
+    cell = C.c_double()
+    lvl = oracle().orc_mlp_level(C.byref(Box(*bnds)), h, C.byref(cell))
+    return lvl, cell.value
+
+
+def mlp_features(rects, bnds, pts, hs):
+    """[n, 72] float32 feature rows (67 features + zero padding), brute force."""
+    r = np.ascontiguousarray(rects, dtype=np.float64)
+    b = Box(*bnds)
+    out = np.zeros((len(pts), MLP_IN_PAD), dtype=np.float32)
+    row = np.zeros(67, dtype=np.float32)
+    L = oracle()
+    for i, ((px, py), h) in enumerate(zip(pts, hs)):
+        L.orc_mlp_features(_ptr(r, _D), len(r), C.byref(b), float(px), float(py), float(h), _ptr(row, _F32))
+        out[i, :67] = row
+    return out
+
+
+def mlp_vmax(bnds, n_rects):
+    return oracle().orc_mlp_vmax(C.byref(Box(*bnds)), n_rects)
+
+
+def mlp_certify(rects, bnds, pts, ds):
+    """1 where the square of half-size d at p is certified free (brute force)."""
+    r = np.ascontiguousarray(rects, dtype=np.float64)
+    b = Box(*bnds)
+    L = oracle()
+    return np.array([L.orc_mlp_certify(_ptr(r, _D), len(r), C.byref(b), float(px), float(py), float(d))
+                     for (px, py), d in zip(pts, ds)], dtype=np.uint8)
+
+
+class _Rounding:
+    def __init__(self, on):
+        self.on = on
+
+    def __enter__(self):
+        oracle().orc_mlp_set_rounding(1 if self.on else 0)
+
+    def __exit__(self, *a):
+        oracle().orc_mlp_set_rounding(1)
+
+
+def mlp_forward(params, X, rounding=True):
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    Xc = np.ascontiguousarray(X, dtype=np.float32)
+    y = np.zeros(len(Xc), dtype=np.float32)
+    with _Rounding(rounding):
+        oracle().orc_mlp_forward(_ptr(P, _F32), _ptr(Xc, _F32), len(Xc), _ptr(y, _F32))
+    return y
+
+
+def mlp_backward(params, X, t, rounding=True):
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    Xc = np.ascontiguousarray(X, dtype=np.float32)
+    tc = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros(len(P), dtype=np.float32)
+    with _Rounding(rounding):
+        loss = oracle().orc_mlp_backward(_ptr(P, _F32), _ptr(Xc, _F32), _ptr(tc, _F32), len(Xc), _ptr(g, _F32))
+    return float(loss), g
