@@ -30,12 +30,18 @@ WORKLOADS = {
     "c2": (10_000, 100_000, 10, 500, 0.0, 0.0, 0, 10_000_000),
     "c3": (1_000_000, 10_000_000, 5, 300, 0.2, 1e5, 100, 10_000_000),
     "c1": (100, 10_000, 10, 200, 0.0, 0.0, 0, 1_000_000),
+    "c5": (1_000_000, 10_000_000, 5, 300, 0.2, 1e5, 100, 10_000_000),
 }
 DESCR = {
     "c2": "plain GPU FRW: 1e4 rects, box 1e5, sides U[10,500], master 0, 1e7 walks/step",
     "c3": "FRW at scale: 1e6 rects, box 1e7, sides U[5,300], 20% in 100 Gaussian blobs "
           "(sigma 1e5), master 0, 1e7 walks/step",
     "c1": "CPU-reference layout: 100 rects, box 1e4, sides U[10,200], master 0, 1e6 walks/step",
+    "c4": "batched: 64 layouts, rects log-uniform in [1e2,1e5] (box 1000 sqrt(n), sides "
+          "U[10,500]), master 0 each, walks per layout U[1e5,1e6], one persistent launch per "
+          "step, graph replay, autotuned launch shape",
+    "c5": "learned proposal at scale: the 1e6-rect layout, network trained on pairs harvested "
+          "from exact walks, master 0, 1e7 walks/step, exact fallback",
 }
 
 
@@ -48,8 +54,14 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2")
-    ap.add_argument("--mode", choices=["exact", "bisect"], default="exact")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["c4"], default="c2")
+    ap.add_argument("--mode", choices=["exact", "bisect", "learned"], default=None)
+    ap.add_argument("--alpha", type=float, default=0.9, help="learned: safety factor")
+    ap.add_argument("--pairs", type=int, default=2_000_000, help="learned: training pairs")
+    ap.add_argument("--epochs", type=int, default=3, help="learned: training epochs")
+    ap.add_argument("--compare", action="store_true",
+                    help="also time bisect / exact / learned on the same layout")
+    ap.add_argument("--compare-walks", type=int, default=2_000_000)
     ap.add_argument("--walks", type=int, default=0, help="walks per step (0 = workload default)")
     ap.add_argument("--leaf-cap", type=int, default=16)
     ap.add_argument("--block", type=int, default=0)
@@ -100,6 +112,120 @@ def cpu_baseline(rects, bounds, master, seconds):
     }
 
 
+def make_layout(frw, workload, seed=1):
+    n, box, smin, smax, cfrac, sigma, ncl, _ = WORKLOADS[workload]
+    return frw.Layout.generate(n, box, smin, smax, seed=seed, cluster_frac=cfrac, sigma=sigma,
+                               n_clusters=ncl)
+
+
+def train_proposal(frw, torch, tree, master, n_pairs, epochs, seed=1):
+    """Harvests (point, previous half-size, d*) pairs from exact walks (walk ids
+    from 3 * 2^30, disjoint from the measured ones) and trains the network."""
+    t0 = time.perf_counter()
+    mlp = frw.Mlp(tree, seed=seed)
+    pts = torch.empty((n_pairs, 2), dtype=torch.float64, device="cuda")
+    hh = torch.empty(n_pairs, dtype=torch.float64, device="cuda")
+    ds = torch.empty(n_pairs, dtype=torch.float64, device="cuda")
+    got, walks = 0, max(1000, n_pairs // 8)
+    got = frw.walk_harvest(tree, master, walks, pts, hh, ds, seed=seed, first_walk=3 << 30)
+    if got < n_pairs:  # more walks to fill the set
+        more = int(walks * (n_pairs / max(got, 1) - 1) * 1.1) + 1000
+        got = frw.walk_harvest(tree, master, walks + more, pts, hh, ds, seed=seed, first_walk=3 << 30)
+    got = min(got, n_pairs)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    mlp.set_samples(pts[:got].contiguous(), hh[:got].contiguous(), ds[:got].contiguous())
+    del pts, hh, ds
+    loss = mlp.train(epochs=epochs, batch=1024, lr=2e-3, seed=seed)
+    t2 = time.perf_counter()
+    log(f"proposal: {got} pairs harvested in {t1 - t0:.2f} s, {epochs} epochs in {t2 - t1:.2f} s, "
+        f"loss {loss:.4f}")
+    return mlp, {"pairs": got, "harvest_s": t1 - t0, "train_s": t2 - t1, "epochs": epochs,
+                 "final_loss": loss}
+
+
+def time_walks(torch, launch, steps, warmup, sync_stream):
+    """Runs warmup + timed launches; returns (wall seconds, per-launch ms)."""
+    for w in range(warmup):
+        launch(w)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    t0 = time.perf_counter()
+    with torch.cuda.stream(sync_stream):
+        for k in range(steps):
+            ev[k][0].record(sync_stream)
+            launch(warmup + k)
+            ev[k][1].record(sync_stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, [a.elapsed_time(b) for a, b in ev]
+
+
+def walk_mode(frw, torch, tree, mlp, mode, master, walks, steps, warmup, stream, alpha, block=0,
+              first_base=0):
+    nr = tree.info().n_rects
+    counts = torch.zeros(nr + 2, dtype=torch.int64, device="cuda")
+    hops = torch.zeros(1, dtype=torch.int64, device="cuda")
+    stats = torch.zeros(2, dtype=torch.int64, device="cuda")
+    sh = stream.cuda_stream
+
+    def launch(step):
+        if mode == "learned":
+            frw.walk_learned_async(tree, mlp, master, walks, counts, hops, stats, seed=1,
+                                   first_walk=first_base + step * walks, alpha=alpha, block=block,
+                                   stream=sh)
+        else:
+            m = frw.MODE_EXACT if mode == "exact" else frw.MODE_BISECT
+            tree.walk_async(master, walks, counts, hops, seed=1, mode=m,
+                            first_walk=first_base + step * walks, block=block, stream=sh)
+
+    # counts/hops/stats cover the timed launches only; warmup launches take the
+    # first walk ranges, timed ones the next
+    for w in range(warmup):
+        launch(w)
+    torch.cuda.synchronize()
+    counts.zero_()
+    hops.zero_()
+    stats.zero_()
+    torch.cuda.synchronize()
+    wall, launch_ms = time_walks(torch, lambda k: launch(warmup + k), steps, 0, stream)
+    c = counts.cpu().numpy()
+    st = stats.cpu().numpy()
+    total_hops = int(hops.item())
+    n_walks = walks * steps
+    return {
+        "wall": wall, "launch_ms": launch_ms, "hops": total_hops, "walks": n_walks,
+        "counts": c, "accepted": int(st[0]), "fallback": int(st[1]),
+        "steps_per_s": total_hops / wall, "walks_per_s": n_walks / wall,
+    }
+
+
+def fraction_other(c, master, n_walks):
+    return 1.0 - c[master] / n_walks
+
+
+def walks_for_1pct(f):
+    """Walks for a 3-sigma half-width of 1% of the fraction f of walks that end
+    anywhere but the master: 3 sqrt(f (1 - f) / N) <= 0.01 f."""
+    return math.ceil(9e4 * (1.0 - f) / f) if f > 0 else float("inf")
+
+
+def traffic_bytes(frw, tree, mlp, mode, master, walks, alpha, first):
+    """Algorithmic bytes per hop from an instrumented run: 32 B per node record
+    and 16 B per rect record the traversals read; learned mode adds 16 B per
+    pyramid probe and 64 B per feature window (8 rows x two words)."""
+    if mode == "learned":
+        nodes, rects, h, acc, fb = frw.walk_learned_traffic(tree, mlp, master, walks, seed=1,
+                                                            alpha=alpha, first_walk=first)
+        b = 32.0 * nodes + 16.0 * rects + 16.0 * (acc + fb) + 64.0 * h
+    else:
+        m = frw.MODE_EXACT if mode == "exact" else frw.MODE_BISECT
+        nodes, rects, h = tree.walk_traffic(master, walks, seed=1, mode=m, first_walk=first)
+        b = 32.0 * nodes + 16.0 * rects
+    return b / max(h, 1), nodes / max(h, 1), rects / max(h, 1)
+
+
 def main():
     args = parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -116,74 +242,48 @@ def main():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
     torch.cuda.init()
-    n, box, smin, smax, cfrac, sigma, ncl, wps = WORKLOADS[args.workload]
-    walks = args.walks or wps
-    mode = frw.MODE_EXACT if args.mode == "exact" else frw.MODE_BISECT
+    if args.workload == "c4":
+        return bench_batched(args, frw, torch, np)
+    walks = args.walks or WORKLOADS[args.workload][7]
     master = 0
+    mode = args.mode or ("learned" if args.workload == "c5" else "exact")
 
-    lay = frw.Layout.generate(n, box, smin, smax, seed=1, cluster_frac=cfrac, sigma=sigma,
-                              n_clusters=ncl)
+    lay = make_layout(frw, args.workload)
     tree = frw.Tree(lay, leaf_cap=args.leaf_cap)
     info = tree.info()
     nr = info.n_rects
-    log(f"layout {n} rects, tree {info.n_nodes} nodes depth {info.depth}, build {info.build_ms:.2f} ms")
-
+    log(f"layout {nr} rects, tree {info.n_nodes} nodes depth {info.depth}, build {info.build_ms:.2f} ms")
     stream = torch.cuda.Stream()
-    sh = stream.cuda_stream
-    counts = torch.zeros(nr + 2, dtype=torch.int64, device="cuda")
-    hops = torch.zeros(1, dtype=torch.int64, device="cuda")
 
-    def launch(step):
-        tree.walk_async(master, walks, counts, hops, seed=1, mode=mode,
-                        first_walk=step * walks, block=args.block, stream=sh)
+    mlp, train_info = None, None
+    if mode == "learned" or args.compare:
+        mlp, train_info = train_proposal(frw, torch, tree, master, args.pairs, args.epochs)
 
-    for w in range(args.warmup):
-        launch(w)
-    torch.cuda.synchronize()
-    log("warmup done")
-    counts.zero_()
-    hops.zero_()
-    torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(args.steps)]
-    t0 = time.perf_counter()
-    with torch.cuda.stream(stream):
-        for k in range(args.steps):
-            ev[k][0].record(stream)
-            launch(args.warmup + k)
-            ev[k][1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    launch_ms = [a.elapsed_time(b) for a, b in ev]
-    log(f"timed {args.steps} steps in {wall:.3f} s")
-    total_hops = int(hops.item())
-    c = counts.cpu().numpy()
-    n_walks = walks * args.steps
-    steps_per_s = total_hops / wall
-    walks_per_s = n_walks / wall
-    f = 1.0 - c[master] / n_walks
-    n1 = math.ceil(9e4 * (1.0 - f) / f) if f > 0 else float("inf")
-    t1pct = info.build_ms / 1e3 + n1 / walks_per_s
+    r = walk_mode(frw, torch, tree, mlp, mode, master, walks, args.steps, args.warmup, stream,
+                  args.alpha, block=args.block)
+    log(f"timed {args.steps} steps of {walks} walks ({mode}) in {r['wall']:.3f} s")
+    f = fraction_other(r["counts"], master, r["walks"])
+    n1 = walks_for_1pct(f)
+    setup_s = info.build_ms / 1e3 + (train_info["harvest_s"] + train_info["train_s"]
+                                     if mode == "learned" else 0.0)
+    t1pct = setup_s + n1 / r["walks_per_s"]
 
-    # algorithmic bytes: node (32 B) and rect (16 B) records the traversals read,
-    # counted by an instrumented run of the first 10^6 walks of the workload
     sub = min(walks, 1_000_000)
-    tr_nodes, tr_rects, tr_hops = tree.walk_traffic(master, sub, seed=1, mode=mode,
-                                                    first_walk=args.warmup * walks)
-    bytes_per_hop = (32.0 * tr_nodes + 16.0 * tr_rects) / max(tr_hops, 1)
-    log(f"traffic run: {tr_hops} hops, {bytes_per_hop:.1f} B/hop")
-    avg_ms = sum(launch_ms) / len(launch_ms)
-    bytes_per_launch = bytes_per_hop * total_hops / args.steps
+    bph, nph, rph = traffic_bytes(frw, tree, mlp, mode, master, sub, args.alpha,
+                                  first=args.warmup * walks)
+    log(f"traffic run: {bph:.1f} B/hop")
+    avg_ms = sum(r["launch_ms"]) / len(r["launch_ms"])
+    bytes_per_launch = bph * r["hops"] / args.steps
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
 
     out = {
         "metric": "walk steps/s",
-        "value": steps_per_s,
+        "value": r["steps_per_s"],
         "unit": "walk steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": r["wall"] * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -193,15 +293,15 @@ def main():
             "workload": DESCR[args.workload],
             "rects": nr,
             "walks_per_step": walks,
-            "mode": args.mode,
+            "mode": mode,
             "leaf_cap": info.leaf_cap,
             "tree_nodes": info.n_nodes,
             "tree_depth": info.depth,
         },
-        "walks_per_s": walks_per_s,
-        "hops_per_walk": total_hops / n_walks,
+        "walks_per_s": r["walks_per_s"],
+        "hops_per_walk": r["hops"] / r["walks"],
         "fraction_other": f,
-        "escaped": int(c[nr]) / n_walks,
+        "escaped": int(r["counts"][nr]) / r["walks"],
         "tree_build_ms": info.build_ms,
         "time_to_1pct_s": t1pct,
         "walks_for_1pct": n1,
@@ -213,12 +313,39 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "kernel": "walk_kernel",
-            "bytes_per_hop": bytes_per_hop,
-            "node_records_per_hop": tr_nodes / max(tr_hops, 1),
-            "rect_records_per_hop": tr_rects / max(tr_hops, 1),
+            "bytes_per_hop": bph,
+            "node_records_per_hop": nph,
+            "rect_records_per_hop": rph,
             "avg_launch_ms": avg_ms,
         },
     }
+    if mode == "learned":
+        out["learned"] = dict(train_info, alpha=args.alpha,
+                              accept_rate=r["accepted"] / max(1, r["accepted"] + r["fallback"]))
+    if args.compare:
+        # the same layout and master in every mode, walk ids from 2^31 (disjoint)
+        cw = min(walks, args.compare_walks)
+        comp = {}
+        for m in ("bisect", "exact", "learned"):
+            q = walk_mode(frw, torch, tree, mlp, m, master, cw, 1, 1, stream, args.alpha,
+                          block=args.block, first_base=2 << 30)
+            fm = fraction_other(q["counts"], master, q["walks"])
+            setup = info.build_ms / 1e3 + (train_info["harvest_s"] + train_info["train_s"]
+                                           if m == "learned" else 0.0)
+            comp[m] = {"walk_steps_per_s": q["steps_per_s"], "walks_per_s": q["walks_per_s"],
+                       "hops_per_walk": q["hops"] / q["walks"], "fraction_other": fm,
+                       "time_to_1pct_s": setup + walks_for_1pct(fm) / q["walks_per_s"]}
+            if m == "learned":
+                comp[m]["accept_rate"] = q["accepted"] / max(1, q["accepted"] + q["fallback"])
+            log(f"compare {m}: {q['steps_per_s']:.3e} steps/s, {q['walks_per_s']:.3e} walks/s")
+        comp["walks"] = cw
+        comp["plain"] = "bisect: the reference's bisection square search on the same device tree"
+        for m in ("exact", "learned"):
+            comp[f"{m}_vs_plain_walks_per_s"] = comp[m]["walks_per_s"] / comp["bisect"]["walks_per_s"]
+            comp[f"{m}_vs_plain_steps_per_s"] = (comp[m]["walk_steps_per_s"] /
+                                                 comp["bisect"]["walk_steps_per_s"])
+        comp["learned_vs_exact_walks_per_s"] = comp["learned"]["walks_per_s"] / comp["exact"]["walks_per_s"]
+        out["comparison"] = comp
     if not args.no_cpu:
         log("cpu baseline ...")
         cb = cpu_baseline(lay.rects(), lay.bounds(), master, args.cpu_seconds)
@@ -226,6 +353,80 @@ def main():
         out["cpu_baseline"] = cb
     else:
         out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def bench_batched(args, frw, torch, np):
+    """64 layouts, rect counts log-uniform in [1e2, 1e5] (density of the 1e4 /
+    1e5-box layout, sides U[10,500]), master 0 each, walks per layout
+    U[1e5, 1e6]: one persistent launch per step, captured in a graph, launch
+    shape from the autotuner."""
+    rng = np.random.default_rng(1)
+    n_lay = 64
+    sizes = np.exp(rng.uniform(np.log(1e2), np.log(1e5), n_lay)).astype(int)
+    wl = rng.integers(100_000, 1_000_001, n_lay)
+    trees, jobs = [], []
+    t0 = time.perf_counter()
+    for i, (n, w) in enumerate(zip(sizes, wl)):
+        box = int(1000 * math.sqrt(n))
+        lay = frw.Layout.generate(int(n), box, 10, 500, seed=100 + i)
+        trees.append(frw.Tree(lay, leaf_cap=args.leaf_cap))
+        jobs.append({"tree": i, "master": 0, "n_walks": int(w), "job": i})
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    log(f"{n_lay} layouts built in {build_s:.2f} s; {int(wl.sum())} walks per step")
+    batch = frw.Batch(trees, jobs)
+    mode = frw.MODE_EXACT if (args.mode or "exact") == "exact" else frw.MODE_BISECT
+    shape, rate = batch.autotune(seed=1, mode=mode, use_graph=False)
+    log(f"autotune: {shape} at {rate:.3e} steps/s")
+    counts = torch.zeros(batch.counts_size(), dtype=torch.int64, device="cuda")
+    hops = torch.zeros(n_lay, dtype=torch.int64, device="cuda")
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+
+    def launch(step):
+        batch.walk(counts, hops, seed=1 + step, mode=mode, block=shape["block"], grid=shape["grid"],
+                   use_graph=True, stream=sh)
+
+    for w in range(args.warmup):
+        launch(w)
+    torch.cuda.synchronize()
+    counts.zero_()
+    hops.zero_()
+    torch.cuda.synchronize()
+    wall, launch_ms = time_walks(torch, lambda k: launch(args.warmup + k), args.steps, 0, stream)
+    batch.check()
+    total_hops = int(hops.sum().item())
+    n_walks = int(wl.sum()) * args.steps
+    c = counts.cpu().numpy()
+    # time to 1% for every job: the slowest job's fraction sets the walk count
+    worst = 0.0
+    for j in range(n_lay):
+        off = batch.counts_offset(j)
+        fj = 1.0 - c[off] / (wl[j] * args.steps)
+        worst = max(worst, walks_for_1pct(fj) / wl[j])
+    out = {
+        "metric": "walk steps/s",
+        "value": total_hops / wall,
+        "unit": "walk steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": DESCR["c4"], "layouts": n_lay, "walks_per_step": int(wl.sum()),
+                   "mode": args.mode or "exact", "autotuned": shape, "graph": True},
+        "walks_per_s": n_walks / wall,
+        "hops_per_walk": total_hops / n_walks,
+        "build_s": build_s,
+        "steps_for_all_jobs_1pct": worst,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
     print(json.dumps(out), flush=True)
 
 

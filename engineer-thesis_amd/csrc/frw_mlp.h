@@ -12,7 +12,7 @@
 // column (verification).
 //
 // Features of a point p whose previous square had half-size h:
-//   Lmax = min(K0, 12), level l = clamp(K0 - floor(log2 max(h, 1)) + 1, 0, Lmax),
+//   Lmax = min(K0, 15), level l = clamp(K0 - floor(log2 max(h, 1)) + 1, 0, Lmax),
 //   cell s = s_l, (ci, cj) = floor((p - bounds corner) / s);
 //   f[v*8+u] = occupancy of cell (ci-4+u, cj-4+v) at level l (1 outside the
 //   grid), f[64] = l / 16, f[65], f[66] = fractional position of p in its
@@ -20,7 +20,7 @@
 // Output y estimates log2(d* / s); the proposed half-size is d = s 2^y alpha.
 //
 // Verification of the square [p - d, p + d]^2 (closed, as the walk builds it):
-// level v = clamp(K0 - floor(log2 d) + 3, 0, Vmax), so cells are at most d/8;
+// level v = clamp(K0 - floor(log2 d) + 4, 0, Vmax), so cells are at most d/16;
 // the square is certified empty when it lies inside the bounds and the cell
 // block covering it, floor((corner - bounds corner) / s_v) per axis, holds no
 // occupied cell (four loads from the summed-area table).  Any rect point in
@@ -38,9 +38,9 @@ constexpr int kMlpIn = 67;   // features (window 64, level, position in cell x a
 constexpr int kMlpInPad = 72;   // K of the first layer, a multiple of 8
 constexpr int kMlpHid = 64;
 constexpr int kMlpStride = 72;  // LDS row stride (bf16 elements) of activations
-constexpr int kMlpMaxLevel = 12;  // feature-window levels
+constexpr int kMlpMaxLevel = 15;  // feature-window levels
 constexpr int kPyrMaxLevel = 15;  // bitmap / summed-area-table levels
-constexpr int kCertShift = 3;     // verification cells are at most d / 2^3
+constexpr int kCertShift = 4;     // verification cells are at most d / 2^4
 
 // FP32 parameter block layout (row-major [out][in]).
 constexpr int kOffW1 = 0;
@@ -158,7 +158,7 @@ __device__ inline void mlp_feature_row(uint16_t* row, uint64_t bits, int level, 
 
 // Verification probe: true when the closed square (x0, y0, x1, y1) of
 // half-size d (> 0) lies in the bounds and its covering cell block at level
-// clamp(K0 - floor(log2 d) + 3, 0, Vmax) holds no occupied cell.
+// clamp(K0 - floor(log2 d) + 4, 0, Vmax) holds no occupied cell.
 __device__ inline bool mlp_certify(const MlpView& m, double x0, double y0, double x1, double y1, double d)
 {
     if (!in_bounds_box(m.bounds, x0, y0, x1, y1)) return false;

@@ -121,11 +121,22 @@ __global__ void sat_cols(int64_t n, uint32_t* __restrict__ S)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i > n) return;
+    const int64_t w = n + 1;
     uint32_t run = 0;
-    for (int64_t j = 1; j <= n; ++j) {
-        uint32_t* e = S + j * (n + 1) + i;
-        run += *e;
-        *e = run;
+    int64_t j = 1;
+    for (; j + 8 <= n + 1; j += 8) {  // eight loads in flight per lane
+        uint32_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = S[(j + k) * w + i];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            run += v[k];
+            S[(j + k) * w + i] = run;
+        }
+    }
+    for (; j <= n; ++j) {
+        run += S[j * w + i];
+        S[j * w + i] = run;
     }
 }
 
@@ -439,12 +450,12 @@ int frw_mlp_create(const frw_tree* t, uint64_t seed, frw_mlp** out)
     const int64_t W = (int64_t)t->bounds.z - t->bounds.x, H = (int64_t)t->bounds.w - t->bounds.y;
     while (((int64_t)1 << m->K0) <= std::max(W, H)) ++m->K0;
     m->Lmax = std::min(m->K0, kMlpMaxLevel);
-    // verification levels: finest cell about 1/64 of the mean rect spacing
+    // verification levels: finest cell about 1/128 of the mean rect spacing
     // sqrt(W H / n), at most level 15 and at least the window levels
     {
         const double spacing = std::sqrt((double)std::max<int64_t>(W, 1) * (double)std::max<int64_t>(H, 1) /
                                          (double)std::max<int64_t>(t->n_rects, 1));
-        const double target = std::max(1.0, spacing / 64.0);
+        const double target = std::max(1.0, spacing / 128.0);
         const int v = m->K0 - (int)std::floor(std::log2(target));
         m->Vmax = std::max(0, std::min({v, m->K0, kPyrMaxLevel}));
     }

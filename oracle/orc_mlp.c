@@ -6,7 +6,7 @@
  * implement, so the GPU features, forward pass and gradients can be checked:
  *
  * Features of a point p with previous square half-size h (67 values):
- *   K0  = smallest k with 2^k > max(W, H) of the bounds, Lmax = min(K0, 12)
+ *   K0  = smallest k with 2^k > max(W, H) of the bounds, Lmax = min(K0, 15)
  *   e   = floor(log2(max(h, 1))), level l = clamp(K0 - e + 1, 0, Lmax),
  *   s   = 2^(K0 - l) (cell size), q = (p - bounds corner) / s, (ci, cj) = floor(q)
  *   f[v*8+u] = 1 if cell (ci-4+u, cj-4+v) meets a rect (closed rect against
@@ -17,9 +17,9 @@
  * distance.
  *
  * Verification of a square of half-size d centred at p (closed):
- *   Vmax = clamp(K0 - floor(log2(max(1, spacing / 64))), 0, min(K0, 15)),
+ *   Vmax = clamp(K0 - floor(log2(max(1, spacing / 128))), 0, min(K0, 15)),
  *          spacing = sqrt(W H / n_rects)
- *   v = clamp(K0 - floor(log2 d) + 3, 0, Vmax), s_v = 2^(K0 - v)
+ *   v = clamp(K0 - floor(log2 d) + 4, 0, Vmax), s_v = 2^(K0 - v)
  *   certified when the square is inside the bounds (inclusive), its covering
  *   cells floor((corner - bounds corner) / s_v) lie in the 2^v grid, and no
  *   rect meets that cell block.
@@ -86,7 +86,7 @@ static int k0_of(const orc_box* b)
 int orc_mlp_level(const orc_box* bounds, double h, double* cell)
 {
     const int K0 = k0_of(bounds);
-    const int Lmax = K0 < 12 ? K0 : 12;
+    const int Lmax = K0 < 15 ? K0 : 15;
     const int e = h >= 1.0 ? ilogb(h) : 0;
     int l = K0 - e + 1;
     if (l < 0) l = 0;
@@ -141,7 +141,7 @@ int orc_mlp_vmax(const orc_box* bounds, int n_rects)
     if (W < 1) W = 1;
     if (H < 1) H = 1;
     const double spacing = sqrt(W * H / (double)(n_rects > 0 ? n_rects : 1));
-    const double target = spacing / 64.0 > 1.0 ? spacing / 64.0 : 1.0;
+    const double target = spacing / 128.0 > 1.0 ? spacing / 128.0 : 1.0;
     int v = K0 - (int)floor(log2(target));
     if (v > K0) v = K0;
     if (v > 15) v = 15;
@@ -156,7 +156,7 @@ int orc_mlp_certify(const double* xyxy, int n, const orc_box* bounds, double px,
     const double x0 = px - d, y0 = py - d, x1 = px + d, y1 = py + d;
     if (!(y0 >= bounds->y0 && x0 >= bounds->x0 && y1 <= bounds->y1 && x1 <= bounds->x1)) return 0;
     const int K0 = k0_of(bounds), Vmax = orc_mlp_vmax(bounds, n);
-    int v = K0 - ilogb(d) + 3;
+    int v = K0 - ilogb(d) + 4;
     if (v < 0) v = 0;
     if (v > Vmax) v = Vmax;
     const double s = ldexp(1.0, K0 - v);

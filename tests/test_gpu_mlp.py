@@ -47,22 +47,38 @@ def setup():
 def test_pyramid_levels(setup):
     lay, rects, bnds, tree, mlp = setup
     K0, Lmax, Vmax, Pmax = mlp.levels()
-    assert K0 == _k0(bnds) and Lmax == min(K0, 12)
+    assert K0 == _k0(bnds) and Lmax == min(K0, 15)
     assert Vmax == O.mlp_vmax(bnds, len(rects)) and Pmax == max(Lmax, Vmax)
 
 
 def test_occupancy_bitmaps(setup):
     lay, rects, bnds, tree, mlp = setup
     K0 = _k0(bnds)
-    for lvl in range(0, mlp.levels()[3] + 1):
+    R = rects.astype(np.int64)
+    ox, oy = int(bnds[0]), int(bnds[1])
+    Pmax = mlp.levels()[3]
+    for lvl in range(0, min(Pmax, 12) + 1):
         n = 1 << lvl
         s = 2 ** (K0 - lvl)
         g = np.zeros((n, n), dtype=np.uint8)
-        for x0, y0, x1, y1 in rects.astype(np.int64):
-            i0, i1 = (x0 - int(bnds[0])) // s, min((x1 - int(bnds[0])) // s, n - 1)
-            j0, j1 = (y0 - int(bnds[1])) // s, min((y1 - int(bnds[1])) // s, n - 1)
+        for x0, y0, x1, y1 in R:
+            i0, i1 = (x0 - ox) // s, min((x1 - ox) // s, n - 1)
+            j0, j1 = (y0 - oy) // s, min((y1 - oy) // s, n - 1)
             g[j0:j1 + 1, i0:i1 + 1] = 1
         assert np.array_equal(mlp.occupancy(lvl), g), lvl
+    # the finest level on sampled cells, against every rect
+    lvl = Pmax
+    s = 2 ** (K0 - lvl)
+    g = mlp.occupancy(lvl)
+    rng = np.random.default_rng(1)
+    cells = rng.integers(0, 1 << lvl, (4000, 2))
+    sel = R[rng.integers(0, len(R), 1000)]  # plus cells under rect corners
+    cells = np.vstack([cells, np.column_stack([(sel[:, 0] - ox) // s, (sel[:, 1] - oy) // s])])
+    for i, j in cells:
+        a, b = i * s, (i + 1) * s
+        c, d = j * s, (j + 1) * s
+        occ = np.any((R[:, 0] - ox < b) & (R[:, 2] - ox >= a) & (R[:, 1] - oy < d) & (R[:, 3] - oy >= c))
+        assert g[j, i] == occ, (i, j)
 
 
 def test_features_bit_exact(setup):

@@ -72,7 +72,7 @@ def test_level_and_cell():
     assert O.mlp_level((0, 0, 1024, 1024), 0.25) == (11, 1.0)      # h < 1 counts as 1
     assert O.mlp_level((0, 0, 1024, 1024), 37.0) == (7, 16.0)      # e = 5 -> l = 7, s = 2^4
     assert O.mlp_level((0, 0, 1024, 1024), 5000.0) == (0, 2048.0)  # clamp at 0, s = 2^K0
-    assert O.mlp_level((0, 0, 10 ** 7, 10 ** 7), 3.0) == (12, 2.0 ** 12)  # Lmax = 12 cap
+    assert O.mlp_level((0, 0, 10 ** 7, 10 ** 7), 3.0) == (15, 2.0 ** 9)  # Lmax = 15 cap (K0 = 24)
 
 
 def _raster(rects, bnds, level, K0):
@@ -179,8 +179,9 @@ def test_rounded_forward_matches_torch_bf16_emulation():
 
 
 def test_vmax_rule():
-    # spacing = sqrt(W H / n); finest verification cell about spacing / 64
-    assert O.mlp_vmax((0, 0, 100000, 100000), 10000) == 17 - 3   # spacing 1000 -> 15.6 -> 2^3
+    # spacing = sqrt(W H / n); finest verification cell about spacing / 128
+    assert O.mlp_vmax((0, 0, 100000, 100000), 10000) == 17 - 2   # spacing 1000 -> 7.8 -> 2^2
+    assert O.mlp_vmax((0, 0, 60000, 60000), 3000) == 16 - 3      # spacing 1095 -> 8.6 -> 2^3
     assert O.mlp_vmax((0, 0, 10 ** 7, 10 ** 7), 10 ** 6) == 15   # capped at 15
     assert O.mlp_vmax((0, 0, 100, 100), 10 ** 6) == 7            # cell >= 1 -> level K0
 
@@ -207,7 +208,7 @@ def test_certify_against_rasterisation_and_exact_q1():
             continue
         sq = (px - d, py - d, px + d, py + d)
         inside = sq[0] >= b[0] and sq[1] >= b[1] and sq[2] <= b[2] and sq[3] <= b[3]
-        v = min(max(K0 - int(np.floor(np.log2(d))) + 3, 0), Vmax)
+        v = min(max(K0 - int(np.floor(np.log2(d))) + 4, 0), Vmax)
         s = 2.0 ** (K0 - v)
         g = _raster(rects, b, v, K0)
         i0, i1 = int(np.floor((sq[0] - b[0]) / s)), int(np.floor((sq[2] - b[0]) / s))
