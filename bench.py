@@ -67,6 +67,8 @@ def parse_args():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB three times (scales the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -242,6 +244,15 @@ def main():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
     torch.cuda.init()
+    if args.copy_calibration:
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied 3 x 256 MiB")
+        return
     if args.workload == "c4":
         return bench_batched(args, frw, torch, np)
     walks = args.walks or WORKLOADS[args.workload][7]

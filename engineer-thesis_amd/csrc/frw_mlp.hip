@@ -24,7 +24,7 @@ struct frw_mlp {
     float* d_adam_m = nullptr;
     float* d_adam_v = nullptr;
     float* d_grad = nullptr;
-    float* d_gs = nullptr;         // weight-gradient scratch, 64 x 72
+    float* d_gs = nullptr;         // weight-gradient split partials, kMaxSplits x 64 x 72
     int64_t adam_step = 0;
     float h_b4 = 0.0f;             // host copy of the output bias (kept in sync)
     // training set
@@ -34,6 +34,7 @@ struct frw_mlp {
     // training scratch (batch-sized)
     int64_t ws_batch = 0;
     float *d_Z1 = nullptr, *d_Z2 = nullptr, *d_Z3 = nullptr, *d_F = nullptr;
+    float *d_dy = nullptr, *d_lossv = nullptr;
     uint16_t *d_A1 = nullptr, *d_A2 = nullptr, *d_A3 = nullptr, *d_D = nullptr, *d_Xb = nullptr;
     float* d_tb = nullptr;
     float* d_loss = nullptr;
@@ -48,6 +49,7 @@ constexpr int kBfW2 = kBfW1 + kMlpHid * kMlpInPad;
 constexpr int kBfW3 = kBfW2 + kMlpHid * kMlpHid;
 constexpr int kBfW4 = kBfW3 + kMlpHid * kMlpHid;
 constexpr int kBfCount = kBfW4 + kMlpHid;
+constexpr int kMaxSplits = 16;  // batch slices of the weight-gradient products
 
 // ---------------------------------------------------------------- occupancy
 // One lane per rect: sets the bits of every cell the closed rect meets, one
@@ -177,17 +179,23 @@ __global__ void features_kernel(MlpView m, const double* __restrict__ pts, const
 // ---------------------------------------------------------------- GEMM
 // C[m][n] = sum_k A(m,k) * B(k,n) with bf16 operands and FP32 accumulation;
 // A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn].  One wave per 32x32
-// tile of C; K a multiple of 8; rows/columns beyond M/N are masked.
+// tile of C; rows/columns beyond M/N are masked.  Split K: blockIdx.y takes
+// k in [y*kc, (y+1)*kc) and writes its partial product to C + y*cstride
+// (kc a multiple of 8).
 __global__ void __launch_bounds__(64) gemm_bf16(const uint16_t* __restrict__ A, int64_t sam, int64_t sak,
                                                 const uint16_t* __restrict__ B, int64_t sbk, int64_t sbn,
-                                                float* __restrict__ C, int64_t ldc, int M, int N, int K)
+                                                float* __restrict__ C, int64_t ldc, int M, int N, int K,
+                                                int kc, int64_t cstride)
 {
     const int tiles_n = (N + 31) / 32;
     const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
     const int lane = threadIdx.x, r32 = lane & 31, q = lane >> 5;
     const int m = tm * 32 + r32, nn = tn * 32 + r32;
+    const int kb = blockIdx.y * kc;
+    const int ke = min(K, kb + kc);
+    float* Cs = C + blockIdx.y * cstride;
     f32x16 acc = {0};
-    for (int k0 = 0; k0 < K; k0 += 8) {
+    for (int k0 = kb; k0 < ke; k0 += 8) {
         bf16x4 a, b;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -200,8 +208,18 @@ __global__ void __launch_bounds__(64) gemm_bf16(const uint16_t* __restrict__ A, 
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int row = tm * 32 + 8 * (r >> 2) + 4 * q + (r & 3);
-        if (row < M && nn < N) C[row * ldc + nn] = acc[r];
+        if (row < M && nn < N) Cs[row * ldc + nn] = acc[r];
     }
+}
+
+// dst[i] += sum over the split partials src[y*n + i], y = 0..splits-1, in order.
+__global__ void add_splits(float* __restrict__ dst, const float* __restrict__ src, int n, int splits)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float acc = 0.0f;
+    for (int y = 0; y < splits; ++y) acc += src[(int64_t)y * n + i];
+    dst[i] += acc;
 }
 
 // Z += bias; A = bf16(gelu(Z)).  Z and A are [B][64] / [B][64].
@@ -215,13 +233,14 @@ __global__ void bias_gelu(float* __restrict__ Z, const float* __restrict__ bias,
     A[i] = f2bf(gelu_f(z));
 }
 
-// Output layer, loss and its gradients: y = w4.a3 + b4, r = y - t,
-// D = bf16(r/B * w4 * gelu'(z3)); dw4, db4 and the loss accumulate atomically.
+// Output layer: y = w4.a3 + b4, r = y - t; per sample dy = r/B and the loss
+// term r^2/(2B); D = bf16(dy * w4 * gelu'(z3)).  The sums over samples are
+// taken by out_grads.
 __global__ void output_layer(const uint16_t* __restrict__ A3, const float* __restrict__ Z3,
                              const uint16_t* __restrict__ w4, const float* __restrict__ b4p,
-                             const float* __restrict__ t,
-                             int B, float* __restrict__ y_out, uint16_t* __restrict__ D,
-                             float* __restrict__ grad, float* __restrict__ loss)
+                             const float* __restrict__ t, int B, float* __restrict__ y_out,
+                             uint16_t* __restrict__ D, float* __restrict__ dy_out,
+                             float* __restrict__ loss_out)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= B) return;
@@ -231,12 +250,44 @@ __global__ void output_layer(const uint16_t* __restrict__ A3, const float* __res
     if (y_out) y_out[s] = y;
     if (!t) return;
     const float r = y - t[s];
-    atomicAdd(loss, 0.5f * r * r / (float)B);
     const float dy = r / (float)B;
-    atomicAdd(&grad[kOffB4], dy);
-    for (int j = 0; j < kMlpHid; ++j) {
-        atomicAdd(&grad[kOffW4 + j], dy * bf2f(A3[s * kMlpHid + j]));
+    dy_out[s] = dy;
+    loss_out[s] = 0.5f * r * r / (float)B;
+    for (int j = 0; j < kMlpHid; ++j)
         D[s * kMlpHid + j] = f2bf(dy * bf2f(w4[j]) * gelu_grad_f(Z3[s * kMlpHid + j]));
+}
+
+// Block sum of one value per lane (256 lanes).
+__device__ inline float block_sum256(float v, float* sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// Sums over the batch, one block per output: blocks 0..63 dw4[j] = sum dy a3[j],
+// block 64 db4 = sum dy, block 65 the loss.
+__global__ void __launch_bounds__(256) out_grads(const uint16_t* __restrict__ A3, const float* __restrict__ dy,
+                                                 const float* __restrict__ lossv, int B,
+                                                 float* __restrict__ grad, float* __restrict__ loss)
+{
+    __shared__ float sh[256];
+    const int j = blockIdx.x;
+    float acc = 0.0f;
+    for (int s = threadIdx.x; s < B; s += 256) {
+        if (j < kMlpHid) acc += dy[s] * bf2f(A3[s * kMlpHid + j]);
+        else if (j == kMlpHid) acc += dy[s];
+        else acc += lossv[s];
+    }
+    const float tot = block_sum256(acc, sh);
+    if (threadIdx.x == 0) {
+        if (j < kMlpHid) grad[kOffW4 + j] += tot;
+        else if (j == kMlpHid) grad[kOffB4] += tot;
+        else *loss += tot;
     }
 }
 
@@ -249,13 +300,15 @@ __global__ void back_gelu(const float* __restrict__ F, const float* __restrict__
     D[i] = f2bf(F[i] * gelu_grad_f(Z[i]));
 }
 
-// Bias gradient: column sums of D [B][64].
-__global__ void bias_grad(const uint16_t* __restrict__ D, int B, float* __restrict__ g)
+// Bias gradient: column sums of D [B][64], one block per column.
+__global__ void __launch_bounds__(256) bias_grad(const uint16_t* __restrict__ D, int B, float* __restrict__ g)
 {
-    const int c = threadIdx.x;
+    __shared__ float sh[256];
+    const int c = blockIdx.x;
     float acc = 0.0f;
-    for (int s = 0; s < B; ++s) acc += bf2f(D[s * kMlpHid + c]);
-    g[c] += acc;
+    for (int s = threadIdx.x; s < B; s += 256) acc += bf2f(D[s * kMlpHid + c]);
+    const float tot = block_sum256(acc, sh);
+    if (threadIdx.x == 0) g[c] += tot;
 }
 
 __global__ void adam_step(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ mom,
@@ -269,12 +322,6 @@ __global__ void adam_step(float* __restrict__ p, const float* __restrict__ g, fl
     mom[i] = m;
     vel[i] = v;
     p[i] -= lr * (m / b1c) / (sqrtf(v / b2c) + 1e-8f);
-}
-
-__global__ void add_into(float* __restrict__ dst, const float* __restrict__ src, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] += src[i];
 }
 
 __global__ void widen_bf16(const uint16_t* __restrict__ X, float* __restrict__ F, int64_t total)
@@ -366,7 +413,7 @@ int ensure_ws(frw_mlp* m, int64_t B)
     if (m->ws_batch >= B) return FRW_OK;
     hipFree(m->d_Z1); hipFree(m->d_Z2); hipFree(m->d_Z3); hipFree(m->d_F);
     hipFree(m->d_A1); hipFree(m->d_A2); hipFree(m->d_A3); hipFree(m->d_D); hipFree(m->d_Xb);
-    hipFree(m->d_tb); hipFree(m->d_idx);
+    hipFree(m->d_tb); hipFree(m->d_idx); hipFree(m->d_dy); hipFree(m->d_lossv);
     const size_t h = (size_t)B * kMlpHid;
     int rc = FRW_OK;
     rc = rc ? rc : check_hip(hipMalloc(&m->d_Z1, h * 4), "mlp ws");
@@ -380,6 +427,8 @@ int ensure_ws(frw_mlp* m, int64_t B)
     rc = rc ? rc : check_hip(hipMalloc(&m->d_Xb, (size_t)B * kMlpInPad * 2), "mlp ws");
     rc = rc ? rc : check_hip(hipMalloc(&m->d_tb, (size_t)B * 4), "mlp ws");
     rc = rc ? rc : check_hip(hipMalloc(&m->d_idx, (size_t)B * 4), "mlp ws");
+    rc = rc ? rc : check_hip(hipMalloc(&m->d_dy, (size_t)B * 4), "mlp ws");
+    rc = rc ? rc : check_hip(hipMalloc(&m->d_lossv, (size_t)B * 4), "mlp ws");
     if (rc == FRW_OK) m->ws_batch = B;
     return rc;
 }
@@ -388,7 +437,23 @@ void gemm(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_
           int64_t ldc, int M, int N, int K, hipStream_t s)
 {
     const int tiles = ((M + 31) / 32) * ((N + 31) / 32);
-    gemm_bf16<<<tiles, 64, 0, s>>>(A, sam, sak, B, sbk, sbn, C, ldc, M, N, K);
+    gemm_bf16<<<dim3(tiles, 1), 64, 0, s>>>(A, sam, sak, B, sbk, sbn, C, ldc, M, N, K, K, 0);
+}
+
+// Weight gradient dst (M x N, row-major) += A^T-shaped product over the batch
+// (K = B rows), split over kMaxSplits slices of the batch into the scratch and
+// summed in a fixed order.
+void gemm_wgrad(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk, int64_t sbn,
+                float* dst, float* scratch, int M, int N, int K, hipStream_t s)
+{
+    int splits = std::max(1, std::min(kMaxSplits, K / 64));
+    int kc = (K + splits - 1) / splits;
+    kc = (kc + 7) / 8 * 8;
+    splits = (K + kc - 1) / kc;
+    const int tiles = ((M + 31) / 32) * ((N + 31) / 32);
+    gemm_bf16<<<dim3(tiles, splits), 64, 0, s>>>(A, sam, sak, B, sbk, sbn, scratch, N, M, N, K, kc,
+                                                 (int64_t)M * N);
+    add_splits<<<blocks((int64_t)M * N, 256), 256, 0, s>>>(dst, scratch, M * N, splits);
 }
 
 // Forward + backward of batch (Xb, tb) of size B (a multiple of 32): adds the
@@ -402,7 +467,7 @@ int gradient(frw_mlp* m, const uint16_t* Xb, const float* tb, int B, hipStream_t
     const uint16_t* w4 = m->d_bf + kBfW4;
     float* P = m->d_params;
     float* G = m->d_grad;
-    float* GS = m->d_gs;  // 64 x 72 weight-gradient scratch
+    float* GS = m->d_gs;  // split-K partials of the weight gradients
     const int64_t h = (int64_t)B * kMlpHid;
     const int H = kMlpHid, IN = kMlpInPad;
     // forward: Z = A_prev * W^T + b, A = bf16(gelu(Z))
@@ -413,24 +478,22 @@ int gradient(frw_mlp* m, const uint16_t* Xb, const float* tb, int B, hipStream_t
     gemm(m->d_A2, H, 1, W3, 1, H, m->d_Z3, H, B, H, H, s);
     bias_gelu<<<blocks(h, 256), 256, 0, s>>>(m->d_Z3, P + kOffB3, m->d_A3, h);
     // output layer: loss, dw4, db4 and D3
-    output_layer<<<blocks(B, 256), 256, 0, s>>>(m->d_A3, m->d_Z3, w4, P + kOffB4, tb, B, nullptr, m->d_D, G,
-                                                m->d_loss);
+    output_layer<<<blocks(B, 256), 256, 0, s>>>(m->d_A3, m->d_Z3, w4, P + kOffB4, tb, B, nullptr, m->d_D,
+                                                m->d_dy, m->d_lossv);
+    out_grads<<<kMlpHid + 2, 256, 0, s>>>(m->d_A3, m->d_dy, m->d_lossv, B, G, m->d_loss);
     // layer 3: dW3 = D3^T A2, db3 = colsum D3, F = D3 W3, D2 = bf16(F gelu'(Z2))
-    gemm(m->d_D, 1, H, m->d_A2, H, 1, GS, H, H, H, B, s);
-    add_into<<<blocks(H * H, 256), 256, 0, s>>>(G + kOffW3, GS, H * H);
-    bias_grad<<<1, H, 0, s>>>(m->d_D, B, G + kOffB3);
+    gemm_wgrad(m->d_D, 1, H, m->d_A2, H, 1, G + kOffW3, GS, H, H, B, s);
+    bias_grad<<<H, 256, 0, s>>>(m->d_D, B, G + kOffB3);
     gemm(m->d_D, H, 1, W3, H, 1, m->d_F, H, B, H, H, s);
     back_gelu<<<blocks(h, 256), 256, 0, s>>>(m->d_F, m->d_Z2, m->d_D, h);
     // layer 2
-    gemm(m->d_D, 1, H, m->d_A1, H, 1, GS, H, H, H, B, s);
-    add_into<<<blocks(H * H, 256), 256, 0, s>>>(G + kOffW2, GS, H * H);
-    bias_grad<<<1, H, 0, s>>>(m->d_D, B, G + kOffB2);
+    gemm_wgrad(m->d_D, 1, H, m->d_A1, H, 1, G + kOffW2, GS, H, H, B, s);
+    bias_grad<<<H, 256, 0, s>>>(m->d_D, B, G + kOffB2);
     gemm(m->d_D, H, 1, W2, H, 1, m->d_F, H, B, H, H, s);
     back_gelu<<<blocks(h, 256), 256, 0, s>>>(m->d_F, m->d_Z1, m->d_D, h);
     // layer 1: dW1 = D1^T X ([64][72]), db1
-    gemm(m->d_D, 1, H, Xb, IN, 1, GS, IN, H, IN, B, s);
-    add_into<<<blocks(H * IN, 256), 256, 0, s>>>(G + kOffW1, GS, H * IN);
-    bias_grad<<<1, H, 0, s>>>(m->d_D, B, G + kOffB1);
+    gemm_wgrad(m->d_D, 1, H, Xb, IN, 1, G + kOffW1, GS, H, IN, B, s);
+    bias_grad<<<H, 256, 0, s>>>(m->d_D, B, G + kOffB1);
     return check_hip(hipGetLastError(), "mlp gradient");
 }
 
@@ -512,7 +575,7 @@ int frw_mlp_create(const frw_tree* t, uint64_t seed, frw_mlp** out)
     fill(kOffW4, 1, kMlpHid, kMlpHid, 1, 4);
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_params, kMlpParams * 4), "hipMalloc params");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_grad, kMlpParams * 4), "hipMalloc grad");
-    if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_gs, kMlpHid * kMlpInPad * 4), "hipMalloc grad scratch");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_gs, (size_t)kMaxSplits * kMlpHid * kMlpInPad * 4), "hipMalloc grad scratch");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_adam_m, kMlpParams * 4), "hipMalloc adam");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_adam_v, kMlpParams * 4), "hipMalloc adam");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_bf, kBfCount * 2), "hipMalloc bf16");
@@ -539,7 +602,7 @@ void frw_mlp_free(frw_mlp* m)
                     (void*)m->d_adam_v, (void*)m->d_grad, (void*)m->d_gs, (void*)m->d_X, (void*)m->d_t, (void*)m->d_Z1,
                     (void*)m->d_Z2, (void*)m->d_Z3, (void*)m->d_F, (void*)m->d_A1, (void*)m->d_A2,
                     (void*)m->d_A3, (void*)m->d_D, (void*)m->d_Xb, (void*)m->d_tb, (void*)m->d_loss,
-                    (void*)m->d_idx})
+                    (void*)m->d_idx, (void*)m->d_dy, (void*)m->d_lossv})
         if (p) hipFree(p);
     delete m;
 }
