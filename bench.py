@@ -28,20 +28,20 @@ HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     # name: (n rects, box, side_min, side_max, cluster_frac, sigma, n_clusters, walks per step)
     "c2": (10_000, 100_000, 10, 500, 0.0, 0.0, 0, 10_000_000),
-    "c3": (1_000_000, 10_000_000, 5, 300, 0.2, 1e5, 100, 10_000_000),
+    "c3": (1_000_000, 10_000_000, 5, 300, 0.2, 1e5, 100, 100_000_000),
     "c1": (100, 10_000, 10, 200, 0.0, 0.0, 0, 1_000_000),
-    "c5": (1_000_000, 10_000_000, 5, 300, 0.2, 1e5, 100, 10_000_000),
+    "c5": (1_000_000, 10_000_000, 5, 300, 0.2, 1e5, 100, 100_000_000),
 }
 DESCR = {
     "c2": "plain GPU FRW: 1e4 rects, box 1e5, sides U[10,500], master 0, 1e7 walks/step",
     "c3": "FRW at scale: 1e6 rects, box 1e7, sides U[5,300], 20% in 100 Gaussian blobs "
-          "(sigma 1e5), master 0, 1e7 walks/step",
+          "(sigma 1e5), master 0, 1e8 walks/step",
     "c1": "CPU-reference layout: 100 rects, box 1e4, sides U[10,200], master 0, 1e6 walks/step",
     "c4": "batched: 64 layouts, rects log-uniform in [1e2,1e5] (box 1000 sqrt(n), sides "
           "U[10,500]), master 0 each, walks per layout U[1e5,1e6], one persistent launch per "
           "step, graph replay, autotuned launch shape",
     "c5": "learned proposal at scale: the 1e6-rect layout, network trained on pairs harvested "
-          "from exact walks, master 0, 1e7 walks/step, exact fallback",
+          "from exact walks, master 0, 1e8 walks/step, exact fallback",
 }
 
 
@@ -59,8 +59,9 @@ def parse_args():
     ap.add_argument("--alpha", type=float, default=0.9, help="learned: safety factor")
     ap.add_argument("--pairs", type=int, default=2_000_000, help="learned: training pairs")
     ap.add_argument("--epochs", type=int, default=3, help="learned: training epochs")
-    ap.add_argument("--compare", action="store_true",
-                    help="also time bisect / exact / learned on the same layout")
+    ap.add_argument("--compare", action="store_true", default=True,
+                    help="also time bisect / exact / learned on the same layout (default)")
+    ap.add_argument("--no-compare", dest="compare", action="store_false")
     ap.add_argument("--compare-walks", type=int, default=2_000_000)
     ap.add_argument("--walks", type=int, default=0, help="walks per step (0 = workload default)")
     ap.add_argument("--leaf-cap", type=int, default=16)
@@ -371,7 +372,8 @@ def bench_batched(args, frw, torch, np):
     """64 layouts, rect counts log-uniform in [1e2, 1e5] (density of the 1e4 /
     1e5-box layout, sides U[10,500]), master 0 each, walks per layout
     U[1e5, 1e6]: one persistent launch per step, captured in a graph, launch
-    shape from the autotuner."""
+    shape from the autotuner.  Learned mode: one network trained on pairs
+    harvested from every layout, each job on its own layout's pyramid."""
     rng = np.random.default_rng(1)
     n_lay = 64
     sizes = np.exp(rng.uniform(np.log(1e2), np.log(1e5), n_lay)).astype(int)
@@ -387,57 +389,112 @@ def bench_batched(args, frw, torch, np):
     build_s = time.perf_counter() - t0
     log(f"{n_lay} layouts built in {build_s:.2f} s; {int(wl.sum())} walks per step")
     batch = frw.Batch(trees, jobs)
-    mode = frw.MODE_EXACT if (args.mode or "exact") == "exact" else frw.MODE_BISECT
-    shape, rate = batch.autotune(seed=1, mode=mode, use_graph=False)
-    log(f"autotune: {shape} at {rate:.3e} steps/s")
+    mode = args.mode or "exact"
+    modes = ["bisect", "exact", "learned"] if args.compare else [mode]
+    net, train_info = None, None
+    if "learned" in modes:
+        t0 = time.perf_counter()
+        pyrs = [frw.Mlp(t, seed=1, max_level=13) for t in trees]
+        per = max(1024, args.pairs // n_lay)
+        net = frw.Mlp(trees[int(np.argmax(sizes))], seed=1, max_level=13)
+        got_total = 0
+        for i, t in enumerate(trees):
+            pts = torch.empty((per, 2), dtype=torch.float64, device="cuda")
+            hh = torch.empty(per, dtype=torch.float64, device="cuda")
+            ds = torch.empty(per, dtype=torch.float64, device="cuda")
+            got = min(per, frw.walk_harvest(t, 0, max(1000, per // 6), pts, hh, ds, seed=1,
+                                            first_walk=3 << 30))
+            net.add_samples(pts[:got].contiguous(), hh[:got].contiguous(), ds[:got].contiguous(),
+                            features_of=pyrs[i])
+            got_total += got
+        t1 = time.perf_counter()
+        loss = net.train(epochs=args.epochs, batch=1024, lr=2e-3, seed=1)
+        t2 = time.perf_counter()
+        batch.set_proposal(pyrs)
+        train_info = {"pairs": got_total, "setup_s": t1 - t0, "train_s": t2 - t1, "epochs": args.epochs,
+                      "final_loss": loss, "alpha": args.alpha}
+        log(f"proposal: {got_total} pairs from {n_lay} layouts ({t1 - t0:.2f} s incl. pyramids), "
+            f"trained in {t2 - t1:.2f} s, loss {loss:.4f}")
     counts = torch.zeros(batch.counts_size(), dtype=torch.int64, device="cuda")
     hops = torch.zeros(n_lay, dtype=torch.int64, device="cuda")
+    stats = torch.zeros(2, dtype=torch.int64, device="cuda")
     stream = torch.cuda.Stream()
     sh = stream.cuda_stream
+    results = {}
+    for m in modes:
+        if m == "learned":
+            shape, rate = batch.autotune_learned(net, alpha=args.alpha, seed=1)
+        else:
+            shape, rate = batch.autotune(seed=1, mode=frw.MODE_EXACT if m == "exact" else frw.MODE_BISECT)
+        log(f"autotune {m}: {shape} at {rate:.3e} steps/s")
 
-    def launch(step):
-        batch.walk(counts, hops, seed=1 + step, mode=mode, block=shape["block"], grid=shape["grid"],
-                   use_graph=True, stream=sh)
+        def launch(step, m=m, shape=shape):
+            if m == "learned":
+                batch.walk_learned(net, counts, hops, stats, alpha=args.alpha, seed=1 + step,
+                                   block=shape["block"], grid=shape["grid"], use_graph=True, stream=sh)
+            else:
+                batch.walk(counts, hops, seed=1 + step, mode=frw.MODE_EXACT if m == "exact" else frw.MODE_BISECT,
+                           block=shape["block"], grid=shape["grid"], use_graph=True, stream=sh)
 
-    for w in range(args.warmup):
-        launch(w)
-    torch.cuda.synchronize()
-    counts.zero_()
-    hops.zero_()
-    torch.cuda.synchronize()
-    wall, launch_ms = time_walks(torch, lambda k: launch(args.warmup + k), args.steps, 0, stream)
-    batch.check()
-    total_hops = int(hops.sum().item())
-    n_walks = int(wl.sum()) * args.steps
-    c = counts.cpu().numpy()
-    # time to 1% for every job: the slowest job's fraction sets the walk count
-    worst = 0.0
-    for j in range(n_lay):
-        off = batch.counts_offset(j)
-        fj = 1.0 - c[off] / (wl[j] * args.steps)
-        worst = max(worst, walks_for_1pct(fj) / wl[j])
+        for w in range(args.warmup):
+            launch(w)
+        torch.cuda.synchronize()
+        counts.zero_()
+        hops.zero_()
+        stats.zero_()
+        torch.cuda.synchronize()
+        wall, launch_ms = time_walks(torch, lambda k: launch(args.warmup + k), args.steps, 0, stream)
+        batch.check()
+        total_hops = int(hops.sum().item())
+        n_walks = int(wl.sum()) * args.steps
+        c = counts.cpu().numpy()
+        worst = 0.0  # steps until every job's fraction is known to 1% (3 sigma)
+        for j in range(n_lay):
+            off = batch.counts_offset(j)
+            fj = 1.0 - c[off] / (wl[j] * args.steps)
+            worst = max(worst, walks_for_1pct(fj) / wl[j])
+        st = stats.cpu().numpy()
+        results[m] = {"walk_steps_per_s": total_hops / wall, "walks_per_s": n_walks / wall,
+                      "hops_per_walk": total_hops / n_walks, "wall": wall, "autotuned": shape,
+                      "steps_for_all_jobs_1pct": worst,
+                      "time_to_all_1pct_s": worst * wall / args.steps}
+        if m == "learned":
+            results[m]["accept_rate"] = int(st[0]) / max(1, int(st[0] + st[1]))
+        log(f"{m}: {results[m]['walk_steps_per_s']:.3e} steps/s, {results[m]['walks_per_s']:.3e} walks/s")
+    main_mode = "learned" if args.compare else mode
+    r = results[main_mode]
     out = {
         "metric": "walk steps/s",
-        "value": total_hops / wall,
+        "value": r["walk_steps_per_s"],
         "unit": "walk steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": r["wall"] * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": DESCR["c4"], "layouts": n_lay, "walks_per_step": int(wl.sum()),
-                   "mode": args.mode or "exact", "autotuned": shape, "graph": True},
-        "walks_per_s": n_walks / wall,
-        "hops_per_walk": total_hops / n_walks,
+                   "mode": main_mode, "autotuned": r["autotuned"], "graph": True},
+        "walks_per_s": r["walks_per_s"],
+        "hops_per_walk": r["hops_per_walk"],
         "build_s": build_s,
-        "steps_for_all_jobs_1pct": worst,
+        "steps_for_all_jobs_1pct": r["steps_for_all_jobs_1pct"],
         "roofline": None,
         "cpu_baseline": None,
     }
+    if train_info:
+        out["learned"] = train_info
+    if args.compare:
+        comp = {m: {k: v for k, v in results[m].items() if k != "wall"} for m in modes}
+        comp["plain"] = "bisect: the reference's bisection square search, same batch, same launch form"
+        for m in ("exact", "learned"):
+            comp[f"{m}_vs_plain_steps_per_s"] = results[m]["walk_steps_per_s"] / results["bisect"]["walk_steps_per_s"]
+            comp[f"{m}_vs_plain_walks_per_s"] = results[m]["walks_per_s"] / results["bisect"]["walks_per_s"]
+        comp["learned_vs_exact_walks_per_s"] = results["learned"]["walks_per_s"] / results["exact"]["walks_per_s"]
+        out["comparison"] = comp
     print(json.dumps(out), flush=True)
 
 

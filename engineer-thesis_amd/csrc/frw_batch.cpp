@@ -28,7 +28,11 @@ struct frw_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     frw_batch_params graph_params{};
+    const frw_mlp* graph_net = nullptr;  // learned-mode arguments the graph was captured with
+    float graph_alpha = 0.0f;
+    uint64_t* graph_stats = nullptr;
     hipStream_t capture = nullptr;  // private stream the graph is captured on
+    std::vector<MlpPyramid> pyrs;   // per tree; occ == nullptr until a proposal is set
 };
 
 namespace {
@@ -59,6 +63,7 @@ int upload_jobs(frw_batch* b, int64_t* counts, int64_t* hops, hipStream_t s)
         w.n_walks = (uint64_t)q.n_walks;
         w.first_walk = (uint32_t)q.first_walk;
         w.job = q.job;
+        w.pyr = b->pyrs[q.tree];
         begin += (uint64_t)q.n_walks;
     }
     int rc = check_hip(hipMemcpyAsync(b->d_jobs, tab.data(), sizeof(WalkJob) * tab.size(),
@@ -108,6 +113,7 @@ int frw_batch_create(const frw_tree* const* trees, int32_t n_trees, const frw_ba
         frw_batch* b = new frw_batch;
         hipGetDevice(&b->device);
         b->trees.assign(trees, trees + n_trees);
+        b->pyrs.assign(n_trees, MlpPyramid{});
         b->jobs.assign(jobs, jobs + n_jobs);
         for (int32_t j = 0; j < n_jobs; ++j) {
             const frw_batch_job& q = jobs[j];
@@ -159,18 +165,27 @@ int frw_batch_counts_offset(const frw_batch* b, int32_t job, int64_t* offset)
     return FRW_OK;
 }
 
-int frw_batch_walk(frw_batch* b, const frw_batch_params* p, int64_t* counts, int64_t* hops,
-                   void* stream)
+}  // extern "C"
+
+namespace {
+
+// One batched launch (or graph replay) in any mode; net/alpha/stats only for
+// the learned mode.
+int batch_walk(frw_batch* b, const frw_batch_params* p, const frw_mlp* net, float alpha, uint64_t* stats,
+               int64_t* counts, int64_t* hops, hipStream_t s)
 {
-    if (!b || !p || !counts || !hops) return set_error(FRW_E_INVALID, "null argument");
-    if (p->mode != FRW_MODE_EXACT && p->mode != FRW_MODE_BISECT)
-        return set_error(FRW_E_INVALID, "unsupported walk mode");
-    hipStream_t s = (hipStream_t)stream;
     int rc = upload_jobs(b, counts, hops, s);
     if (rc != FRW_OK) return rc;
-    const WalkLaunch L = launch_of(b, p);
+    WalkLaunch L = launch_of(b, p);
+    MlpNet mnet{};
+    if (p->mode == FRW_MODE_LEARNED) {
+        mnet = mlp_net(net, alpha);
+        L.net = &mnet;
+        L.l_stats = (unsigned long long*)stats;
+    }
     if (!p->use_graph) return launch_walk_jobs(b->d_jobs, L, s);
-    if (!b->exec || !same_params(b->graph_params, *p)) {
+    if (!b->exec || !same_params(b->graph_params, *p) || b->graph_net != net || b->graph_alpha != alpha ||
+        b->graph_stats != stats) {
         drop_graph(b);
         int block = 0, grid = 0;
         size_t lds = 0;
@@ -196,14 +211,27 @@ int frw_batch_walk(frw_batch* b, const frw_batch_params* p, int64_t* counts, int
             return rc;
         }
         b->graph_params = *p;
+        b->graph_net = net;
+        b->graph_alpha = alpha;
+        b->graph_stats = stats;
     }
     return check_hip(hipGraphLaunch(b->exec, s), "graph launch");
 }
 
-int frw_batch_autotune(frw_batch* b, const frw_batch_params* base, frw_batch_params* best,
-                       double* best_hops_per_s)
+int check_learned_batch(const frw_batch* b, const frw_mlp* net, float alpha)
 {
-    if (!b || !base || !best) return set_error(FRW_E_INVALID, "null argument");
+    if (!net) return set_error(FRW_E_INVALID, "learned mode needs a network");
+    if (!(alpha > 0.0f)) return set_error(FRW_E_INVALID, "alpha must be positive");
+    for (const frw_batch_job& q : b->jobs)
+        if (!b->pyrs[q.tree].occ) return set_error(FRW_E_INVALID, "a job's layout has no occupancy pyramid");
+    return FRW_OK;
+}
+
+// Times each launch shape (block x waves per CU) with `run` and keeps the fastest.
+template <class Run>
+int autotune(frw_batch* b, const frw_batch_params* base, frw_batch_params* best, double* best_hops_per_s,
+             Run run)
+{
     int64_t *d_counts = nullptr, *d_hops = nullptr;
     int rc = check_hip(hipMalloc(&d_counts, sizeof(int64_t) * b->counts_size), "hipMalloc counts");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hops, sizeof(int64_t) * b->jobs.size()), "hipMalloc hops");
@@ -228,7 +256,7 @@ int frw_batch_autotune(frw_batch* b, const frw_batch_params* base, frw_batch_par
             hipMemsetAsync(d_counts, 0, sizeof(int64_t) * b->counts_size, nullptr);
             hipMemsetAsync(d_hops, 0, sizeof(int64_t) * b->jobs.size(), nullptr);
             hipEventRecord(e0, nullptr);
-            rc = frw_batch_walk(b, &cand, d_counts, d_hops, nullptr);
+            rc = run(&cand, d_counts, d_hops);
             hipEventRecord(e1, nullptr);
             if (rc != FRW_OK) break;
             if ((rc = check_hip(hipDeviceSynchronize(), "autotune run")) != FRW_OK) break;
@@ -255,6 +283,67 @@ int frw_batch_autotune(frw_batch* b, const frw_batch_params* base, frw_batch_par
     best->use_graph = base->use_graph;
     if (best_hops_per_s) *best_hops_per_s = best_rate;
     return FRW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int frw_batch_walk(frw_batch* b, const frw_batch_params* p, int64_t* counts, int64_t* hops,
+                   void* stream)
+{
+    if (!b || !p || !counts || !hops) return set_error(FRW_E_INVALID, "null argument");
+    if (p->mode != FRW_MODE_EXACT && p->mode != FRW_MODE_BISECT)
+        return set_error(FRW_E_INVALID, "unsupported walk mode");
+    return batch_walk(b, p, nullptr, 0.0f, nullptr, counts, hops, (hipStream_t)stream);
+}
+
+int frw_batch_set_proposal(frw_batch* b, const frw_mlp* const* per_tree, int32_t n_trees)
+{
+    if (!b || !per_tree || n_trees != (int32_t)b->trees.size())
+        return set_error(FRW_E_INVALID, "one network (pyramid source) per tree is needed");
+    for (int32_t i = 0; i < n_trees; ++i)
+        if (!per_tree[i] || mlp_tree(per_tree[i]) != b->trees[i])
+            return set_error(FRW_E_INVALID, "network " + std::to_string(i) + " was built for another tree");
+    for (int32_t i = 0; i < n_trees; ++i) b->pyrs[i] = mlp_pyramid(per_tree[i]);
+    b->up_counts = b->up_hops = nullptr;  // re-upload the job table
+    drop_graph(b);
+    return FRW_OK;
+}
+
+int frw_batch_walk_learned(frw_batch* b, const frw_batch_params* p, const frw_mlp* net, float alpha,
+                           int64_t* counts, int64_t* hops, uint64_t* stats_dev, void* stream)
+{
+    if (!b || !p || !counts || !hops) return set_error(FRW_E_INVALID, "null argument");
+    int rc = check_learned_batch(b, net, alpha);
+    if (rc != FRW_OK) return rc;
+    frw_batch_params q = *p;
+    q.mode = FRW_MODE_LEARNED;
+    return batch_walk(b, &q, net, alpha, stats_dev, counts, hops, (hipStream_t)stream);
+}
+
+int frw_batch_autotune_learned(frw_batch* b, const frw_batch_params* base, const frw_mlp* net, float alpha,
+                               frw_batch_params* best, double* best_hops_per_s)
+{
+    if (!b || !base || !best) return set_error(FRW_E_INVALID, "null argument");
+    int rc = check_learned_batch(b, net, alpha);
+    if (rc != FRW_OK) return rc;
+    frw_batch_params q = *base;
+    q.mode = FRW_MODE_LEARNED;
+    return autotune(b, &q, best, best_hops_per_s, [&](const frw_batch_params* c, int64_t* cn, int64_t* hp) {
+        return batch_walk(b, c, net, alpha, nullptr, cn, hp, nullptr);
+    });
+}
+
+int frw_batch_autotune(frw_batch* b, const frw_batch_params* base, frw_batch_params* best,
+                       double* best_hops_per_s)
+{
+    if (!b || !base || !best) return set_error(FRW_E_INVALID, "null argument");
+    if (base->mode != FRW_MODE_EXACT && base->mode != FRW_MODE_BISECT)
+        return set_error(FRW_E_INVALID, "unsupported walk mode");
+    return autotune(b, base, best, best_hops_per_s, [&](const frw_batch_params* c, int64_t* cn, int64_t* hp) {
+        return batch_walk(b, c, nullptr, 0.0f, nullptr, cn, hp, nullptr);
+    });
 }
 
 int frw_batch_check(const frw_batch* b)

@@ -26,6 +26,7 @@ struct WalkJob {
     uint64_t n_walks;
     uint32_t first_walk;           // Philox walk id of the job's first walk
     uint32_t job;                  // Philox counter word 1
+    MlpPyramid pyr;                // learned mode: the layout's occupancy pyramid
 };
 
 struct WalkLaunch {
@@ -49,7 +50,7 @@ struct WalkLaunch {
     unsigned long long* h_count = nullptr;
     uint64_t h_cap = 0;
     // learned mode
-    const MlpView* mlp = nullptr;
+    const MlpNet* net = nullptr;
     unsigned long long* l_stats = nullptr;
 };
 
@@ -60,7 +61,8 @@ struct WalkExtras {
     double* h_dstar = nullptr;
     unsigned long long* h_count = nullptr;
     uint64_t h_cap = 0;
-    const MlpView* mlp = nullptr;
+    const MlpNet* net = nullptr;
+    const MlpPyramid* pyr = nullptr;
     unsigned long long* l_stats = nullptr;
 };
 
@@ -125,5 +127,10 @@ int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds);
 int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s);
 int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o,
                 hipStream_t s, unsigned long long* traffic = nullptr, const WalkExtras* extra = nullptr);
+
+// learned proposal views (frw_mlp.hip)
+MlpPyramid mlp_pyramid(const frw_mlp* m);
+MlpNet mlp_net(const frw_mlp* m, float alpha);
+const frw_tree* mlp_tree(const frw_mlp* m);
 
 }  // namespace frw

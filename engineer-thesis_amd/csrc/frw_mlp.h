@@ -56,7 +56,21 @@ constexpr int kMlpParams = kOffB4 + 1;
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-struct MlpView {
+// Occupancy pyramid of one layout, passed by value (walk jobs carry one each).
+struct MlpPyramid {
+    const uint32_t* occ;      // bitmaps, level l at word occ_off[l]
+    const uint32_t* sat;      // summed-area tables, level l at entry sat_off[l]
+    const int64_t* occ_off;   // device arrays of Pmax + 1 / Vmax + 1 offsets
+    const int64_t* sat_off;
+    int4 bounds;
+    int K0;
+    int Lmax;
+    int Vmax;
+    int pad;
+};
+
+// Network weights (bf16 copies of the FP32 parameters) and the safety factor.
+struct MlpNet {
     const uint16_t* W1;   // bf16 [64][72]
     const uint16_t* W2;   // bf16 [64][64]
     const uint16_t* W3;   // bf16 [64][64]
@@ -64,16 +78,8 @@ struct MlpView {
     const float* b1;
     const float* b2;
     const float* b3;
-    float b4;
-    float alpha;          // safety factor on the proposal
-    const uint32_t* occ;  // occupancy bitmaps, level l at word occ_off[l]
-    int64_t occ_off[kPyrMaxLevel + 1];
-    const uint32_t* sat;  // summed-area tables, level l at entry sat_off[l]
-    int64_t sat_off[kPyrMaxLevel + 1];
-    int4 bounds;
-    int K0;
-    int Lmax;
-    int Vmax;
+    const float* b4;      // device scalar
+    float alpha;
 };
 
 __host__ __device__ inline uint16_t f2bf(float x)
@@ -95,7 +101,7 @@ __device__ inline float gelu_grad_f(float z)
     return cdf + z * pdf;
 }
 
-__device__ inline int mlp_level(const MlpView& m, double h, double* cell)
+__device__ inline int mlp_level(const MlpPyramid& m, double h, double* cell)
 {
     const int e = h >= 1.0 ? ilogb(h) : 0;
     int l = m.K0 - e + 1;
@@ -107,7 +113,7 @@ __device__ inline int mlp_level(const MlpView& m, double h, double* cell)
 // 64 occupancy bits of the 8x8 window (bit v*8+u) at level l around p.  Each
 // window row is one unaligned 8-bit field of the level's row-major bitmap,
 // read as two words (every level carries one padding word at its end).
-__device__ inline uint64_t mlp_window(const MlpView& m, int l, double cell, double px, double py, float* frac_x,
+__device__ inline uint64_t mlp_window(const MlpPyramid& m, int l, double cell, double px, double py, float* frac_x,
                                       float* frac_y)
 {
     const int64_t n = (int64_t)1 << l;
@@ -159,7 +165,7 @@ __device__ inline void mlp_feature_row(uint16_t* row, uint64_t bits, int level, 
 // Verification probe: true when the closed square (x0, y0, x1, y1) of
 // half-size d (> 0) lies in the bounds and its covering cell block at level
 // clamp(K0 - floor(log2 d) + 4, 0, Vmax) holds no occupied cell.
-__device__ inline bool mlp_certify(const MlpView& m, double x0, double y0, double x1, double y1, double d)
+__device__ inline bool mlp_certify(const MlpPyramid& m, double x0, double y0, double x1, double y1, double d)
 {
     if (!in_bounds_box(m.bounds, x0, y0, x1, y1)) return false;
     int v = m.K0 - ilogb(d) + kCertShift;
@@ -218,7 +224,7 @@ __device__ inline void mlp_layer(const uint16_t* in, uint16_t* out, const uint16
 // Forward pass of the wave's 64 feature rows in xs (LDS, 64 x kMlpStride);
 // hs is a second LDS buffer of the same size.  Returns row `lane`'s output.
 // Must be called by all 64 lanes together.
-__device__ inline float mlp_forward_wave(const MlpView& m, uint16_t* xs, uint16_t* hs, int lane)
+__device__ inline float mlp_forward_wave(const MlpNet& m, uint16_t* xs, uint16_t* hs, int lane)
 {
     wave_sync();
     mlp_layer<kMlpInPad>(xs, hs, m.W1, m.b1, lane);
@@ -230,7 +236,7 @@ __device__ inline float mlp_forward_wave(const MlpView& m, uint16_t* xs, uint16_
     float y = 0.0f;
     for (int j = 0; j < kMlpHid; ++j) y += bf2f(m.w4[j]) * bf2f(hs[lane * kMlpStride + j]);
     wave_sync();
-    return y + m.b4;
+    return y + *m.b4;
 }
 
 }  // namespace frw

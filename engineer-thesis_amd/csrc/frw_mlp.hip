@@ -19,6 +19,8 @@ struct frw_mlp {
     std::vector<int64_t> sat_off;  // summed-area-table entry offsets, levels 0..Vmax
     uint32_t* d_occ = nullptr;
     uint32_t* d_sat = nullptr;
+    int64_t* d_occ_off = nullptr;  // device copies of the offsets
+    int64_t* d_sat_off = nullptr;
     float* d_params = nullptr;     // FP32 master parameters
     uint16_t* d_bf = nullptr;      // bf16 copy of W1 | W2 | W3 | w4
     float* d_adam_m = nullptr;
@@ -26,7 +28,6 @@ struct frw_mlp {
     float* d_grad = nullptr;
     float* d_gs = nullptr;         // weight-gradient split partials, kMaxSplits x 64 x 72
     int64_t adam_step = 0;
-    float h_b4 = 0.0f;             // host copy of the output bias (kept in sync)
     // training set
     int64_t n_samples = 0;
     uint16_t* d_X = nullptr;       // [n][72] bf16
@@ -143,7 +144,7 @@ __global__ void sat_cols(int64_t n, uint32_t* __restrict__ S)
 }
 
 // Verification probe for host-supplied squares (test hook).
-__global__ void certify_kernel(MlpView m, const double* __restrict__ pts, const double* __restrict__ d,
+__global__ void certify_kernel(MlpPyramid m, const double* __restrict__ pts, const double* __restrict__ d,
                                int64_t n, uint8_t* __restrict__ out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -156,7 +157,7 @@ __global__ void certify_kernel(MlpView m, const double* __restrict__ pts, const 
 
 // ---------------------------------------------------------------- features
 // One lane per point: level, cell and the 72-wide bf16 feature row.
-__global__ void features_kernel(MlpView m, const double* __restrict__ pts, const double* __restrict__ hs,
+__global__ void features_kernel(MlpPyramid m, const double* __restrict__ pts, const double* __restrict__ hs,
                                 int64_t n, uint16_t* __restrict__ X, const double* __restrict__ dstar,
                                 float* __restrict__ target, double* __restrict__ cell_out)
 {
@@ -364,7 +365,7 @@ __global__ void gather_rows(const uint16_t* __restrict__ X, const float* __restr
 
 // Wave-level MFMA forward over rows of X (the walk kernel's inference path):
 // one wave per 64 rows.
-__global__ void __launch_bounds__(64) forward_waves(MlpView m, const uint16_t* __restrict__ X, int64_t n,
+__global__ void __launch_bounds__(64) forward_waves(MlpNet m, const uint16_t* __restrict__ X, int64_t n,
                                                     float* __restrict__ y)
 {
     __shared__ uint16_t xs[64 * kMlpStride];
@@ -381,9 +382,24 @@ inline int blocks(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 }  // namespace
 
-MlpView mlp_view(const frw_mlp* m, float alpha)
+MlpPyramid mlp_pyramid(const frw_mlp* m)
 {
-    MlpView v;
+    MlpPyramid v;
+    v.occ = m->d_occ;
+    v.sat = m->d_sat;
+    v.occ_off = m->d_occ_off;
+    v.sat_off = m->d_sat_off;
+    v.bounds = m->tree->bounds;
+    v.K0 = m->K0;
+    v.Lmax = m->Lmax;
+    v.Vmax = m->Vmax;
+    v.pad = 0;
+    return v;
+}
+
+MlpNet mlp_net(const frw_mlp* m, float alpha)
+{
+    MlpNet v;
     v.W1 = m->d_bf + kBfW1;
     v.W2 = m->d_bf + kBfW2;
     v.W3 = m->d_bf + kBfW3;
@@ -391,20 +407,12 @@ MlpView mlp_view(const frw_mlp* m, float alpha)
     v.b1 = m->d_params + kOffB1;
     v.b2 = m->d_params + kOffB2;
     v.b3 = m->d_params + kOffB3;
-    v.b4 = m->h_b4;
+    v.b4 = m->d_params + kOffB4;
     v.alpha = alpha;
-    v.occ = m->d_occ;
-    for (int l = 0; l <= kPyrMaxLevel; ++l) {
-        v.occ_off[l] = l <= m->Pmax ? m->occ_off[l] : 0;
-        v.sat_off[l] = l <= m->Vmax ? m->sat_off[l] : 0;
-    }
-    v.sat = m->d_sat;
-    v.bounds = m->tree->bounds;
-    v.K0 = m->K0;
-    v.Lmax = m->Lmax;
-    v.Vmax = m->Vmax;
     return v;
 }
+
+const frw_tree* mlp_tree(const frw_mlp* m) { return m->tree; }
 
 namespace {
 
@@ -504,23 +512,26 @@ using namespace frw;
 
 extern "C" {
 
-int frw_mlp_create(const frw_tree* t, uint64_t seed, frw_mlp** out)
+int frw_mlp_create(const frw_tree* t, uint64_t seed, int32_t max_level, frw_mlp** out)
 {
     if (!t || !out) return set_error(FRW_E_INVALID, "null argument");
+    if (max_level < 0 || max_level > kPyrMaxLevel) return set_error(FRW_E_RANGE, "max_level must be in 0..15");
+    const int cap = max_level ? max_level : kPyrMaxLevel;
     frw_mlp* m = new frw_mlp;
     m->tree = t;
     m->device = t->device;
     const int64_t W = (int64_t)t->bounds.z - t->bounds.x, H = (int64_t)t->bounds.w - t->bounds.y;
     while (((int64_t)1 << m->K0) <= std::max(W, H)) ++m->K0;
-    m->Lmax = std::min(m->K0, kMlpMaxLevel);
+    m->Lmax = std::min({m->K0, kMlpMaxLevel, cap});
     // verification levels: finest cell about 1/128 of the mean rect spacing
-    // sqrt(W H / n), at most level 15 and at least the window levels
+    // sqrt(W H / n), at most level `cap` (15 unless the caller limits memory:
+    // a level-l table takes (2^l + 1)^2 x 4 bytes)
     {
         const double spacing = std::sqrt((double)std::max<int64_t>(W, 1) * (double)std::max<int64_t>(H, 1) /
                                          (double)std::max<int64_t>(t->n_rects, 1));
         const double target = std::max(1.0, spacing / 128.0);
         const int v = m->K0 - (int)std::floor(std::log2(target));
-        m->Vmax = std::max(0, std::min({v, m->K0, kPyrMaxLevel}));
+        m->Vmax = std::max(0, std::min({v, m->K0, cap}));
     }
     m->Pmax = std::max(m->Lmax, m->Vmax);
     int64_t words = 0, entries = 0;
@@ -535,15 +546,18 @@ int frw_mlp_create(const frw_tree* t, uint64_t seed, frw_mlp** out)
     }
     int rc = check_hip(hipMalloc(&m->d_occ, words * 4), "hipMalloc occupancy");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_sat, entries * 4), "hipMalloc summed-area tables");
-    int64_t* d_off = nullptr;
     if (rc == FRW_OK) rc = check_hip(hipMemset(m->d_occ, 0, words * 4), "memset occupancy");
-    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_off, sizeof(int64_t) * m->occ_off.size()), "hipMalloc off");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_occ_off, sizeof(int64_t) * m->occ_off.size()), "hipMalloc off");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_sat_off, sizeof(int64_t) * m->sat_off.size()), "hipMalloc off");
     if (rc == FRW_OK)
-        rc = check_hip(hipMemcpy(d_off, m->occ_off.data(), sizeof(int64_t) * m->occ_off.size(),
+        rc = check_hip(hipMemcpy(m->d_occ_off, m->occ_off.data(), sizeof(int64_t) * m->occ_off.size(),
+                                 hipMemcpyHostToDevice), "copy off");
+    if (rc == FRW_OK)
+        rc = check_hip(hipMemcpy(m->d_sat_off, m->sat_off.data(), sizeof(int64_t) * m->sat_off.size(),
                                  hipMemcpyHostToDevice), "copy off");
     if (rc == FRW_OK) {
         occ_mark<<<blocks(t->n_rects, 256), 256>>>(t->d_rects, (int)t->n_rects, t->bounds, m->K0, m->Pmax,
-                                                   d_off, m->d_occ);
+                                                   m->d_occ_off, m->d_occ);
         rc = check_hip(hipGetLastError(), "occ_mark");
     }
     for (int l = 0; l <= m->Vmax && rc == FRW_OK; ++l) {
@@ -556,7 +570,6 @@ int frw_mlp_create(const frw_tree* t, uint64_t seed, frw_mlp** out)
         rc = check_hip(hipGetLastError(), "summed-area tables");
     }
     if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "occupancy pyramid");
-    if (d_off) hipFree(d_off);
     // Xavier-uniform initial weights from Philox, zero biases
     std::vector<float> P(kMlpParams, 0.0f);
     auto fill = [&](int off, int rows, int cols, int fan_in, int fan_out, uint32_t tag) {
@@ -598,7 +611,7 @@ int frw_mlp_create(const frw_tree* t, uint64_t seed, frw_mlp** out)
 void frw_mlp_free(frw_mlp* m)
 {
     if (!m) return;
-    for (void* p : {(void*)m->d_occ, (void*)m->d_sat, (void*)m->d_params, (void*)m->d_bf, (void*)m->d_adam_m,
+    for (void* p : {(void*)m->d_occ, (void*)m->d_sat, (void*)m->d_occ_off, (void*)m->d_sat_off, (void*)m->d_params, (void*)m->d_bf, (void*)m->d_adam_m,
                     (void*)m->d_adam_v, (void*)m->d_grad, (void*)m->d_gs, (void*)m->d_X, (void*)m->d_t, (void*)m->d_Z1,
                     (void*)m->d_Z2, (void*)m->d_Z3, (void*)m->d_F, (void*)m->d_A1, (void*)m->d_A2,
                     (void*)m->d_A3, (void*)m->d_D, (void*)m->d_Xb, (void*)m->d_tb, (void*)m->d_loss,
@@ -620,7 +633,6 @@ int frw_mlp_set_params(frw_mlp* m, const float* in, int32_t n)
     if (!m || !in || n != kMlpParams) return set_error(FRW_E_INVALID, "bad parameter buffer");
     int rc = check_hip(hipMemcpy(m->d_params, in, kMlpParams * 4, hipMemcpyHostToDevice), "copy params");
     if (rc != FRW_OK) return rc;
-    m->h_b4 = in[kOffB4];
     to_bf16<<<blocks(kMlpHid * kMlpInPad, 256), 256>>>(m->d_params, m->d_bf);
     return check_hip(hipDeviceSynchronize(), "to_bf16");
 }
@@ -650,7 +662,7 @@ int frw_mlp_certify(const frw_mlp* m, const double* pts, const double* d, int64_
     if (!m || n < 0 || (n > 0 && (!pts || !d || !out))) return set_error(FRW_E_INVALID, "bad arguments");
     if (n == 0) return FRW_OK;
     hipStream_t s = (hipStream_t)stream;
-    certify_kernel<<<blocks(n, 256), 256, 0, s>>>(mlp_view(m, 1.0f), pts, d, n, out);
+    certify_kernel<<<blocks(n, 256), 256, 0, s>>>(mlp_pyramid(m), pts, d, n, out);
     return check_hip(hipStreamSynchronize(s), "certify");
 }
 
@@ -663,8 +675,7 @@ int frw_mlp_features(const frw_mlp* m, const double* pts, const double* h, int64
     uint16_t* X = nullptr;
     int rc = check_hip(hipMalloc(&X, (size_t)n * kMlpInPad * 2), "hipMalloc features");
     if (rc != FRW_OK) return rc;
-    const MlpView v = mlp_view(m, 1.0f);
-    features_kernel<<<blocks(n, 256), 256, 0, s>>>(v, pts, h, n, X, nullptr, nullptr, nullptr);
+    features_kernel<<<blocks(n, 256), 256, 0, s>>>(mlp_pyramid(m), pts, h, n, X, nullptr, nullptr, nullptr);
     widen_bf16<<<blocks(n * kMlpInPad, 256), 256, 0, s>>>(X, feats, n * kMlpInPad);
     rc = check_hip(hipStreamSynchronize(s), "features");
     hipFree(X);
@@ -680,7 +691,7 @@ int frw_mlp_forward(const frw_mlp* m, const float* X, int64_t n, float* y, void*
     int rc = check_hip(hipMalloc(&Xb, (size_t)n * kMlpInPad * 2), "hipMalloc forward");
     if (rc != FRW_OK) return rc;
     narrow_bf16<<<blocks(n * kMlpInPad, 256), 256, 0, s>>>(X, Xb, n * kMlpInPad);
-    forward_waves<<<blocks(n, 64), 64, 0, s>>>(mlp_view(m, 1.0f), Xb, n, y);
+    forward_waves<<<blocks(n, 64), 64, 0, s>>>(mlp_net(m, 1.0f), Xb, n, y);
     rc = check_hip(hipStreamSynchronize(s), "mlp forward");
     hipFree(Xb);
     return rc;
@@ -705,6 +716,38 @@ int frw_mlp_gradient(frw_mlp* m, const float* X, const float* t, int64_t n, floa
     return rc;
 }
 
+int frw_mlp_add_samples(frw_mlp* m, const frw_mlp* features_of, const double* pts, const double* h,
+                        const double* dstar, int64_t n)
+{
+    if (!m || !pts || !h || !dstar || n <= 0) return set_error(FRW_E_INVALID, "bad arguments");
+    const frw_mlp* src = features_of ? features_of : m;
+    const int64_t old_n = m->n_samples, tot = old_n + n;
+    uint16_t* X = nullptr;
+    float* T = nullptr;
+    int rc = check_hip(hipMalloc(&X, (size_t)tot * kMlpInPad * 2), "hipMalloc samples");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&T, (size_t)tot * 4), "hipMalloc targets");
+    if (rc == FRW_OK && old_n) {
+        rc = check_hip(hipMemcpy(X, m->d_X, (size_t)old_n * kMlpInPad * 2, hipMemcpyDeviceToDevice), "copy samples");
+        if (rc == FRW_OK) rc = check_hip(hipMemcpy(T, m->d_t, (size_t)old_n * 4, hipMemcpyDeviceToDevice), "copy targets");
+    }
+    if (rc == FRW_OK) {
+        features_kernel<<<blocks(n, 256), 256>>>(mlp_pyramid(src), pts, h, n, X + old_n * kMlpInPad, dstar,
+                                                 T + old_n, nullptr);
+        rc = check_hip(hipDeviceSynchronize(), "features");
+    }
+    if (rc != FRW_OK) {
+        if (X) hipFree(X);
+        if (T) hipFree(T);
+        return rc;
+    }
+    hipFree(m->d_X);
+    hipFree(m->d_t);
+    m->d_X = X;
+    m->d_t = T;
+    m->n_samples = tot;
+    return FRW_OK;
+}
+
 int frw_mlp_set_samples(frw_mlp* m, const double* pts, const double* h, const double* dstar, int64_t n)
 {
     if (!m || !pts || !h || !dstar || n <= 0) return set_error(FRW_E_INVALID, "bad arguments");
@@ -713,13 +756,7 @@ int frw_mlp_set_samples(frw_mlp* m, const double* pts, const double* h, const do
     m->d_X = nullptr;
     m->d_t = nullptr;
     m->n_samples = 0;
-    int rc = check_hip(hipMalloc(&m->d_X, (size_t)n * kMlpInPad * 2), "hipMalloc samples");
-    if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_t, (size_t)n * 4), "hipMalloc targets");
-    if (rc != FRW_OK) return rc;
-    features_kernel<<<blocks(n, 256), 256>>>(mlp_view(m, 1.0f), pts, h, n, m->d_X, dstar, m->d_t, nullptr);
-    rc = check_hip(hipDeviceSynchronize(), "features");
-    if (rc == FRW_OK) m->n_samples = n;
-    return rc;
+    return frw_mlp_add_samples(m, nullptr, pts, h, dstar, n);
 }
 
 int frw_mlp_train(frw_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* last_loss)
@@ -768,7 +805,6 @@ int frw_mlp_train(frw_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t 
     }
     if (d_perm) hipFree(d_perm);
     if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "train");
-    if (rc == FRW_OK) rc = check_hip(hipMemcpy(&m->h_b4, m->d_params + kOffB4, 4, hipMemcpyDeviceToHost), "read b4");
     if (rc == FRW_OK && last_loss) *last_loss = loss_n ? loss_sum / loss_n : 0.f;
     return rc;
 }
@@ -835,10 +871,12 @@ int learned_sync(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p, 
     unsigned long long* d_st = nullptr;
     int rc = check_hip(hipMalloc(&d_st, 16), "hipMalloc");
     if (rc == FRW_OK) rc = check_hip(hipMemset(d_st, 0, 16), "memset");
-    const MlpView v = mlp_view(m, alpha);
+    const MlpNet net = mlp_net(m, alpha);
+    const MlpPyramid pyr = mlp_pyramid(m);
     if (rc == FRW_OK) {
         WalkExtras ex;
-        ex.mlp = &v;
+        ex.net = &net;
+        ex.pyr = &pyr;
         ex.l_stats = d_st;
         rc = launch_walk(t, &q, o, nullptr, d_traffic, &ex);
     }
@@ -866,9 +904,11 @@ int frw_walk_learned_async(const frw_tree* t, const frw_mlp* m, const frw_walk_p
     if (!o || !o->counts || !o->hops) return set_error(FRW_E_INVALID, "null outputs");
     frw_walk_params q = *p;
     q.mode = FRW_MODE_LEARNED;
-    const MlpView v = mlp_view(m, alpha);
+    const MlpNet net = mlp_net(m, alpha);
+    const MlpPyramid pyr = mlp_pyramid(m);
     WalkExtras ex;
-    ex.mlp = &v;
+    ex.net = &net;
+    ex.pyr = &pyr;
     ex.l_stats = (unsigned long long*)stats_dev;
     return launch_walk(t, &q, o, (hipStream_t)stream, nullptr, &ex);
 }

@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(64) walk_prologue(WalkJob* jobs, int n_jobs, u
 }
 
 template <int kMode, bool kStats, bool kHarvest>
-__global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpView mlp)
+__global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
 {
     extern __shared__ int32_t stack_mem[];
     __shared__ uint32_t sT[kNSample];
@@ -108,6 +108,8 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
     uint64_t w = 0;  // walk index within the job
     uint32_t hop = 0;
     DevTree t = jobs[0].tree;
+    MlpPyramid pyr;
+    if (kMode == kModeLearned) pyr = jobs[0].pyr;
     DBox sq{0, 0, 0, 0};
     uint32_t rnd[4] = {0, 0, 0, 0};
     unsigned long long my_hops = 0;  // hops accumulated for job j
@@ -135,6 +137,7 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
                         my_hops = 0;
                         j = nj;
                         t = jobs[j].tree;
+                        if (kMode == kModeLearned) pyr = jobs[j].pyr;
                     }
                     w = g - jobs[j].walk_begin;
                     sq = DBox{jobs[j].surface[0], jobs[j].surface[1], jobs[j].surface[2],
@@ -171,11 +174,11 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
             int level = 0;
             float fx = 0.0f, fy = 0.0f;
             if (active && !escaped) {
-                level = mlp_level(mlp, hprev, &cell);
-                bits = mlp_window(mlp, level, cell, px, py, &fx, &fy);
+                level = mlp_level(pyr, hprev, &cell);
+                bits = mlp_window(pyr, level, cell, px, py, &fx, &fy);
             }
             mlp_feature_row(mlp_xs + lane * kMlpStride, bits, level, fx, fy);
-            y_mlp = mlp_forward_wave(mlp, mlp_xs, mlp_hs, lane);
+            y_mlp = mlp_forward_wave(net, mlp_xs, mlp_hs, lane);
         }
         if (!active) continue;
 
@@ -185,10 +188,12 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
         } else {
             bool proposed = false;
             if (kMode == kModeLearned) {
-                const double d = cell * exp2((double)y_mlp) * (double)mlp.alpha;
+                // outputs beyond the training targets' range [-4, 4] are not trusted
+                const float y = fminf(fmaxf(y_mlp, -4.0f), 4.0f);
+                const double d = cell * exp2((double)y) * (double)net.alpha;
                 if (d > 2.0 && d < 1e300) {
                     const DBox c{px - d, py - d, px + d, py + d};
-                    if (mlp_certify(mlp, c.x0, c.y0, c.x1, c.y1, d)) {
+                    if (mlp_certify(pyr, c.x0, c.y0, c.x1, c.y1, d)) {
                         sq = c;
                         proposed = true;
                     }
@@ -268,7 +273,7 @@ __global__ void set_job(WalkJob* slot, WalkJob jb) { *slot = jb; }
 
 template <int kMode, bool kStats, bool kHarvest>
 void launch_kernel(int grid, int block, size_t lds, hipStream_t s, const WalkJob* jobs, const WalkArgs& a,
-                   const MlpView& m)
+                   const MlpNet& m)
 {
     walk_kernel<kMode, kStats, kHarvest><<<grid, block, lds, s>>>(jobs, a, m);
 }
@@ -336,11 +341,11 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
     a.h_count = L.h_count;
     a.h_cap = L.h_cap;
     a.l_stats = L.l_stats;
-    MlpView m{};
-    if (L.mlp) m = *L.mlp;
+    MlpNet m{};
+    if (L.net) m = *L.net;
     const bool st = L.traffic != nullptr;
     if (L.mode == kModeLearned) {
-        if (!L.mlp) return set_error(FRW_E_INVALID, "learned mode needs a network");
+        if (!L.net) return set_error(FRW_E_INVALID, "learned mode needs a network");
         if (st) launch_kernel<kModeLearned, true, false>(grid, block, lds, s, d_jobs, a, m);
         else launch_kernel<kModeLearned, false, false>(grid, block, lds, s, d_jobs, a, m);
     } else if (L.mode == kModeBisect) {
@@ -370,6 +375,7 @@ int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outp
     jb.first_walk = (uint32_t)p->first_walk;
     jb.job = p->job;
     jb.master = t->host_rects[p->master];
+    if (extra && extra->pyr) jb.pyr = *extra->pyr;
     set_job<<<1, 1, 0, s>>>(t->d_job, jb);
     int rc = check_hip(hipGetLastError(), "set_job");
     if (rc != FRW_OK) return rc;
@@ -393,7 +399,7 @@ int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outp
         L.h_dstar = extra->h_dstar;
         L.h_count = extra->h_count;
         L.h_cap = extra->h_cap;
-        L.mlp = extra->mlp;
+        L.net = extra->net;
         L.l_stats = extra->l_stats;
     }
     return launch_walk_jobs(t->d_job, L, s);

@@ -86,7 +86,8 @@ EXPORTS = [
     "frw_mlp_set_params", "frw_mlp_occupancy", "frw_mlp_features", "frw_mlp_forward",
     "frw_mlp_gradient", "frw_mlp_set_samples", "frw_mlp_train", "frw_walk_harvest",
     "frw_walk_learned", "frw_walk_learned_async", "frw_walk_learned_traffic",
-    "frw_mlp_levels", "frw_mlp_certify",
+    "frw_mlp_levels", "frw_mlp_certify", "frw_mlp_add_samples", "frw_batch_set_proposal",
+    "frw_batch_walk_learned", "frw_batch_autotune_learned",
 ]
 
 _lib = None
@@ -146,7 +147,12 @@ def lib():
     L.frw_batch_check.argtypes = [vp]
     L.frw_batch_free.argtypes = [vp]
     L.frw_batch_free.restype = None
-    L.frw_mlp_create.argtypes = [vp, u64, C.POINTER(vp)]
+    L.frw_mlp_create.argtypes = [vp, u64, i32, C.POINTER(vp)]
+    L.frw_mlp_add_samples.argtypes = [vp, vp, vp, vp, vp, i64]
+    L.frw_batch_set_proposal.argtypes = [vp, vp, i32]
+    L.frw_batch_walk_learned.argtypes = [vp, C.POINTER(BatchParams), vp, C.c_float, vp, vp, vp, vp]
+    L.frw_batch_autotune_learned.argtypes = [vp, C.POINTER(BatchParams), vp, C.c_float,
+                                             C.POINTER(BatchParams), C.POINTER(C.c_double)]
     L.frw_mlp_free.argtypes = [vp]
     L.frw_mlp_free.restype = None
     L.frw_mlp_get_params.argtypes = [vp, vp, i32]
@@ -365,6 +371,29 @@ class Batch:
         _check(lib().frw_batch_autotune(self.h, C.byref(base), C.byref(best), C.byref(rate)))
         return {"block": best.block, "grid": best.grid}, rate.value
 
+    def set_proposal(self, per_tree):
+        """per_tree: one Mlp per tree of the batch (their occupancy pyramids)."""
+        arr = (C.c_void_p * len(per_tree))(*[m.h.value for m in per_tree])
+        self._pyr = list(per_tree)
+        _check(lib().frw_batch_set_proposal(self.h, arr, len(per_tree)))
+
+    def walk_learned(self, net, counts, hops, stats=None, alpha=0.9, seed=1, block=0, grid=0,
+                     use_graph=False, max_hops=0, stream=None):
+        p = BatchParams(mode=MODE_LEARNED, block=block, grid=grid, use_graph=int(use_graph), seed=seed,
+                        max_hops=max_hops)
+        _check(lib().frw_batch_walk_learned(self.h, C.byref(p), net.h, alpha, counts.data_ptr(),
+                                            hops.data_ptr(), stats.data_ptr() if stats is not None else None,
+                                            stream))
+
+    def autotune_learned(self, net, alpha=0.9, seed=1, use_graph=False, max_hops=0):
+        base = BatchParams(mode=MODE_LEARNED, block=0, grid=0, use_graph=int(use_graph), seed=seed,
+                           max_hops=max_hops)
+        best = BatchParams()
+        rate = C.c_double()
+        _check(lib().frw_batch_autotune_learned(self.h, C.byref(base), net.h, alpha, C.byref(best),
+                                                C.byref(rate)))
+        return {"block": best.block, "grid": best.grid}, rate.value
+
     def check(self):
         _check(lib().frw_batch_check(self.h))
 
@@ -381,10 +410,10 @@ class Mlp:
     """Learned square-size proposal bound to one tree (occupancy pyramid +
     67-64-64-64-1 network)."""
 
-    def __init__(self, tree, seed=1):
+    def __init__(self, tree, seed=1, max_level=0):
         self.tree = tree
         h = C.c_void_p()
-        _check(lib().frw_mlp_create(tree.h, seed, C.byref(h)))
+        _check(lib().frw_mlp_create(tree.h, seed, max_level, C.byref(h)))
         self.h = h
         self.n_params = lib().frw_mlp_param_count()
 
@@ -433,6 +462,12 @@ class Mlp:
         _check(lib().frw_mlp_set_samples(self.h, pts.data_ptr(), h.data_ptr(), dstar.data_ptr(),
                                          pts.shape[0]))
 
+    def add_samples(self, pts, h, dstar, features_of=None):
+        """Appends samples with features from `features_of`'s pyramid (default: own)."""
+        src = features_of.h if features_of is not None else None
+        _check(lib().frw_mlp_add_samples(self.h, src, pts.data_ptr(), h.data_ptr(), dstar.data_ptr(),
+                                         pts.shape[0]))
+
     def train(self, epochs=2, batch=1024, lr=1e-3, seed=1):
         loss = C.c_float()
         _check(lib().frw_mlp_train(self.h, epochs, batch, lr, seed, C.byref(loss)))
@@ -459,10 +494,10 @@ def walk_harvest(tree, master, n_walks, pts, h, dstar, seed=1, first_walk=0, job
 
 
 def walk_learned(tree, mlp, master, n_walks, counts, hops, hit_out=None, hops_out=None, seed=1,
-                 first_walk=0, job=0, alpha=0.8, block=0, grid=0):
+                 first_walk=0, job=0, alpha=0.8, block=0, grid=0, max_hops=0):
     """Walks with the learned proposal; returns (accepted, fallback) counts."""
     p = WalkParams(master=master, mode=MODE_LEARNED, n_walks=n_walks, first_walk=first_walk,
-                   seed=seed, job=job, block=block, grid=grid, reserved=0, max_hops=0)
+                   seed=seed, job=job, block=block, grid=grid, reserved=0, max_hops=max_hops)
     o = WalkOutputs(counts.data_ptr(), hops.data_ptr(),
                     hit_out.data_ptr() if hit_out is not None else None,
                     hops_out.data_ptr() if hops_out is not None else None)

@@ -210,7 +210,7 @@ typedef struct {
 } frw_batch_job;
 
 typedef struct {
-    int32_t mode;        /* FRW_MODE_EXACT or FRW_MODE_BISECT */
+    int32_t mode;        /* FRW_MODE_EXACT or FRW_MODE_BISECT (learned: frw_batch_walk_learned) */
     int32_t block;       /* 0 = default */
     int32_t grid;        /* 0 = fill the device */
     int32_t use_graph;   /* capture the launch as a HIP graph once, then replay */
@@ -250,8 +250,10 @@ void frw_batch_free(frw_batch* b);
 typedef struct frw_mlp frw_mlp;
 
 /* Occupancy pyramid of the tree's layout and Philox-initialised weights.
- * The tree must outlive the network. */
-int frw_mlp_create(const frw_tree* t, uint64_t seed, frw_mlp** out);
+ * max_level (0 = 15) caps the finest pyramid level: a level-l summed-area
+ * table takes (2^l + 1)^2 x 4 bytes (4.3 GB at 15, 268 MB at 13).  The tree
+ * must outlive the network. */
+int frw_mlp_create(const frw_tree* t, uint64_t seed, int32_t max_level, frw_mlp** out);
 void frw_mlp_free(frw_mlp* m);
 /* Number of FP32 parameters: W1[64][72] b1 W2[64][64] b2 W3[64][64] b3 w4[64] b4. */
 int frw_mlp_param_count(void);
@@ -278,6 +280,10 @@ int frw_mlp_gradient(frw_mlp* m, const float* X, const float* t, int64_t n, floa
 /* Training set from device arrays (pts n x 2, h n, d* n): features and
  * targets log2(d* over cell) are computed on the device. */
 int frw_mlp_set_samples(frw_mlp* m, const double* pts, const double* h, const double* dstar, int64_t n);
+/* Appends samples whose features come from another network's pyramid
+ * (features_of; NULL = m's own): one network trained on several layouts. */
+int frw_mlp_add_samples(frw_mlp* m, const frw_mlp* features_of, const double* pts, const double* h,
+                        const double* dstar, int64_t n);
 /* Adam over the training set (batch a multiple of 32). */
 int frw_mlp_train(frw_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* last_loss);
 /* Runs exact walks and records, for up to `capacity` hops that did not end
@@ -299,6 +305,18 @@ int frw_walk_learned_async(const frw_tree* t, const frw_mlp* m, const frw_walk_p
  * out = {node records read, rect records read, hops, accepted, fallbacks}. */
 int frw_walk_learned_traffic(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p, float alpha,
                              int64_t out[5]);
+
+/* Learned walks of a batch.  frw_batch_set_proposal gives each tree of the
+ * batch its occupancy pyramid (per_tree[i] must have been created for tree
+ * i; only its pyramid is used).  One network `net` proposes for every job of
+ * a launch (its features are scale-free; a wave evaluates one weight set).
+ * stats_dev as in frw_walk_learned_async; graph capture and the autotuner
+ * work as for frw_batch_walk. */
+int frw_batch_set_proposal(frw_batch* b, const frw_mlp* const* per_tree, int32_t n_trees);
+int frw_batch_walk_learned(frw_batch* b, const frw_batch_params* p, const frw_mlp* net, float alpha,
+                           int64_t* counts, int64_t* hops, uint64_t* stats_dev, void* stream);
+int frw_batch_autotune_learned(frw_batch* b, const frw_batch_params* base, const frw_mlp* net, float alpha,
+                               frw_batch_params* best, double* best_hops_per_s);
 
 /* Measurement hook (no reference counterpart): the same walks as frw_walk
  * with per-hop traffic counters compiled in.  out = {node records read

@@ -123,6 +123,7 @@ int   orc_mlp_level(const orc_box* bounds, double h, double* cell);
 void  orc_mlp_features(const double* xyxy, int n, const orc_box* bounds, double px, double py,
                        double h, float out[67]);
 int   orc_mlp_vmax(const orc_box* bounds, int n_rects);
+int   orc_mlp_vmax_capped(const orc_box* bounds, int n_rects, int cap);
 int   orc_mlp_certify(const double* xyxy, int n, const orc_box* bounds, double px, double py, double d);
 int   orc_mlp_param_count(void);
 void  orc_mlp_forward(const float* params, const float* X, int B, float* y);

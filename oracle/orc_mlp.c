@@ -136,6 +136,11 @@ void orc_mlp_features(const double* xyxy, int n, const orc_box* bounds, double p
 
 int orc_mlp_vmax(const orc_box* bounds, int n_rects)
 {
+    return orc_mlp_vmax_capped(bounds, n_rects, 15);
+}
+
+int orc_mlp_vmax_capped(const orc_box* bounds, int n_rects, int cap)
+{
     const int K0 = k0_of(bounds);
     double W = bounds->x1 - bounds->x0, H = bounds->y1 - bounds->y0;
     if (W < 1) W = 1;
@@ -144,7 +149,7 @@ int orc_mlp_vmax(const orc_box* bounds, int n_rects)
     const double target = spacing / 128.0 > 1.0 ? spacing / 128.0 : 1.0;
     int v = K0 - (int)floor(log2(target));
     if (v > K0) v = K0;
-    if (v > 15) v = 15;
+    if (v > cap) v = cap;
     if (v < 0) v = 0;
     return v;
 }

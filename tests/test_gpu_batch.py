@@ -102,3 +102,74 @@ def test_batch_many_layouts():
         nr = trees[j].info().n_rects
         c1, h1 = _single(trees[j], 0, q["n_walks"], q["job"], 0)
         assert np.array_equal(counts[off:off + nr + 2], c1) and hops[j] == h1
+
+
+def _shared_params(n_params, seed=3):
+    rng = np.random.default_rng(seed)
+    P = rng.normal(0.0, 0.3, n_params).astype(np.float32)
+    P[: 64 * 72].reshape(64, 72)[:, 67:] = 0.0
+    return P
+
+
+def test_batch_learned_equals_single_learned():
+    """A batched learned launch (one network, each job on its own layout's
+    occupancy pyramid) gives, job by job, the tallies of the single-layout
+    learned walks with the same network and Philox streams; graph replays
+    repeat them.  The network is random, so walks are capped at 2000 hops
+    (a poor proposal makes walks long, never wrong)."""
+    torch = torch_dev()
+    trees, jobs = _fixture_batch()
+    batch = frw.Batch(trees, jobs)
+    pyrs = [frw.Mlp(t, seed=1) for t in trees]
+    P = _shared_params(pyrs[0].n_params)
+    for m in pyrs:
+        m.set_params(P)
+    net = pyrs[0]
+    batch.set_proposal(pyrs)
+    counts = torch.zeros(batch.counts_size(), dtype=torch.int64, device="cuda")
+    hops = torch.zeros(batch.n_jobs, dtype=torch.int64, device="cuda")
+    stats = torch.zeros(2, dtype=torch.int64, device="cuda")
+    batch.walk_learned(net, counts, hops, stats, alpha=0.9, seed=1, max_hops=2000)
+    torch.cuda.synchronize()
+    batch.check()
+    ref_c, ref_h, ref_s = counts.cpu().numpy(), hops.cpu().numpy(), stats.cpu().numpy()
+    assert ref_s[0] > 0 and ref_s[0] + ref_s[1] == ref_h.sum()
+    acc_sum = 0
+    for j, q in enumerate(jobs):
+        t = trees[q["tree"]]
+        nr = t.info().n_rects
+        c1 = torch.zeros(nr + 2, dtype=torch.int64, device="cuda")
+        h1 = torch.zeros(1, dtype=torch.int64, device="cuda")
+        acc, fb = frw.walk_learned(t, pyrs[q["tree"]], q["master"], q["n_walks"], c1, h1, seed=1,
+                                   first_walk=q["first_walk"], job=q["job"], alpha=0.9, max_hops=2000)
+        off = batch.counts_offset(j)
+        assert np.array_equal(ref_c[off:off + nr + 2], c1.cpu().numpy()), j
+        assert ref_h[j] == int(h1.item()), j
+        acc_sum += acc
+    assert acc_sum == ref_s[0]
+    for rep in range(3):
+        counts.zero_()
+        hops.zero_()
+        stats.zero_()
+        torch.cuda.synchronize()
+        batch.walk_learned(net, counts, hops, stats, alpha=0.9, seed=1, use_graph=True, max_hops=2000)
+        torch.cuda.synchronize()
+        assert np.array_equal(counts.cpu().numpy(), ref_c), rep
+        assert np.array_equal(stats.cpu().numpy(), ref_s), rep
+    print(f"\nbatched learned (random network): {ref_h.sum()} hops, accepted {ref_s[0]}, "
+          f"capped walks {sum(ref_c[batch.counts_offset(j) + trees[q['tree']].info().n_rects + 1] for j, q in enumerate(jobs))}")
+    best, rate = batch.autotune_learned(net, alpha=0.9, max_hops=500)
+    assert best["block"] in (64, 128, 192, 256) and best["grid"] > 0 and rate > 0
+
+
+def test_batch_learned_needs_pyramids():
+    torch = torch_dev()
+    trees, jobs = _fixture_batch()
+    batch = frw.Batch(trees, jobs)
+    net = frw.Mlp(trees[0], seed=1)
+    counts = torch.zeros(batch.counts_size(), dtype=torch.int64, device="cuda")
+    hops = torch.zeros(batch.n_jobs, dtype=torch.int64, device="cuda")
+    with pytest.raises(frw.FrwError):
+        batch.walk_learned(net, counts, hops)
+    with pytest.raises(frw.FrwError):  # a network built for another tree
+        batch.set_proposal([net] * len(trees))
