@@ -97,6 +97,8 @@ int     orc_q2(const orc_index* ix, double px, double py);
 double  orc_nearest(const orc_index* ix, double px, double py, int* argmin);
 /* Q3: drawBiggestSquareAtPoint, literal restatement; *n_q1 = Q1 calls. */
 orc_box orc_q3(const orc_index* ix, double px, double py, int* n_q1);
+void    orc_queries_many(const orc_index* ix, const double* pts, const double* boxes, int64_t n,
+                         int64_t n_q3, uint8_t* q1, int32_t* q2, double* q3, int nthreads);
 /* Starting surface of a walk from `master` (creatGaussianSurfFrom(r, 1.5)). */
 orc_box orc_gauss_from(const orc_index* ix, int master);
 

@@ -445,3 +445,31 @@ int64_t orc_walk_many(const orc_index* ix, int master, uint64_t seed, uint32_t j
     free(part);
     return total;
 }
+
+/* Many queries at once (OpenMP over queries), for parity checks at scale:
+ * q1[i] = Q1(boxes[i]), q2[i] = Q2(pts[i]) for i < n; q3[i] = Q3(pts[i]) for
+ * i < n_q3 (any output may be NULL). */
+void orc_queries_many(const orc_index* ix, const double* pts, const double* boxes, int64_t n, int64_t n_q3,
+                      uint8_t* q1, int32_t* q2, double* q3, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+    nthreads = 1;
+#endif
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+        if (q1 && boxes) {
+            const orc_box b = {boxes[4 * i], boxes[4 * i + 1], boxes[4 * i + 2], boxes[4 * i + 3]};
+            q1[i] = (uint8_t)orc_q1(ix, &b, NULL);
+        }
+        if (q2) q2[i] = orc_q2(ix, pts[2 * i], pts[2 * i + 1]);
+        if (q3 && i < n_q3) {
+            const orc_box s = orc_q3(ix, pts[2 * i], pts[2 * i + 1], NULL);
+            q3[4 * i] = s.x0;
+            q3[4 * i + 1] = s.y0;
+            q3[4 * i + 2] = s.x1;
+            q3[4 * i + 3] = s.y1;
+        }
+    }
+}

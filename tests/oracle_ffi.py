@@ -83,6 +83,8 @@ def oracle():
     L.orc_q3.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_int)]
     L.orc_gauss_from.restype = Box
     L.orc_gauss_from.argtypes = [C.c_void_p, C.c_int]
+    L.orc_queries_many.argtypes = [C.c_void_p, _D, _D, C.c_int64, C.c_int64, C.POINTER(C.c_uint8), _I32,
+                                   _D, C.c_int]
     L.orc_bf16.restype = C.c_float
     L.orc_bf16.argtypes = [C.c_float]
     L.orc_mlp_set_rounding.argtypes = [C.c_int]
@@ -206,6 +208,19 @@ class Index:
 
     def q2(self, px, py):
         return self.L.orc_q2(self.h, px, py)
+
+    def queries_many(self, pts, boxes, n_q3=0, nthreads=0):
+        """(q1[n], q2[n], q3[n_q3, 4]) for points and boxes, OpenMP over queries."""
+        P = np.ascontiguousarray(pts, dtype=np.float64)
+        B = np.ascontiguousarray(boxes, dtype=np.float64)
+        n = len(P)
+        q1 = np.zeros(n, dtype=np.uint8)
+        q2 = np.zeros(n, dtype=np.int32)
+        q3 = np.zeros((max(n_q3, 1), 4), dtype=np.float64)
+        self.L.orc_queries_many(self.h, _ptr(P, _D), _ptr(B, _D), n, n_q3,
+                                q1.ctypes.data_as(C.POINTER(C.c_uint8)), _ptr(q2, _I32), _ptr(q3, _D),
+                                nthreads)
+        return q1, q2, q3[:n_q3]
 
     def nearest(self, px, py):
         a = C.c_int()
