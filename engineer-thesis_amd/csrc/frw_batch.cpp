@@ -57,6 +57,7 @@ int upload_jobs(frw_batch* b, int64_t* counts, int64_t* hops, hipStream_t s)
         std::memset(&w, 0, sizeof(w));
         w.tree = dev_view(t);
         w.master = t->host_rects[q.master];
+        w.master_id = q.master;
         w.counts = (unsigned long long*)(counts + b->count_off[j]);
         w.hops = (unsigned long long*)(hops + j);
         w.walk_begin = begin;

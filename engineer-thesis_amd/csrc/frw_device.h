@@ -206,10 +206,20 @@ __host__ __device__ inline bool q1_box(const DevTree& t, const LaneStack& st, do
 // The minimum equals a brute-force scan bit for bit: the same per-rect
 // expression is evaluated, and a node is skipped only when its box distance
 // (a lower bound under monotone rounding) cannot beat the best.
+//
+// `hint` (a sorted rect index, or -1) seeds the bound with that rect's
+// distance, typically the rect nearest to the walker's previous point; *arg
+// (optional) receives the sorted index of a nearest rect.
 __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack& st, double px, double py,
-                                      Counters* cnt)
+                                      Counters* cnt, int hint = -1, int* arg = nullptr)
 {
     double best = __builtin_inf();
+    int besti = -1;
+    if (hint >= 0) {
+        best = linf(t.rects[hint], px, py);
+        besti = hint;
+        if (cnt) cnt->rects++;
+    }
     int sp = 0;
     st.at(sp++) = 0;
     while (sp > 0) {
@@ -219,7 +229,13 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
         if (!(linf(nd.box, px, py) < best)) continue;
         if (cnt) cnt->rects += (uint32_t)(nd.rend - nd.rbeg);
         if (cnt && nd.first_child >= 0) cnt->nodes += 4u;
-        for (int r = nd.rbeg; r < nd.rend; ++r) best = fmin(best, linf(t.rects[r], px, py));
+        for (int r = nd.rbeg; r < nd.rend; ++r) {
+            const double d = linf(t.rects[r], px, py);
+            if (d < best) {
+                best = d;
+                besti = r;
+            }
+        }
         if (nd.first_child >= 0) {
             double lb[4];
             int id[4];
@@ -242,6 +258,7 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
                 if (lb[c] < best) st.at(sp++) = id[c];
         }
     }
+    if (arg) *arg = besti;
     return best;
 }
 

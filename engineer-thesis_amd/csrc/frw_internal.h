@@ -26,6 +26,7 @@ struct WalkJob {
     uint64_t n_walks;
     uint32_t first_walk;           // Philox walk id of the job's first walk
     uint32_t job;                  // Philox counter word 1
+    int32_t master_id;             // original index of the master (its hits are counted per lane)
     MlpPyramid pyr;                // learned mode: the layout's occupancy pyramid
 };
 

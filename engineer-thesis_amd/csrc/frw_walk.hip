@@ -12,8 +12,9 @@
 //   panel = largest i with T[i] <= u          (Green CDF thresholds in LDS)
 //   p     = getPointFromNindex(square, panel) (FP64, reference op order)
 //   p outside the strict bounds            -> walk escaped (hop not counted)
-//   exact mode:   d* = nearest-rect distance; d* <= 0 -> hit (Q2 for the id),
-//                 else square = biggest square replayed from d*
+//   exact mode:   d* = nearest-rect distance (branch and bound seeded with
+//                 the rect nearest to the previous point); d* <= 0 -> hit
+//                 (Q2 for the id), else square = biggest square replayed from d*
 //   bisect mode:  square = literal bisection (one Q1 per step), hit = Q2(p)
 //   learned mode: the wave's 64 points go through the proposal network
 //                 (bf16 MFMA); an occupancy-pyramid probe that certifies the
@@ -50,6 +51,7 @@ struct WalkArgs {
 };
 
 constexpr int kLdsJobs = 128;                                   // job walk offsets cached in LDS
+constexpr int kChunk = 4;                                       // walk indices a lane takes per queue visit
 constexpr int kMlpLdsPerWave = 2 * 64 * kMlpStride * 2;         // two bf16 activation tiles
 
 __device__ inline int find_job(const WalkJob* jobs, const uint64_t* s_begin, int n_jobs, uint64_t g)
@@ -107,45 +109,74 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
     int j = 0;
     uint64_t w = 0;  // walk index within the job
     uint32_t hop = 0;
+    int hint = -1;  // sorted index of the rect nearest to the walker's previous point
     DevTree t = jobs[0].tree;
+    int mid = jobs[0].master_id;
     MlpPyramid pyr;
     if (kMode == kModeLearned) pyr = jobs[0].pyr;
     DBox sq{0, 0, 0, 0};
     uint32_t rnd[4] = {0, 0, 0, 0};
     unsigned long long my_hops = 0;  // hops accumulated for job j
+    // walks of job j that ended on its master / escaped: counted per lane and
+    // added to the tallies when the lane changes job or finishes (these two
+    // bins take most walks; one atomic per walk on them would serialise)
+    uint32_t my_master = 0, my_escaped = 0;
+    uint64_t nxt = 0, end = 0;  // walk indices this lane has taken from the queue
     Counters cnt{0u, 0u};
     Counters* cp = kStats ? &cnt : nullptr;
     unsigned long long my_nodes = 0, my_rects = 0, my_acc = 0, my_fb = 0;
 
+    auto flush = [&]() {
+        if (my_master) atomicAdd(&jobs[j].counts[mid], (unsigned long long)my_master);
+        if (my_escaped) atomicAdd(&jobs[j].counts[t.n_rects], (unsigned long long)my_escaped);
+        my_master = my_escaped = 0;
+    };
+
     for (;;) {
         const bool need = !active && !done;
-        const unsigned long long mask = __ballot(need);
+        uint64_t g = 0;
+        bool got = false;
+        if (need && nxt < end) {
+            g = nxt++;
+            got = true;
+        }
+        // lanes without a local index take kChunk more from the queue: one
+        // atomic per wave for all of them
+        const unsigned long long mask = __ballot(need && !got);
         if (mask) {
             const int leader = __ffsll((long long)mask) - 1;
             unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(a.work, (unsigned long long)__popcll(mask));
+            if (lane == leader) base = atomicAdd(a.work, (unsigned long long)__popcll(mask) * kChunk);
             base = __shfl(base, leader);
-            if (need) {
+            if (need && !got) {
                 const unsigned long long below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
-                const uint64_t g = base + (unsigned long long)__popcll(below);
-                if (g < a.total_walks) {
-                    active = true;
-                    hop = 0;
-                    const int nj = a.n_jobs == 1 ? 0 : find_job(jobs, s_begin, a.n_jobs, g);
-                    if (nj != j) {
-                        if (my_hops) atomicAdd(jobs[j].hops, my_hops);
-                        my_hops = 0;
-                        j = nj;
-                        t = jobs[j].tree;
-                        if (kMode == kModeLearned) pyr = jobs[j].pyr;
-                    }
-                    w = g - jobs[j].walk_begin;
-                    sq = DBox{jobs[j].surface[0], jobs[j].surface[1], jobs[j].surface[2],
-                              jobs[j].surface[3]};
+                const uint64_t g0 = base + (uint64_t)__popcll(below) * kChunk;
+                if (g0 < a.total_walks) {
+                    g = g0;
+                    got = true;
+                    nxt = g0 + 1;
+                    end = g0 + kChunk < a.total_walks ? g0 + kChunk : a.total_walks;
                 } else {
                     done = true;
                 }
             }
+        }
+        if (got) {
+            active = true;
+            hop = 0;
+            hint = -1;
+            const int nj = a.n_jobs == 1 ? 0 : find_job(jobs, s_begin, a.n_jobs, g);
+            if (nj != j) {
+                if (my_hops) atomicAdd(jobs[j].hops, my_hops);
+                my_hops = 0;
+                flush();
+                j = nj;
+                t = jobs[j].tree;
+                mid = jobs[j].master_id;
+                if (kMode == kModeLearned) pyr = jobs[j].pyr;
+            }
+            w = g - jobs[j].walk_begin;
+            sq = DBox{jobs[j].surface[0], jobs[j].surface[1], jobs[j].surface[2], jobs[j].surface[3]};
         }
         if (__ballot(active) == 0ull) break;
 
@@ -202,7 +233,9 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
             }
             if (!proposed) {
                 if (kMode != kModeBisect) {
-                    const double ds = nearest_rect(t, st, px, py, cp);
+                    int arg = -1;
+                    const double ds = nearest_rect(t, st, px, py, cp, hint, &arg);
+                    hint = arg;
                     if (ds <= 0.0) {
                         result = q2_point(t, st, px, py, cp);
                     } else {
@@ -232,8 +265,14 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
             if (result == -3 && hop >= a.max_hops) result = -2;
         }
         if (result != -3) {
-            const int bin = result >= 0 ? result : (result == -1 ? t.n_rects : t.n_rects + 1);
-            atomicAdd(&jobs[j].counts[bin], 1ull);
+            if (result == mid) {
+                my_master++;
+            } else if (result == -1) {
+                my_escaped++;
+            } else {
+                const int bin = result >= 0 ? result : t.n_rects + 1;
+                atomicAdd(&jobs[j].counts[bin], 1ull);
+            }
             my_hops += hop;
             if (jobs[j].hit_out) jobs[j].hit_out[w] = result;
             if (jobs[j].hops_out) jobs[j].hops_out[w] = (int32_t)hop;
@@ -241,10 +280,20 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
         }
     }
     if (a.n_jobs == 1) {
-        for (int off = 32; off >= 1; off >>= 1) my_hops += __shfl_xor(my_hops, off);
-        if (lane == 0 && my_hops) atomicAdd(jobs[0].hops, my_hops);
-    } else if (my_hops) {
-        atomicAdd(jobs[j].hops, my_hops);
+        unsigned long long m = my_master, e = my_escaped;
+        for (int off = 32; off >= 1; off >>= 1) {
+            my_hops += __shfl_xor(my_hops, off);
+            m += __shfl_xor(m, off);
+            e += __shfl_xor(e, off);
+        }
+        if (lane == 0) {
+            if (my_hops) atomicAdd(jobs[0].hops, my_hops);
+            if (m) atomicAdd(&jobs[0].counts[mid], m);
+            if (e) atomicAdd(&jobs[0].counts[t.n_rects], e);
+        }
+    } else {
+        if (my_hops) atomicAdd(jobs[j].hops, my_hops);
+        flush();
     }
     if (kStats) {
         for (int off = 32; off >= 1; off >>= 1) {
@@ -375,6 +424,7 @@ int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outp
     jb.first_walk = (uint32_t)p->first_walk;
     jb.job = p->job;
     jb.master = t->host_rects[p->master];
+    jb.master_id = p->master;
     if (extra && extra->pyr) jb.pyr = *extra->pyr;
     set_job<<<1, 1, 0, s>>>(t->d_job, jb);
     int rc = check_hip(hipGetLastError(), "set_job");

@@ -231,10 +231,23 @@ void hs_gauss(void* h, int master, double* out)
     out[0] = s.x0; out[1] = s.y0; out[2] = s.x1; out[3] = s.y1;
 }
 
+int64_t hs_walk_hinted(void* h, int master, int mode, uint64_t seed, uint32_t job, uint32_t first_walk,
+                       int64_t n_walks, uint32_t max_hops, const uint32_t* T, int32_t* hit_out,
+                       int32_t* hops_out, int64_t* traffic, int use_hint);
+
 // The walk_kernel loop for one walk at a time.
 int64_t hs_walk(void* h, int master, int mode, uint64_t seed, uint32_t job, uint32_t first_walk,
                 int64_t n_walks, uint32_t max_hops, const uint32_t* T, int32_t* hit_out,
                 int32_t* hops_out, int64_t* traffic)
+{
+    return hs_walk_hinted(h, master, mode, seed, job, first_walk, n_walks, max_hops, T, hit_out, hops_out,
+                          traffic, 0);
+}
+
+// use_hint: seed each hop's nearest-rect search with the previous hop's nearest rect.
+int64_t hs_walk_hinted(void* h, int master, int mode, uint64_t seed, uint32_t job, uint32_t first_walk,
+                       int64_t n_walks, uint32_t max_hops, const uint32_t* T, int32_t* hit_out,
+                       int32_t* hops_out, int64_t* traffic, int use_hint)
 {
     const HostTree* t = static_cast<HostTree*>(h);
     const DevTree v = view(*t);
@@ -250,6 +263,7 @@ int64_t hs_walk(void* h, int master, int mode, uint64_t seed, uint32_t job, uint
         uint32_t rnd[4] = {0, 0, 0, 0};
         uint32_t hop = 0;
         int result = -3;
+        int hint = -1;
         while (result == -3) {
             if ((hop & 3u) == 0u) {
                 rnd[0] = first_walk + (uint32_t)w;
@@ -266,7 +280,9 @@ int64_t hs_walk(void* h, int master, int mode, uint64_t seed, uint32_t job, uint
                 break;
             }
             if (mode == kModeExact) {
-                const double ds = nearest_rect(v, st, px, py, traffic ? &cnt : nullptr);
+                int arg = -1;
+                const double ds = nearest_rect(v, st, px, py, traffic ? &cnt : nullptr, use_hint ? hint : -1, &arg);
+                hint = arg;
                 if (ds <= 0.0)
                     result = q2_point(v, st, px, py, traffic ? &cnt : nullptr);
                 else
