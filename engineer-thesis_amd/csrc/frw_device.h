@@ -238,24 +238,17 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
         }
         if (nd.first_child >= 0) {
             double lb[4];
-            int id[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                lb[c] = linf(t.nodes[nd.first_child + c].box, px, py);
-                id[c] = nd.first_child + c;
-            }
-            // sort descending so the nearest child is pushed last (popped first)
+            for (int c = 0; c < 4; ++c) lb[c] = linf(t.nodes[nd.first_child + c].box, px, py);
+            // the nearest child is pushed last (popped first); the others in order
+            int near = 0;
 #pragma unroll
-            for (int i = 1; i < 4; ++i)
-#pragma unroll
-                for (int j = i; j > 0; --j)
-                    if (lb[j] > lb[j - 1]) {
-                        const double tl = lb[j]; lb[j] = lb[j - 1]; lb[j - 1] = tl;
-                        const int ti = id[j]; id[j] = id[j - 1]; id[j - 1] = ti;
-                    }
+            for (int c = 1; c < 4; ++c)
+                if (lb[c] < lb[near]) near = c;
 #pragma unroll
             for (int c = 0; c < 4; ++c)
-                if (lb[c] < best) st.at(sp++) = id[c];
+                if (c != near && lb[c] < best) st.at(sp++) = nd.first_child + c;
+            if (lb[near] < best) st.at(sp++) = nd.first_child + near;
         }
     }
     if (arg) *arg = besti;
@@ -314,9 +307,23 @@ __host__ __device__ inline DBox biggest_square(const DevTree& t, const LaneStack
                                       double dstar, uint32_t* n_q1, Counters* cnt, int* err)
 {
     const double MIN_DIST = (double)0.095f;
+    // h tracks the nominal half-size (1, then +-dist per step).  After n < 60
+    // steps every computed half-extent of the box is within (2n + 2) 2^(k-53)
+    // of h (one rounding per coordinate update, per h update and per
+    // difference; coordinates and h below 2^k), so a decision taken on h
+    // with the band 8 m = 2^(k-45) > m + that error is the decision of the
+    // four-extent test below; inside the band the four-extent test runs.
+    // (the thresholds' own rounding, below 2^(k-53), fits in the band's slack)
+    const double h_hit = dstar + 8.0 * t.margin, h_free = dstar - 8.0 * t.margin;
+    double h = 1.0;
+    int step = 0;
     auto collides = [&](double a, double c, double b, double e) -> bool {
         if (!in_bounds_box(t.bounds, a, c, b, e)) return true;
         if (!kLiteral) {
+            if (step < 60) {
+                if (h >= h_hit) return true;
+                if (h < h_free) return false;
+            }
             const double lo = fmin(fmin(px - a, b - px), fmin(py - c, e - py));
             const double hi = fmax(fmax(px - a, b - px), fmax(py - c, e - py));
             if (lo >= dstar + t.margin) return true;
@@ -328,10 +335,13 @@ __host__ __device__ inline DBox biggest_square(const DevTree& t, const LaneStack
     DBox o = {px - 1, py - 1, px + 1, py + 1};
     if (collides(o.x0, o.y0, o.x1, o.y1)) return o;
     const DBox init = {px - 2, py - 2, px + 2, py + 2};
+    h = 2.0;
     if (collides(init.x0, init.y0, init.x1, init.y1)) return init;
+    h = 1.0;
     double dist = t.dist0;
     bool max_reached = false;
     for (int it = 0;; ++it) {
+        step = it;
         const bool c = collides(o.x0, o.y0, o.x1, o.y1);
         if (!(c || !max_reached || dist > MIN_DIST)) break;
         if (it >= kMaxBisect) {  // cannot happen for a finite layout; reported
@@ -342,9 +352,11 @@ __host__ __device__ inline DBox biggest_square(const DevTree& t, const LaneStack
             if (dist > MIN_DIST) dist /= 2;
             max_reached = true;
             o.x0 += dist; o.y0 += dist; o.x1 -= dist; o.y1 -= dist;
+            h -= dist;
         } else {
             if (max_reached) dist /= 2;
             o.x0 -= dist; o.y0 -= dist; o.x1 += dist; o.y1 += dist;
+            h += dist;
         }
     }
     return o;
