@@ -123,6 +123,39 @@ __host__ __device__ inline double linf(const int4& r, double px, double py)
     return fmax(dx, dy);
 }
 
+// Saturating int32 subtraction (v_sub_i32 with clamp on the device).
+__host__ __device__ inline int sat_sub(int a, int b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_elementwise_sub_sat(a, b);
+#else
+    const long long r = (long long)a - (long long)b;
+    return r > 2147483647LL ? 2147483647 : (r < -2147483648LL ? (int)-2147483648LL : (int)r);
+#endif
+}
+
+// Integer lower bound of linf(box, p) from fx = floor(px), cx = ceil(px)
+// (and fy, cy), saturated to int32: x0 - cx <= x0 - px and fx - x1 <= px - x1,
+// so the bound never exceeds the exact distance; a dummy (inverted) box gets
+// a huge bound.
+__host__ __device__ inline int linf_lb(const int4& b, int fx, int cx, int fy, int cy)
+{
+    const int dx = max(sat_sub(b.x, cx), sat_sub(fx, b.z));
+    const int dy = max(sat_sub(b.y, cy), sat_sub(fy, b.w));
+    return max(dx, dy);
+}
+
+// Smallest int32 >= d (saturated), for comparisons of integer bounds with d.
+__host__ __device__ inline int ceil_int(double d)
+{
+    return d >= 2147483647.0 ? 2147483647 : (d <= -2147483648.0 ? (int)-2147483648LL : (int)ceil(d));
+}
+
+__host__ __device__ inline int floor_int(double d)
+{
+    return d >= 2147483647.0 ? 2147483647 : (d <= -2147483648.0 ? (int)-2147483648LL : (int)floor(d));
+}
+
 // Rect equality by value (the reference compares RectHost by value when it
 // ignores the master, randomWalk/CPU/quadTree.cpp:219-226).
 __host__ __device__ inline bool same_rect(const int4& a, const int4& b)
@@ -204,8 +237,8 @@ __host__ __device__ inline bool q1_box(const DevTree& t, const LaneStack& st, do
 // Nearest rect in the L-infinity metric: min over rects of linf(rect, p),
 // by branch and bound over the node boxes (children visited nearest first).
 // The minimum equals a brute-force scan bit for bit: the same per-rect
-// expression is evaluated, and a node is skipped only when its box distance
-// (a lower bound under monotone rounding) cannot beat the best.
+// expression is evaluated, and a node is skipped only when an integer lower
+// bound of its box distance cannot beat the best.
 //
 // `hint` (a sorted rect index, or -1) seeds the bound with that rect's
 // distance, typically the rect nearest to the walker's previous point; *arg
@@ -213,10 +246,15 @@ __host__ __device__ inline bool q1_box(const DevTree& t, const LaneStack& st, do
 __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack& st, double px, double py,
                                       Counters* cnt, int hint = -1, int* arg = nullptr)
 {
+    // node pruning and ordering use integer lower bounds (an integer lb is
+    // below best exactly when it is below ceil(best)); rect distances are exact
+    const int fx = floor_int(px), cx = ceil_int(px), fy = floor_int(py), cy = ceil_int(py);
     double best = __builtin_inf();
+    int best_c = 2147483647;
     int besti = -1;
     if (hint >= 0) {
         best = linf(t.rects[hint], px, py);
+        best_c = ceil_int(best);
         besti = hint;
         if (cnt) cnt->rects++;
     }
@@ -226,20 +264,21 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
         const int ni = st.at(--sp);
         const Node nd = t.nodes[ni];
         if (cnt) cnt->nodes++;
-        if (!(linf(nd.box, px, py) < best)) continue;
+        if (!(linf_lb(nd.box, fx, cx, fy, cy) < best_c)) continue;
         if (cnt) cnt->rects += (uint32_t)(nd.rend - nd.rbeg);
         if (cnt && nd.first_child >= 0) cnt->nodes += 4u;
         for (int r = nd.rbeg; r < nd.rend; ++r) {
             const double d = linf(t.rects[r], px, py);
             if (d < best) {
                 best = d;
+                best_c = ceil_int(d);
                 besti = r;
             }
         }
         if (nd.first_child >= 0) {
-            double lb[4];
+            int lb[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) lb[c] = linf(t.nodes[nd.first_child + c].box, px, py);
+            for (int c = 0; c < 4; ++c) lb[c] = linf_lb(t.nodes[nd.first_child + c].box, fx, cx, fy, cy);
             // the nearest child is pushed last (popped first); the others in order
             int near = 0;
 #pragma unroll
@@ -247,8 +286,8 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
                 if (lb[c] < lb[near]) near = c;
 #pragma unroll
             for (int c = 0; c < 4; ++c)
-                if (c != near && lb[c] < best) st.at(sp++) = nd.first_child + c;
-            if (lb[near] < best) st.at(sp++) = nd.first_child + near;
+                if (c != near && lb[c] < best_c) st.at(sp++) = nd.first_child + c;
+            if (lb[near] < best_c) st.at(sp++) = nd.first_child + near;
         }
     }
     if (arg) *arg = besti;

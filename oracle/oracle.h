@@ -84,6 +84,7 @@ typedef struct orc_index orc_index;
 orc_index* orc_index_new(const double* xyxy, int n, const orc_box* bounds, int kind);
 void       orc_index_free(orc_index* ix);
 int        orc_index_kind(const orc_index* ix);
+orc_box    orc_index_bounds(const orc_index* ix);
 /* Tree shape summary: nodes, max level, stored object count (with duplicates). */
 void       orc_index_stats(const orc_index* ix, int64_t out[3]);
 
@@ -130,6 +131,20 @@ int   orc_mlp_certify(const double* xyxy, int n, const orc_box* bounds, double p
 int   orc_mlp_param_count(void);
 void  orc_mlp_forward(const float* params, const float* X, int B, float* y);
 float orc_mlp_backward(const float* params, const float* X, const float* t, int B, float* grad);
+
+/* ---- weighted (physical) estimator (orc_weight.c) ------------------------ */
+double  orc_square_density(double s, double u, double v);
+void    orc_green_grad(double gx[ORC_NSAMPLE], double gy[ORC_NSAMPLE]);
+void    orc_panel_centre(double x0, double y0, double a, int k, double* px, double* py);
+void    orc_gauss_point(const orc_box* G, uint64_t seed, uint32_t walk, uint32_t job, double* rx, double* ry,
+                        double* nx, double* ny, double* L);
+int     orc_walk_weighted_one(const orc_index* ix, const orc_box* G, const uint32_t T[ORC_NSAMPLE],
+                              const double gx[ORC_NSAMPLE], const double gy[ORC_NSAMPLE],
+                              const double pw[ORC_NSAMPLE], uint64_t seed, uint32_t job, uint32_t walk,
+                              int64_t max_hops, double eps, int64_t* hops, double* weight);
+int64_t orc_walk_weighted_many(const orc_index* ix, int master, uint64_t seed, uint32_t job,
+                               int64_t first_walk, int64_t n_walks, int64_t max_hops, double eps,
+                               int32_t* hit_out, int32_t* hops_out, double* w_out);
 
 #ifdef __cplusplus
 }

@@ -250,6 +250,8 @@ void orc_index_free(orc_index* ix)
 
 int orc_index_kind(const orc_index* ix) { return ix->kind; }
 
+orc_box orc_index_bounds(const orc_index* ix) { return ix->bounds; }
+
 void orc_index_stats(const orc_index* ix, int64_t out[3])
 {
     int64_t maxlvl = 0, objs = 0;

@@ -85,6 +85,12 @@ def oracle():
     L.orc_gauss_from.argtypes = [C.c_void_p, C.c_int]
     L.orc_queries_many.argtypes = [C.c_void_p, _D, _D, C.c_int64, C.c_int64, C.POINTER(C.c_uint8), _I32,
                                    _D, C.c_int]
+    L.orc_square_density.restype = C.c_double
+    L.orc_square_density.argtypes = [C.c_double, C.c_double, C.c_double]
+    L.orc_green_grad.argtypes = [_D, _D]
+    L.orc_walk_weighted_many.restype = C.c_int64
+    L.orc_walk_weighted_many.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_uint32, C.c_int64, C.c_int64,
+                                         C.c_int64, C.c_double, _I32, _I32, _D]
     L.orc_bf16.restype = C.c_float
     L.orc_bf16.argtypes = [C.c_float]
     L.orc_mlp_set_rounding.argtypes = [C.c_int]
@@ -221,6 +227,15 @@ class Index:
                                 q1.ctypes.data_as(C.POINTER(C.c_uint8)), _ptr(q2, _I32), _ptr(q3, _D),
                                 nthreads)
         return q1, q2, q3[:n_q3]
+
+    def walk_weighted(self, master, n_walks, seed=1, job=0, first_walk=0, max_hops=10**6, eps=1.0):
+        """(hit[n], hops[n], weight[n]) of weighted walks (Gaussian-surface start)."""
+        hit = np.zeros(n_walks, dtype=np.int32)
+        hops = np.zeros(n_walks, dtype=np.int32)
+        w = np.zeros(n_walks, dtype=np.float64)
+        self.L.orc_walk_weighted_many(self.h, master, seed, job, first_walk, n_walks, max_hops, eps,
+                                      _ptr(hit, _I32), _ptr(hops, _I32), _ptr(w, _D))
+        return hit, hops, w
 
     def nearest(self, px, py):
         a = C.c_int()
@@ -378,3 +393,16 @@ def mlp_backward(params, X, t, rounding=True):
     with _Rounding(rounding):
         loss = oracle().orc_mlp_backward(_ptr(P, _F32), _ptr(Xc, _F32), _ptr(tc, _F32), len(Xc), _ptr(g, _F32))
     return float(loss), g
+
+
+# ---------------------------------------------------------------- weighted estimator
+
+def square_density(s, u, v):
+    return oracle().orc_square_density(s, u, v)
+
+
+def green_grad():
+    gx = np.zeros(NSAMPLE, dtype=np.float64)
+    gy = np.zeros(NSAMPLE, dtype=np.float64)
+    oracle().orc_green_grad(_ptr(gx, _D), _ptr(gy, _D))
+    return gx, gy
