@@ -150,6 +150,7 @@ void frw_tree_free(frw_tree* t)
     if (t->d_ids) hipFree(t->d_ids);
     if (t->d_nodes) hipFree(t->d_nodes);
     if (t->d_thresholds) hipFree(t->d_thresholds);
+    if (t->d_wtab) hipFree(t->d_wtab);
     if (t->d_work) hipFree(t->d_work);
     if (t->d_err) hipFree(t->d_err);
     if (t->d_surface) hipFree(t->d_surface);
@@ -189,6 +190,10 @@ int frw_tree_build(const frw_layout* l, const frw_tree_params* p, void* stream, 
         green_thresholds(T);
         rc = check_hip(hipMalloc(&t->d_thresholds, sizeof(T)), "hipMalloc thresholds");
         if (rc == FRW_OK) rc = check_hip(hipMemcpy(t->d_thresholds, T, sizeof(T), hipMemcpyHostToDevice), "copy thresholds");
+        double wt[3 * kNSample];
+        green_grad_tables(wt, wt + kNSample, wt + 2 * kNSample);
+        if (rc == FRW_OK) rc = check_hip(hipMalloc(&t->d_wtab, sizeof(wt)), "hipMalloc weight tables");
+        if (rc == FRW_OK) rc = check_hip(hipMemcpy(t->d_wtab, wt, sizeof(wt), hipMemcpyHostToDevice), "copy weight tables");
         if (rc == FRW_OK) rc = check_hip(hipMalloc(&t->d_work, sizeof(unsigned long long)), "hipMalloc work");
         if (rc == FRW_OK) rc = check_hip(hipMalloc(&t->d_err, sizeof(int)), "hipMalloc err");
         if (rc == FRW_OK) rc = check_hip(hipMemset(t->d_err, 0, sizeof(int)), "memset err");
@@ -303,6 +308,61 @@ int frw_walk_async(const frw_tree* t, const frw_walk_params* p, const frw_walk_o
     if (!o || !o->counts || !o->hops) return set_error(FRW_E_INVALID, "walk outputs need counts and hops");
     FRW_GUARD_BEGIN
     return launch_walk(t, p, o, (hipStream_t)stream);
+    FRW_GUARD_END
+}
+
+int frw_walk_weighted_async(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o, double eps,
+                            double* wsum, double* wsq, double* w_out, void* stream)
+{
+    if (!t || !p) return set_error(FRW_E_INVALID, "null argument");
+    frw_walk_params q = *p;
+    q.mode = FRW_MODE_EXACT;
+    int rc = check_walk_params(t, &q);
+    if (rc != FRW_OK) return rc;
+    if (!o || !o->counts || !o->hops || !wsum || !wsq)
+        return set_error(FRW_E_INVALID, "weighted walks need counts, hops, wsum and wsq");
+    if (!(eps > 0.0) || !std::isfinite(eps)) return set_error(FRW_E_INVALID, "eps must be positive");
+    FRW_GUARD_BEGIN
+    q.mode = FRW_MODE_WEIGHTED;
+    WalkExtras ex;
+    ex.wsum = wsum;
+    ex.wsq = wsq;
+    ex.w_out = w_out;
+    ex.eps = eps;
+    return launch_walk(t, &q, o, (hipStream_t)stream, nullptr, &ex);
+    FRW_GUARD_END
+}
+
+int frw_walk_weighted(const frw_tree* t, const frw_walk_params* p, double eps, int64_t* counts_host,
+                      double* wsum_host, double* wsq_host, int64_t* hops)
+{
+    if (!counts_host || !wsum_host || !wsq_host) return set_error(FRW_E_INVALID, "null output");
+    FRW_GUARD_BEGIN
+    const int64_t nb = t ? t->n_rects + 2 : 0;
+    int64_t *d_counts = nullptr, *d_hops = nullptr;
+    double *d_ws = nullptr, *d_wq = nullptr;
+    int rc = t ? FRW_OK : set_error(FRW_E_INVALID, "null tree");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_counts, sizeof(int64_t) * nb), "hipMalloc counts");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hops, sizeof(int64_t)), "hipMalloc hops");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_ws, sizeof(double) * nb), "hipMalloc wsum");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_wq, sizeof(double) * nb), "hipMalloc wsq");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_counts, 0, sizeof(int64_t) * nb), "memset");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_hops, 0, sizeof(int64_t)), "memset");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_ws, 0, sizeof(double) * nb), "memset");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_wq, 0, sizeof(double) * nb), "memset");
+    if (rc == FRW_OK) {
+        frw_walk_outputs o{d_counts, d_hops, nullptr, nullptr};
+        rc = frw_walk_weighted_async(t, p, &o, eps, d_ws, d_wq, nullptr, nullptr);
+    }
+    if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "weighted walk");
+    if (rc == FRW_OK) rc = check_err_flag(t, nullptr);
+    if (rc == FRW_OK) rc = check_hip(hipMemcpy(counts_host, d_counts, sizeof(int64_t) * nb, hipMemcpyDeviceToHost), "copy");
+    if (rc == FRW_OK) rc = check_hip(hipMemcpy(wsum_host, d_ws, sizeof(double) * nb, hipMemcpyDeviceToHost), "copy");
+    if (rc == FRW_OK) rc = check_hip(hipMemcpy(wsq_host, d_wq, sizeof(double) * nb, hipMemcpyDeviceToHost), "copy");
+    if (rc == FRW_OK && hops) rc = check_hip(hipMemcpy(hops, d_hops, sizeof(int64_t), hipMemcpyDeviceToHost), "copy");
+    for (void* q : {(void*)d_counts, (void*)d_hops, (void*)d_ws, (void*)d_wq})
+        if (q) hipFree(q);
+    return rc;
     FRW_GUARD_END
 }
 

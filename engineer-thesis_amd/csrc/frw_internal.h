@@ -27,6 +27,9 @@ struct WalkJob {
     uint32_t first_walk;           // Philox walk id of the job's first walk
     uint32_t job;                  // Philox counter word 1
     int32_t master_id;             // original index of the master (its hits are counted per lane)
+    double* wsum;                  // weighted mode: per-bin sums of weights and of squared weights
+    double* wsq;
+    double* w_out;                 // weighted mode: optional per-walk weights
     MlpPyramid pyr;                // learned mode: the layout's occupancy pyramid
 };
 
@@ -53,6 +56,9 @@ struct WalkLaunch {
     // learned mode
     const MlpNet* net = nullptr;
     unsigned long long* l_stats = nullptr;
+    // weighted mode
+    const double* wtab = nullptr;
+    double eps = 1.0;
 };
 
 // Optional extras of a single-layout walk launch.
@@ -65,6 +71,10 @@ struct WalkExtras {
     const MlpNet* net = nullptr;
     const MlpPyramid* pyr = nullptr;
     unsigned long long* l_stats = nullptr;
+    double* wsum = nullptr;  // weighted mode outputs
+    double* wsq = nullptr;
+    double* w_out = nullptr;
+    double eps = 1.0;
 };
 
 }  // namespace frw
@@ -90,6 +100,7 @@ struct frw_tree {
     int32_t* d_ids = nullptr;
     frw::Node* d_nodes = nullptr;
     uint32_t* d_thresholds = nullptr;  // Green CDF panel thresholds
+    double* d_wtab = nullptr;          // weighted estimator: gx | gy | pw, 3 x 200
     // walk scratch owned by the tree handle
     unsigned long long* d_work = nullptr;  // walk work-queue counter
     int* d_err = nullptr;                  // kernel error bits
@@ -105,6 +116,7 @@ int check_hip(hipError_t e, const char* what);
 
 // host tables
 void green_thresholds(uint32_t T[kNSample]);
+void green_grad_tables(double gx[kNSample], double gy[kNSample], double pw[kNSample]);
 
 // device build (frw_build.hip)
 int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 bounds, int leaf_cap,

@@ -151,6 +151,54 @@ void green_thresholds(uint32_t T[kNSample])
     for (int i = 0; i < kNSample; ++i) T[i] = (uint32_t)std::nearbyint((intg[i] / tot) * 4294967296.0);
 }
 
+// First-hop weight tables of the weighted estimator: the gradient of each
+// panel's exit probability with respect to the source, for the unit square
+// with the source at its centre (y down, panels clockwise from the top-left
+// corner), and each panel's draw probability under the thresholds T.
+// Exit density of a side for a source at u along it and v from it:
+//   G(s; u, v) = sum_n 2 sin(n pi s) sin(n pi u) sinh(n pi (1 - v)) / sinh(n pi),
+// whose centre derivatives are
+//   dG/du = sum_{n even} n pi cos(n pi / 2) sin(n pi s) / cosh(n pi / 2),
+//   dG/dv = -sum_{n odd} n pi sin(n pi / 2) sin(n pi s) / sinh(n pi / 2);
+// (u, v) per side: top (x, y), right (y, 1 - x), bottom (1 - x, 1 - y),
+// left (1 - y, x).  Panel values are midpoint densities / (50 T), T the
+// normalisation of the panel probabilities.
+void green_grad_tables(double gx[kNSample], double gy[kNSample], double pw[kNSample])
+{
+    const double pi = 3.14159265358979323846;
+    const int per = kNSample / 4;
+    double raw = 0.0;
+    for (int j = 0; j < per; ++j) {
+        const double s = (j + 0.5) / per;
+        double k = 0.0;
+        for (int n = 1; n <= 19; n += 2) k += (((n & 3) == 1) ? 1.0 : -1.0) * std::sin(n * pi * s) / std::cosh(n * pi / 2.0);
+        raw += k / per;
+    }
+    const double T = 4.0 * raw;
+    for (int k = 0; k < kNSample; ++k) {
+        const double s = ((k % per) + 0.5) / per;
+        double a = 0.0, b = 0.0;
+        for (int n = 2; n <= 40; n += 2)
+            a += n * pi * ((n % 4 == 0) ? 1.0 : -1.0) * std::sin(n * pi * s) / std::cosh(n * pi / 2.0);
+        for (int n = 1; n <= 39; n += 2)
+            b -= n * pi * (((n & 3) == 1) ? 1.0 : -1.0) * std::sin(n * pi * s) / std::sinh(n * pi / 2.0);
+        a /= per * T;
+        b /= per * T;
+        switch (k / per) {
+        case 0: gx[k] = a;  gy[k] = b;  break;
+        case 1: gx[k] = -b; gy[k] = a;  break;
+        case 2: gx[k] = -a; gy[k] = -b; break;
+        default: gx[k] = b; gy[k] = -a; break;
+        }
+    }
+    uint32_t th[kNSample];
+    green_thresholds(th);
+    for (int k = 0; k < kNSample; ++k) {
+        const double hi = k + 1 < kNSample ? (double)th[k + 1] : 4294967296.0;
+        pw[k] = (hi - (double)th[k]) / 4294967296.0;
+    }
+}
+
 // ---------------------------------------------------------------- generator
 namespace {
 

@@ -31,7 +31,8 @@ static_assert(sizeof(Node) == 32, "node must stay 32 bytes");
 enum WalkMode : int32_t {
     kModeExact = 0,    // nearest-rect distance + replayed bisection schedule
     kModeBisect = 1,   // reference bisection, one Q1 per step
-    kModeLearned = 2,  // MLP proposal verified by one Q1, exact fallback
+    kModeLearned = 2,  // network proposal verified by a pyramid probe, exact fallback
+    kModeWeighted = 3, // exact walks from a Gaussian-surface start with first-hop weights
 };
 
 }  // namespace frw

@@ -48,10 +48,23 @@ struct WalkArgs {
     uint64_t h_cap;
     // learned proposal
     unsigned long long* l_stats;  // accepted, fallback
+    // weighted estimator: first-hop tables gx[200] | gy[200] | pw[200], permittivity
+    const double* wtab;
+    double eps;
 };
 
 constexpr int kLdsJobs = 128;                                   // job walk offsets cached in LDS
 constexpr int kChunk = 4;                                       // walk indices a lane takes per queue visit
+constexpr uint32_t kGausDomain = 0x47415553u;                   // 'GAUS': Gaussian-surface start points
+
+// Compensated (Kahan) accumulation; -ffp-contract=off keeps it exact.
+__device__ inline void kahan_add(double& s, double& c, double x)
+{
+    const double y = x - c;
+    const double t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
 constexpr int kMlpLdsPerWave = 2 * 64 * kMlpStride * 2;         // two bf16 activation tiles
 
 __device__ inline int find_job(const WalkJob* jobs, const uint64_t* s_begin, int n_jobs, uint64_t g)
@@ -122,6 +135,12 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
     // bins take most walks; one atomic per walk on them would serialise)
     uint32_t my_master = 0, my_escaped = 0;
     uint64_t nxt = 0, end = 0;  // walk indices this lane has taken from the queue
+    // weighted estimator: the walk's weight, its preset first point, and
+    // per-lane compensated sums (and sums of squares) for the master and
+    // escape bins
+    double wgt = 0.0, fpx = 0.0, fpy = 0.0;
+    bool first_pt = false, start_stuck = false;
+    double wm_s = 0.0, wm_c = 0.0, wm_q = 0.0, we_s = 0.0, we_c = 0.0, we_q = 0.0;
     Counters cnt{0u, 0u};
     Counters* cp = kStats ? &cnt : nullptr;
     unsigned long long my_nodes = 0, my_rects = 0, my_acc = 0, my_fb = 0;
@@ -130,6 +149,17 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
         if (my_master) atomicAdd(&jobs[j].counts[mid], (unsigned long long)my_master);
         if (my_escaped) atomicAdd(&jobs[j].counts[t.n_rects], (unsigned long long)my_escaped);
         my_master = my_escaped = 0;
+        if (kMode == kModeWeighted) {
+            if (wm_s != 0.0 || wm_q != 0.0) {
+                atomicAdd(&jobs[j].wsum[mid], wm_s - wm_c);
+                atomicAdd(&jobs[j].wsq[mid], wm_q);
+            }
+            if (we_s != 0.0 || we_q != 0.0) {
+                atomicAdd(&jobs[j].wsum[t.n_rects], we_s - we_c);
+                atomicAdd(&jobs[j].wsq[t.n_rects], we_q);
+            }
+            wm_s = wm_c = wm_q = we_s = we_c = we_q = 0.0;
+        }
     };
 
     for (;;) {
@@ -177,21 +207,74 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
             }
             w = g - jobs[j].walk_begin;
             sq = DBox{jobs[j].surface[0], jobs[j].surface[1], jobs[j].surface[2], jobs[j].surface[3]};
+            if (kMode == kModeWeighted) {
+                // start: a uniform point r on the Gaussian surface G (outward
+                // normal n, perimeter L), the largest empty square at r inside
+                // the bounds (half-side h), a Green-CDF panel of it (its centre
+                // is the first point) and the weight -eps L (n . g_k) / (p_k 2h)
+                uint32_t c[4] = {jobs[j].first_walk + (uint32_t)w, jobs[j].job, 0u, kGausDomain};
+                philox4x32_10(c, k0, k1);
+                const double gw = sq.x1 - sq.x0, gh = sq.y1 - sq.y0;
+                const double L = 2.0 * (gw + gh);
+                const double tp = (double)c[0] * (1.0 / 4294967296.0) * L;
+                double rx, ry, nx, ny;
+                if (tp < gw) {
+                    rx = sq.x0 + tp; ry = sq.y0; nx = 0.0; ny = -1.0;
+                } else if (tp < gw + gh) {
+                    rx = sq.x1; ry = sq.y0 + (tp - gw); nx = 1.0; ny = 0.0;
+                } else if (tp < 2.0 * gw + gh) {
+                    rx = sq.x1 - (tp - gw - gh); ry = sq.y1; nx = 0.0; ny = 1.0;
+                } else {
+                    rx = sq.x0; ry = sq.y1 - (tp - 2.0 * gw - gh); nx = -1.0; ny = 0.0;
+                }
+                double hh = nearest_rect(t, st, rx, ry, cp);
+                hh = fmin(hh, fmin(fmin(rx - (double)t.bounds.x, (double)t.bounds.z - rx),
+                                   fmin(ry - (double)t.bounds.y, (double)t.bounds.w - ry)));
+                wgt = 0.0;
+                start_stuck = !(hh > 0.0);
+                first_pt = false;
+                if (!start_stuck) {
+                    rnd[0] = jobs[j].first_walk + (uint32_t)w;
+                    rnd[1] = jobs[j].job;
+                    rnd[2] = 0u;
+                    rnd[3] = kWalkDomain;
+                    philox4x32_10(rnd, k0, k1);
+                    const int k = sample_panel(sT, rnd[0]);
+                    const double side = 2.0 * hh;
+                    const double tk = ((double)(k % (kNSample / 4)) + 0.5) / (kNSample / 4) * side;
+                    const double x0 = rx - hh, y0 = ry - hh;
+                    switch (k / (kNSample / 4)) {
+                    case 0: fpx = x0 + tk; fpy = y0; break;
+                    case 1: fpx = x0 + side; fpy = y0 + tk; break;
+                    case 2: fpx = x0 + side - tk; fpy = y0 + side; break;
+                    default: fpx = x0; fpy = y0 + side - tk; break;
+                    }
+                    wgt = -a.eps * L * (nx * a.wtab[k] + ny * a.wtab[kNSample + k]) /
+                          (a.wtab[2 * kNSample + k] * 2.0 * hh);
+                    first_pt = true;
+                }
+            }
         }
         if (__ballot(active) == 0ull) break;
 
         double px = 0.0, py = 0.0;
         bool escaped = false;
         if (active) {
-            if ((hop & 3u) == 0u) {
-                rnd[0] = jobs[j].first_walk + (uint32_t)w;
-                rnd[1] = jobs[j].job;
-                rnd[2] = hop >> 2;
-                rnd[3] = kWalkDomain;
-                philox4x32_10(rnd, k0, k1);
+            if (kMode == kModeWeighted && first_pt) {
+                px = fpx;
+                py = fpy;
+                first_pt = false;
+            } else {
+                if ((hop & 3u) == 0u) {
+                    rnd[0] = jobs[j].first_walk + (uint32_t)w;
+                    rnd[1] = jobs[j].job;
+                    rnd[2] = hop >> 2;
+                    rnd[3] = kWalkDomain;
+                    philox4x32_10(rnd, k0, k1);
+                }
+                const int panel = sample_panel(sT, rnd[hop & 3u]);
+                point_from_panel(sq.x0, sq.y0, sq.x1, sq.y1, panel, px, py);
             }
-            const int panel = sample_panel(sT, rnd[hop & 3u]);
-            point_from_panel(sq.x0, sq.y0, sq.x1, sq.y1, panel, px, py);
             escaped = !in_bounds_strict(t.bounds, px, py);
         }
         // half-size of the square the point was sampled on
@@ -214,7 +297,10 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
         if (!active) continue;
 
         int result = -3;  // walk continues
-        if (escaped) {
+        if (kMode == kModeWeighted && start_stuck) {
+            result = -2;  // no empty square at the start point
+            start_stuck = false;
+        } else if (escaped) {
             result = -1;
         } else {
             bool proposed = false;
@@ -273,6 +359,20 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
                 const int bin = result >= 0 ? result : t.n_rects + 1;
                 atomicAdd(&jobs[j].counts[bin], 1ull);
             }
+            if (kMode == kModeWeighted) {
+                if (jobs[j].w_out) jobs[j].w_out[w] = wgt;
+                if (result == mid) {
+                    kahan_add(wm_s, wm_c, wgt);
+                    wm_q += wgt * wgt;
+                } else if (result == -1) {
+                    kahan_add(we_s, we_c, wgt);
+                    we_q += wgt * wgt;
+                } else {
+                    const int bin = result >= 0 ? result : t.n_rects + 1;
+                    atomicAdd(&jobs[j].wsum[bin], wgt);
+                    atomicAdd(&jobs[j].wsq[bin], wgt * wgt);
+                }
+            }
             my_hops += hop;
             if (jobs[j].hit_out) jobs[j].hit_out[w] = result;
             if (jobs[j].hops_out) jobs[j].hops_out[w] = (int32_t)hop;
@@ -290,6 +390,10 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
             if (my_hops) atomicAdd(jobs[0].hops, my_hops);
             if (m) atomicAdd(&jobs[0].counts[mid], m);
             if (e) atomicAdd(&jobs[0].counts[t.n_rects], e);
+        }
+        if (kMode == kModeWeighted) {
+            my_master = my_escaped = 0;  // counted above
+            flush();
         }
     } else {
         if (my_hops) atomicAdd(jobs[j].hops, my_hops);
@@ -390,6 +494,8 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
     a.h_count = L.h_count;
     a.h_cap = L.h_cap;
     a.l_stats = L.l_stats;
+    a.wtab = L.wtab;
+    a.eps = L.eps;
     MlpNet m{};
     if (L.net) m = *L.net;
     const bool st = L.traffic != nullptr;
@@ -400,6 +506,10 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
     } else if (L.mode == kModeBisect) {
         if (st) launch_kernel<kModeBisect, true, false>(grid, block, lds, s, d_jobs, a, m);
         else launch_kernel<kModeBisect, false, false>(grid, block, lds, s, d_jobs, a, m);
+    } else if (L.mode == kModeWeighted) {
+        if (!L.wtab) return set_error(FRW_E_INVALID, "weighted mode needs its tables");
+        if (st) launch_kernel<kModeWeighted, true, false>(grid, block, lds, s, d_jobs, a, m);
+        else launch_kernel<kModeWeighted, false, false>(grid, block, lds, s, d_jobs, a, m);
     } else if (L.h_count) {
         launch_kernel<kModeExact, false, true>(grid, block, lds, s, d_jobs, a, m);
     } else {
@@ -425,6 +535,11 @@ int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outp
     jb.job = p->job;
     jb.master = t->host_rects[p->master];
     jb.master_id = p->master;
+    if (extra) {
+        jb.wsum = extra->wsum;
+        jb.wsq = extra->wsq;
+        jb.w_out = extra->w_out;
+    }
     if (extra && extra->pyr) jb.pyr = *extra->pyr;
     set_job<<<1, 1, 0, s>>>(t->d_job, jb);
     int rc = check_hip(hipGetLastError(), "set_job");
@@ -450,6 +565,8 @@ int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outp
         L.h_count = extra->h_count;
         L.h_cap = extra->h_cap;
         L.net = extra->net;
+        L.eps = extra->eps;
+        L.wtab = extra->wsum ? t->d_wtab : nullptr;
         L.l_stats = extra->l_stats;
     }
     return launch_walk_jobs(t->d_job, L, s);

@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfrw.so")
 
-MODE_EXACT, MODE_BISECT, MODE_LEARNED = 0, 1, 2
+MODE_EXACT, MODE_BISECT, MODE_LEARNED, MODE_WEIGHTED = 0, 1, 2, 3
 GEN_UNIFORM, GEN_CLUSTERED = 0, 1
 NSAMPLE = 200
 
@@ -87,7 +87,7 @@ EXPORTS = [
     "frw_mlp_gradient", "frw_mlp_set_samples", "frw_mlp_train", "frw_walk_harvest",
     "frw_walk_learned", "frw_walk_learned_async", "frw_walk_learned_traffic",
     "frw_mlp_levels", "frw_mlp_certify", "frw_mlp_add_samples", "frw_batch_set_proposal",
-    "frw_batch_walk_learned", "frw_batch_autotune_learned",
+    "frw_batch_walk_learned", "frw_batch_autotune_learned", "frw_walk_weighted_async", "frw_walk_weighted",
 ]
 
 _lib = None
@@ -136,6 +136,9 @@ def lib():
     L.frw_walk_async.argtypes = [vp, C.POINTER(WalkParams), C.POINTER(WalkOutputs), vp]
     L.frw_walk.argtypes = [vp, C.POINTER(WalkParams), vp, C.POINTER(WalkSummary)]
     L.frw_walk_traffic.argtypes = [vp, C.POINTER(WalkParams), vp]
+    L.frw_walk_weighted_async.argtypes = [vp, C.POINTER(WalkParams), C.POINTER(WalkOutputs), C.c_double,
+                                          vp, vp, vp, vp]
+    L.frw_walk_weighted.argtypes = [vp, C.POINTER(WalkParams), C.c_double, vp, vp, vp, vp]
     L.frw_write_results.argtypes = [vp, vp, i64, C.c_char_p]
     L.frw_batch_create.argtypes = [vp, i32, vp, i32, C.POINTER(vp)]
     L.frw_batch_counts_size.restype = i64
@@ -322,6 +325,33 @@ class Tree:
                         hit_out.data_ptr() if hit_out is not None else None,
                         hops_out.data_ptr() if hops_out is not None else None)
         _check(lib().frw_walk_async(self.h, C.byref(p), C.byref(o), stream))
+
+    def walk_weighted_async(self, master, n_walks, counts, hops, wsum, wsq, w_out=None, hit_out=None,
+                            hops_out=None, seed=1, eps=1.0, job=0, first_walk=0, max_hops=0, block=0,
+                            grid=0, stream=None):
+        """Weighted (physical) estimator into device tensors: wsum[j] / n_walks
+        estimates the capacitance C_master,j (eps per unit length)."""
+        p = WalkParams(master=master, mode=MODE_WEIGHTED, n_walks=n_walks, first_walk=first_walk, seed=seed,
+                       job=job, block=block, grid=grid, reserved=0, max_hops=max_hops)
+        o = WalkOutputs(counts.data_ptr(), hops.data_ptr(),
+                        hit_out.data_ptr() if hit_out is not None else None,
+                        hops_out.data_ptr() if hops_out is not None else None)
+        _check(lib().frw_walk_weighted_async(self.h, C.byref(p), C.byref(o), eps, wsum.data_ptr(),
+                                             wsq.data_ptr(), w_out.data_ptr() if w_out is not None else None,
+                                             stream))
+
+    def walk_weighted(self, master, n_walks, seed=1, eps=1.0, job=0, first_walk=0, max_hops=0):
+        """Synchronous weighted walks: (counts, wsum, wsq, hops) on the host."""
+        p = WalkParams(master=master, mode=MODE_WEIGHTED, n_walks=n_walks, first_walk=first_walk, seed=seed,
+                       job=job, block=0, grid=0, reserved=0, max_hops=max_hops)
+        nb = self.info().n_rects + 2
+        counts = np.zeros(nb, dtype=np.int64)
+        wsum = np.zeros(nb, dtype=np.float64)
+        wsq = np.zeros(nb, dtype=np.float64)
+        hops = C.c_int64()
+        _check(lib().frw_walk_weighted(self.h, C.byref(p), eps, counts.ctypes.data, wsum.ctypes.data,
+                                       wsq.ctypes.data, C.byref(hops)))
+        return counts, wsum, wsq, hops.value
 
     def __del__(self):
         try:

@@ -41,7 +41,7 @@ enum {
     FRW_E_KERNEL = -8   /* a kernel raised an internal error flag */
 };
 
-enum { FRW_MODE_EXACT = 0, FRW_MODE_BISECT = 1, FRW_MODE_LEARNED = 2 };
+enum { FRW_MODE_EXACT = 0, FRW_MODE_BISECT = 1, FRW_MODE_LEARNED = 2, FRW_MODE_WEIGHTED = 3 };
 enum { FRW_GEN_UNIFORM = 0, FRW_GEN_CLUSTERED = 1 };
 enum { FRW_NSAMPLE = 200 };
 
@@ -317,6 +317,26 @@ int frw_batch_walk_learned(frw_batch* b, const frw_batch_params* p, const frw_ml
                            int64_t* counts, int64_t* hops, uint64_t* stats_dev, void* stream);
 int frw_batch_autotune_learned(frw_batch* b, const frw_batch_params* base, const frw_mlp* net, float alpha,
                                frw_batch_params* best, double* best_hops_per_s);
+
+/* Weighted (physical) estimator of the capacitance row of the master (no
+ * reference counterpart: the reference computes Green's-function gradients
+ * but never uses them, randomWalk/CPU/getAvgPathLen.cpp:83-84).  Each walk
+ * starts at a uniform point r on the master's Gaussian surface G (perimeter L,
+ * outward normal n), takes the largest empty square at r (half-side h), a
+ * Green-CDF panel k of it (the panel centre is the first point) and the
+ * weight w = -eps L (n . g_k) / (p_k 2h) (g_k: gradient of the panel
+ * probability for the unit square, p_k: its draw probability), then walks as
+ * FRW_MODE_EXACT.  wsum[j] / n_walks estimates C_mj (per unit length; the
+ * escape bin is the grounded boundary), wsq[j] the second moment.
+ * counts/hops as for frw_walk_async; wsum, wsq: device double[n_rects + 2],
+ * accumulated (caller zeroes); w_out: optional device double[n_walks].
+ * Sums are compensated per lane and added across lanes with FP64 atomics
+ * (their order, hence the last bits, varies run to run). */
+int frw_walk_weighted_async(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o, double eps,
+                            double* wsum, double* wsq, double* w_out, void* stream);
+/* Synchronous form into host arrays (counts int64[n+2], wsum/wsq double[n+2]). */
+int frw_walk_weighted(const frw_tree* t, const frw_walk_params* p, double eps, int64_t* counts_host,
+                      double* wsum_host, double* wsq_host, int64_t* hops);
 
 /* Measurement hook (no reference counterpart): the same walks as frw_walk
  * with per-hop traffic counters compiled in.  out = {node records read

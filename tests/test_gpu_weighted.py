@@ -1,0 +1,114 @@
+"""Weighted (physical) FRW estimator on the GPU against the CPU oracle.
+
+* walk for walk: the same start point, first square, panel and weight, then
+  the same exact walk (hit rect and hop count) as the oracle's restatement;
+* the per-conductor weight sums equal the oracle's within FP64 summation
+  order;
+* physics: a parallel-plate pair couples with about -length/gap (plus
+  fringe), the self term is positive, and all weights (escapes included)
+  average to zero within the Monte Carlo error, at 10^6 rects too."""
+import numpy as np
+import pytest
+
+import frw
+import oracle_ffi as O
+from conftest import FIXTURES
+from gpu_common import load_fixture, oracle_index, torch_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_weighted(tree, master, n, seed=1, first=0, job=0, max_hops=0, eps=1.0):
+    torch = torch_dev()
+    nr = tree.info().n_rects
+    counts = torch.zeros(nr + 2, dtype=torch.int64, device="cuda")
+    hops = torch.zeros(1, dtype=torch.int64, device="cuda")
+    wsum = torch.zeros(nr + 2, dtype=torch.float64, device="cuda")
+    wsq = torch.zeros(nr + 2, dtype=torch.float64, device="cuda")
+    w_out = torch.zeros(n, dtype=torch.float64, device="cuda")
+    hit = torch.full((n,), -7, dtype=torch.int32, device="cuda")
+    hp = torch.full((n,), -7, dtype=torch.int32, device="cuda")
+    tree.walk_weighted_async(master, n, counts, hops, wsum, wsq, w_out=w_out, hit_out=hit, hops_out=hp,
+                             seed=seed, eps=eps, first_walk=first, job=job, max_hops=max_hops)
+    torch.cuda.synchronize()
+    return (counts.cpu().numpy(), int(hops.item()), wsum.cpu().numpy(), wsq.cpu().numpy(),
+            w_out.cpu().numpy(), hit.cpu().numpy(), hp.cpu().numpy())
+
+
+@pytest.mark.parametrize("name,master", FIXTURES[:4])
+def test_weighted_walks_replay_oracle(name, master):
+    lay, rects, bounds = load_fixture(name)
+    tree = frw.Tree(lay)
+    n = 3000
+    counts, hops, wsum, wsq, w, hit, hp = gpu_weighted(tree, master, n, seed=5)
+    ix = oracle_index(rects, bounds, O.REFTREE_DFS)
+    ohit, ohp, ow = ix.walk_weighted(master, n, seed=5)
+    assert np.array_equal(hit, ohit), np.nonzero(hit != ohit)[0][:10]
+    assert np.array_equal(hp, ohp)
+    np.testing.assert_allclose(w, ow, rtol=1e-13, atol=0)
+    assert hops == ohp.sum()
+    nr = len(rects)
+    bins = np.where(ohit >= 0, ohit, np.where(ohit == -1, nr, nr + 1))
+    assert np.array_equal(counts, np.bincount(bins, minlength=nr + 2))
+    want = np.bincount(bins, weights=ow, minlength=nr + 2)
+    want_q = np.bincount(bins, weights=ow * ow, minlength=nr + 2)
+    scale = np.abs(ow).sum() + 1e-300
+    assert np.abs(wsum - want).max() <= 1e-12 * scale
+    assert np.abs(wsq - want_q).max() <= 1e-12 * (ow * ow).sum()
+
+
+def test_weighted_seed_job_offset_and_max_hops():
+    lay, rects, bounds = load_fixture("test100.txt")
+    tree = frw.Tree(lay, leaf_cap=4)
+    ix = oracle_index(rects, bounds, O.REFTREE_DFS)
+    for seed, job, first, mh in [(9, 3, 12345, 0), (1, 0, 0, 2)]:
+        c, h, ws, wq, w, hit, hp = gpu_weighted(tree, 3, 1500, seed=seed, first=first, job=job, max_hops=mh)
+        ohit, ohp, ow = ix.walk_weighted(3, 1500, seed=seed, job=job, first_walk=first,
+                                         max_hops=mh if mh else 10 ** 6)
+        assert np.array_equal(hit, ohit) and np.array_equal(hp, ohp)
+        np.testing.assert_allclose(w, ow, rtol=1e-13, atol=0)
+
+
+def _plates(gap=100, length=2000, thick=10):
+    return np.array([[0, 0, length, thick], [0, thick + gap, length, 2 * thick + gap]], dtype=np.int32)
+
+
+def test_parallel_plates():
+    lay = frw.Layout.from_rects(_plates())
+    tree = frw.Tree(lay)
+    n = 2_000_000
+    counts, wsum, wsq, hops = tree.walk_weighted(0, n, seed=11)
+    c = wsum / n
+    sig = np.sqrt(np.maximum(wsq / n - c * c, 0) / n)
+    assert counts.sum() == n
+    assert -25.0 < c[1] < -20.0, (c, sig)  # -length/gap = -20, plus fringe
+    assert c[0] > 0 and c[2] < 0           # self term; the grounded boundary
+    tot = wsum.sum() / n
+    sig_tot = np.sqrt(max(wsq.sum() / n - tot * tot, 0) / n)
+    assert abs(tot) < 4 * sig_tot, (tot, sig_tot)
+    # the same walks again: identical tallies, sums equal to the last bits
+    counts2, wsum2, wsq2, hops2 = tree.walk_weighted(0, n, seed=11)
+    assert np.array_equal(counts, counts2) and hops == hops2
+    np.testing.assert_allclose(wsum2, wsum, rtol=1e-12, atol=1e-9 * np.abs(wsum).max())
+
+
+def test_weighted_at_scale():
+    lay = frw.Layout.generate(10 ** 6, 10 ** 7, 5, 300, seed=1, cluster_frac=0.2, sigma=1e5, n_clusters=100)
+    tree = frw.Tree(lay)
+    n = 10 ** 7
+    counts, wsum, wsq, hops = tree.walk_weighted(0, n, seed=2)
+    assert counts.sum() == n and np.isfinite(wsum).all() and np.isfinite(wsq).all()
+    c = wsum / n
+    assert c[0] > 0
+    tot = wsum.sum() / n
+    sig_tot = np.sqrt(max(wsq.sum() / n - tot * tot, 0) / n)
+    assert abs(tot) < 4 * sig_tot, (tot, sig_tot)
+
+
+def test_weighted_rejects_bad_arguments():
+    lay, rects, bounds = load_fixture("test")
+    tree = frw.Tree(lay)
+    with pytest.raises(frw.FrwError):
+        tree.walk_weighted(0, 100, eps=0.0)
+    with pytest.raises(frw.FrwError):
+        tree.walk_weighted(len(rects), 100)
