@@ -358,6 +358,40 @@ def main():
             "avg_launch_ms": avg_ms,
         },
     }
+    # the physical capacitance row of the master (weighted estimator), same layout
+    nw = min(walks, 10_000_000)
+    wc = torch.zeros(nr + 2, dtype=torch.int64, device="cuda")
+    wh = torch.zeros(1, dtype=torch.int64, device="cuda")
+    wsum = torch.zeros(nr + 2, dtype=torch.float64, device="cuda")
+    wsq = torch.zeros(nr + 2, dtype=torch.float64, device="cuda")
+    tree.walk_weighted_async(master, nw, wc, wh, wsum, wsq, seed=1, first_walk=1 << 31,
+                             stream=stream.cuda_stream)  # warm-up launch
+    torch.cuda.synchronize()
+    for t_ in (wc, wh, wsum, wsq):
+        t_.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tree.walk_weighted_async(master, nw, wc, wh, wsum, wsq, seed=1, first_walk=0, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    wt = time.perf_counter() - t0
+    ws, wq = wsum.cpu().numpy(), wsq.cpu().numpy()
+    c_self = ws[master] / nw
+    var = max(wq[master] / nw - c_self * c_self, 0.0)
+    n1w = math.ceil(var * (300.0 / c_self) ** 2) if c_self > 0 else float("inf")
+    out["weighted"] = {
+        "walks": nw,
+        "walks_per_s": nw / wt,
+        "walk_steps_per_s": int(wh.item()) / wt,
+        "c_self": c_self,
+        "c_self_rel_3sigma": 3.0 * math.sqrt(var / nw) / c_self if c_self > 0 else None,
+        "c_escape": ws[nr] / nw,
+        "sum_over_bins": float(ws.sum() / nw),
+        "walks_for_1pct": n1w,
+        "time_to_1pct_s": info.build_ms / 1e3 + n1w / (nw / wt),
+        "units": "eps per unit length",
+    }
+    log(f"weighted: C_self {c_self:.4g} (3 sigma {out['weighted']['c_self_rel_3sigma']:.2%}), "
+        f"{nw / wt:.3e} walks/s")
     if mode == "learned":
         out["learned"] = dict(train_info, alpha=args.alpha,
                               accept_rate=r["accepted"] / max(1, r["accepted"] + r["fallback"]))
