@@ -146,6 +146,7 @@ int frw_green_thresholds(uint32_t out[FRW_NSAMPLE])
 void frw_tree_free(frw_tree* t)
 {
     if (!t) return;
+    free_refcompat(t);
     if (t->d_rects) hipFree(t->d_rects);
     if (t->d_ids) hipFree(t->d_ids);
     if (t->d_nodes) hipFree(t->d_nodes);
@@ -164,6 +165,8 @@ int frw_tree_build(const frw_layout* l, const frw_tree_params* p, void* stream, 
     if (l->rects.empty()) return set_error(FRW_E_RANGE, "layout has no rects");
     const int cap = (p && p->leaf_cap > 0) ? p->leaf_cap : 16;
     if (cap > 64) return set_error(FRW_E_RANGE, "leaf_cap must be <= 64");
+    const int kind = p ? p->kind : FRW_TREE_MORTON;
+    if (kind != FRW_TREE_MORTON && kind != FRW_TREE_REFCOMPAT) return set_error(FRW_E_INVALID, "unknown tree kind");
     FRW_GUARD_BEGIN
     hipStream_t s = (hipStream_t)stream;
     frw_tree* t = new frw_tree;
@@ -199,6 +202,7 @@ int frw_tree_build(const frw_layout* l, const frw_tree_params* p, void* stream, 
         if (rc == FRW_OK) rc = check_hip(hipMemset(t->d_err, 0, sizeof(int)), "memset err");
         if (rc == FRW_OK) rc = check_hip(hipMalloc(&t->d_surface, 4 * sizeof(double)), "hipMalloc surface");
         if (rc == FRW_OK) rc = check_hip(hipMalloc(&t->d_job, sizeof(WalkJob)), "hipMalloc job");
+        if (rc == FRW_OK && kind == FRW_TREE_REFCOMPAT) rc = build_refcompat(t);
     }
     if (rc != FRW_OK) {
         frw_tree_free(t);
@@ -249,6 +253,30 @@ static int check_err_flag(const frw_tree* t, hipStream_t s)
     return FRW_OK;
 }
 
+int frw_query_refcompat(const frw_tree* t, const double* pts, const double* boxes, int64_t n, uint8_t* q1,
+                        int32_t* q2, double* q3, void* stream)
+{
+    if (!t || n < 0 || (n > 0 && ((q1 && !boxes) || ((q2 || q3) && !pts))))
+        return set_error(FRW_E_INVALID, "bad query arguments");
+    FRW_GUARD_BEGIN
+    return launch_ref_queries(t, pts, boxes, n, q1, q2, q3, (hipStream_t)stream);
+    FRW_GUARD_END
+}
+
+int frw_refcompat_gauss_surface(const frw_tree* t, int32_t master, double out[4])
+{
+    if (!t || !out || master < 0 || master >= t->n_rects) return set_error(FRW_E_INVALID, "bad arguments");
+    FRW_GUARD_BEGIN
+    return refcompat_gauss(t, master, out);
+    FRW_GUARD_END
+}
+
+int frw_refcompat_info(const frw_tree* t, int64_t out[2])
+{
+    if (!t || !out) return set_error(FRW_E_INVALID, "null argument");
+    return refcompat_info(t, out);
+}
+
 int frw_query_collide(const frw_tree* t, const double* boxes, int64_t n, uint8_t* out, void* stream)
 {
     if (!t || n < 0 || (n > 0 && (!boxes || !out))) return set_error(FRW_E_INVALID, "bad query arguments");
@@ -295,8 +323,10 @@ static int check_walk_params(const frw_tree* t, const frw_walk_params* p)
     if (p->n_walks <= 0) return set_error(FRW_E_RANGE, "walk count must be positive");
     if (p->first_walk < 0 || p->first_walk + p->n_walks > ((int64_t)1 << 32))
         return set_error(FRW_E_RANGE, "walk ids must fit in 32 bits");
-    if (p->mode != FRW_MODE_EXACT && p->mode != FRW_MODE_BISECT)
+    if (p->mode != FRW_MODE_EXACT && p->mode != FRW_MODE_BISECT && p->mode != FRW_MODE_REFCOMPAT)
         return set_error(FRW_E_INVALID, "unsupported walk mode");
+    if (p->mode == FRW_MODE_REFCOMPAT && !t->ref)
+        return set_error(FRW_E_INVALID, "reference-compatible walks need a tree built with FRW_TREE_REFCOMPAT");
     if (p->max_hops < 0 || p->max_hops > 0xFFFFFFFFll) return set_error(FRW_E_RANGE, "bad max_hops");
     return FRW_OK;
 }

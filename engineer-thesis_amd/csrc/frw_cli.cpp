@@ -14,7 +14,7 @@ static const char* kHelp =
     "\t-M\t--measure\t\tprint build/walk timings\n"
     "\t-O\t--object\t\tmaster rect id\n"
     "\t-I\t--iterations\t\tnumber of walks\n"
-    "\t-G\t--GPU\t\t\texact-search engine (default: reference bisection)\n"
+    "\t-G\t--GPU\t\t\texact-search engine (default: reference-compatible tree and queries)\n"
     "\t-L\t--layer\t\t\tlayer number\n"
     "\t\t--help\t\t\tthis text\n";
 

@@ -84,8 +84,13 @@ struct frw_layout {
     int4 bounds;              // padded absorbing bounds
 };
 
+namespace frw {
+struct RefCompat;
+}
+
 struct frw_tree {
     int device = 0;
+    frw::RefCompat* ref = nullptr;  // reference-compatible index (FRW_TREE_REFCOMPAT)
     int4 bounds{};
     int64_t n_rects = 0;
     int64_t n_nodes = 0;
@@ -140,6 +145,15 @@ int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds);
 int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s);
 int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o,
                 hipStream_t s, unsigned long long* traffic = nullptr, const WalkExtras* extra = nullptr);
+
+// reference-compatible index (frw_refcompat.hip)
+int build_refcompat(frw_tree* t);
+void free_refcompat(frw_tree* t);
+int refcompat_info(const frw_tree* t, int64_t out[2]);
+int refcompat_gauss(const frw_tree* t, int master, double out[4]);
+int launch_ref_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o, hipStream_t s);
+int launch_ref_queries(const frw_tree* t, const double* pts, const double* boxes, int64_t n, uint8_t* q1,
+                       int32_t* q2, double* q3, hipStream_t s);
 
 // learned proposal views (frw_mlp.hip)
 MlpPyramid mlp_pyramid(const frw_mlp* m);

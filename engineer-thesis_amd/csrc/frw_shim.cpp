@@ -61,7 +61,9 @@ floatingPoint run_engine(char* path, int iters, int rect_id, bool measure, int l
         return -1.0;
     }
     frw_tree* t = nullptr;
-    if (frw_tree_build(l, nullptr, nullptr, &t) != FRW_OK) {
+    frw_tree_params tp{};
+    tp.kind = mode == FRW_MODE_REFCOMPAT ? FRW_TREE_REFCOMPAT : FRW_TREE_MORTON;
+    if (frw_tree_build(l, &tp, nullptr, &t) != FRW_OK) {
         std::fprintf(stderr, "%s %s\n", tag, frw_last_error_string());
         frw_layout_free(l);
         return -1.0;
@@ -145,7 +147,7 @@ void runRandomWalk(char* path, int ITER_NUM, int RECT_ID, bool GPU_FLAG, bool me
 
 floatingPoint getAvgPathLen(char* path, int ITER_NUM, int RECT_ID, bool measure, int layerID)
 {
-    return run_engine(path, ITER_NUM, RECT_ID, measure, layerID, FRW_MODE_BISECT, "[CPU]");
+    return run_engine(path, ITER_NUM, RECT_ID, measure, layerID, FRW_MODE_REFCOMPAT, "[CPU]");
 }
 
 floatingPoint getAvgPathLenCUDA(char* path, int ITER_NUM, int RECT_ID, bool measure, int layerID)

@@ -33,6 +33,7 @@ enum WalkMode : int32_t {
     kModeBisect = 1,   // reference bisection, one Q1 per step
     kModeLearned = 2,  // network proposal verified by a pyramid probe, exact fallback
     kModeWeighted = 3, // exact walks from a Gaussian-surface start with first-hop weights
+    kModeRefCompat = 4,  // the reference CPU tree and its queries, defects included
 };
 
 }  // namespace frw

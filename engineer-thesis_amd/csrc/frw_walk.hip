@@ -523,6 +523,10 @@ int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outp
                 hipStream_t s, unsigned long long* traffic, const WalkExtras* extra)
 {
     if (p->n_walks == 0) return FRW_OK;
+    if (p->mode == kModeRefCompat) {
+        if (traffic) return set_error(FRW_E_INVALID, "no traffic counters in the reference-compatible mode");
+        return launch_ref_walk(t, p, o, s);
+    }
     WalkJob jb{};
     jb.tree = dev_view(t);
     jb.counts = (unsigned long long*)o->counts;

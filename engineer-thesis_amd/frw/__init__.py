@@ -14,7 +14,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfrw.so")
 
-MODE_EXACT, MODE_BISECT, MODE_LEARNED, MODE_WEIGHTED = 0, 1, 2, 3
+MODE_EXACT, MODE_BISECT, MODE_LEARNED, MODE_WEIGHTED, MODE_REFCOMPAT = 0, 1, 2, 3, 4
+TREE_MORTON, TREE_REFCOMPAT = 0, 1
 GEN_UNIFORM, GEN_CLUSTERED = 0, 1
 NSAMPLE = 200
 
@@ -35,7 +36,7 @@ class GenParams(C.Structure):
 
 
 class TreeParams(C.Structure):
-    _fields_ = [("leaf_cap", C.c_int32), ("reserved", C.c_int32 * 7)]
+    _fields_ = [("leaf_cap", C.c_int32), ("kind", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 class TreeInfo(C.Structure):
@@ -88,6 +89,7 @@ EXPORTS = [
     "frw_walk_learned", "frw_walk_learned_async", "frw_walk_learned_traffic",
     "frw_mlp_levels", "frw_mlp_certify", "frw_mlp_add_samples", "frw_batch_set_proposal",
     "frw_batch_walk_learned", "frw_batch_autotune_learned", "frw_walk_weighted_async", "frw_walk_weighted",
+    "frw_query_refcompat", "frw_refcompat_gauss_surface", "frw_refcompat_info",
 ]
 
 _lib = None
@@ -129,6 +131,9 @@ def lib():
     L.frw_tree_free.argtypes = [vp]
     L.frw_tree_free.restype = None
     L.frw_query_collide.argtypes = [vp, vp, i64, vp, vp]
+    L.frw_query_refcompat.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp]
+    L.frw_refcompat_gauss_surface.argtypes = [vp, i32, vp]
+    L.frw_refcompat_info.argtypes = [vp, vp]
     L.frw_query_point.argtypes = [vp, vp, i64, vp, vp]
     L.frw_query_nearest.argtypes = [vp, vp, i64, vp, vp]
     L.frw_query_maxsquare.argtypes = [vp, vp, i64, i32, vp, vp]
@@ -258,9 +263,9 @@ class Layout:
 class Tree:
     """Device quadtree of a layout (Morton sort-and-split build)."""
 
-    def __init__(self, layout, leaf_cap=0, stream=None):
+    def __init__(self, layout, leaf_cap=0, stream=None, ref_compat=False):
         self.layout = layout
-        p = TreeParams(leaf_cap=leaf_cap)
+        p = TreeParams(leaf_cap=leaf_cap, kind=TREE_REFCOMPAT if ref_compat else TREE_MORTON)
         h = C.c_void_p()
         _check(lib().frw_tree_build(layout.h, C.byref(p), stream, C.byref(h)))
         self.h = h
@@ -291,6 +296,22 @@ class Tree:
     def maxsquare(self, pts, out, mode=MODE_EXACT, stream=None):
         _check(lib().frw_query_maxsquare(self.h, pts.data_ptr(), pts.shape[0], mode, out.data_ptr(),
                                          stream))
+
+    def refcompat_queries(self, pts=None, boxes=None, q1=None, q2=None, q3=None, stream=None):
+        """Reference-compatible Q1 (boxes), Q2 and Q3 (points) into device tensors."""
+        n = pts.shape[0] if pts is not None else boxes.shape[0]
+        ptr = lambda x: x.data_ptr() if x is not None else None
+        _check(lib().frw_query_refcompat(self.h, ptr(pts), ptr(boxes), n, ptr(q1), ptr(q2), ptr(q3), stream))
+
+    def refcompat_gauss_surface(self, master):
+        out = np.zeros(4, dtype=np.float64)
+        _check(lib().frw_refcompat_gauss_surface(self.h, master, out.ctypes.data))
+        return tuple(out)
+
+    def refcompat_info(self):
+        out = np.zeros(2, dtype=np.int64)
+        _check(lib().frw_refcompat_info(self.h, out.ctypes.data))
+        return tuple(int(v) for v in out)
 
     def gauss_surface(self, master):
         out = np.zeros(4, dtype=np.float64)

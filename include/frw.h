@@ -41,7 +41,8 @@ enum {
     FRW_E_KERNEL = -8   /* a kernel raised an internal error flag */
 };
 
-enum { FRW_MODE_EXACT = 0, FRW_MODE_BISECT = 1, FRW_MODE_LEARNED = 2, FRW_MODE_WEIGHTED = 3 };
+enum { FRW_MODE_EXACT = 0, FRW_MODE_BISECT = 1, FRW_MODE_LEARNED = 2, FRW_MODE_WEIGHTED = 3, FRW_MODE_REFCOMPAT = 4 };
+enum { FRW_TREE_MORTON = 0, FRW_TREE_REFCOMPAT = 1 };
 enum { FRW_GEN_UNIFORM = 0, FRW_GEN_CLUSTERED = 1 };
 enum { FRW_NSAMPLE = 200 };
 
@@ -94,7 +95,9 @@ int frw_green_thresholds(uint32_t out[FRW_NSAMPLE]);
 
 typedef struct {
     int32_t leaf_cap;     /* max rects per leaf (0 = default 16, <= 64) */
-    int32_t reserved[7];
+    int32_t kind;         /* FRW_TREE_MORTON, or FRW_TREE_REFCOMPAT: also build the reference's own
+                             CPU tree (randomWalk/CPU/quadTree.cpp:73-146) for FRW_MODE_REFCOMPAT */
+    int32_t reserved[6];
 } frw_tree_params;
 
 typedef struct {
@@ -337,6 +340,22 @@ int frw_walk_weighted_async(const frw_tree* t, const frw_walk_params* p, const f
 /* Synchronous form into host arrays (counts int64[n+2], wsum/wsq double[n+2]). */
 int frw_walk_weighted(const frw_tree* t, const frw_walk_params* p, double eps, int64_t* counts_host,
                       double* wsum_host, double* wsq_host, int64_t* hops);
+
+/* Reference-compatible mode (FRW_MODE_REFCOMPAT walks on a tree built with
+ * FRW_TREE_REFCOMPAT): the reference CPU tree with its insertion behaviour
+ * and its queries as written — Q1 along one root-to-leaf path
+ * (randomWalk/CPU/quadTree.cpp:148-217), Q2 in the first child containing p
+ * (quadTree.cpp:240-262), Q3 the literal bisection (quadTree.cpp:264-312) —
+ * so walks reproduce the reference CPU FRW (randomWalk/CPU/mainFunctions.cpp:
+ * 48-90) walk for walk for the same random stream.  One lane per walk.
+ * Query hook: q1[i] for boxes[i] (x0,y0,x1,y1), q2[i] / q3[i] (4 doubles)
+ * for pts[i]; any output may be NULL.  Synchronous. */
+int frw_query_refcompat(const frw_tree* t, const double* pts, const double* boxes, int64_t n, uint8_t* q1,
+                        int32_t* q2, double* q3, void* stream);
+/* The reference's Gaussian surface of `master` on its own tree (quadTree.cpp:315-364). */
+int frw_refcompat_gauss_surface(const frw_tree* t, int32_t master, double out[4]);
+/* {nodes, object entries} of the reference-compatible tree. */
+int frw_refcompat_info(const frw_tree* t, int64_t out[2]);
 
 /* Measurement hook (no reference counterpart): the same walks as frw_walk
  * with per-hop traffic counters compiled in.  out = {node records read

@@ -8,8 +8,10 @@
 //   - errors that the reference answers with exit(0) print a message to
 //     stderr and make the function return (runRandomWalk) or return -1.0;
 //   - both engines run on the GPU: GPU_FLAG=true runs the exact-search walk,
-//     GPU_FLAG=false runs the reference's literal bisection (the algorithm of
-//     the reference CPU path) through the same device engine;
+//     GPU_FLAG=false runs the reference-compatible mode (the reference CPU
+//     tree rebuilt with its insertion behaviour, its single-path queries and
+//     literal bisection), which reproduces the reference CPU FRW walk for
+//     walk under the same random stream;
 //   - walk streams are Philox4x32-10 keyed by FRW_SEED (default 1), so runs
 //     are reproducible; the reference re-seeds from /dev/urandom.
 #ifndef FRW_RANDOMWALK_H
