@@ -2,6 +2,7 @@
 // returns a status code; errors are recorded per thread and never thrown
 // across the boundary.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -193,8 +194,8 @@ int frw_tree_build(const frw_layout* l, const frw_tree_params* p, void* stream, 
         green_thresholds(T);
         rc = check_hip(hipMalloc(&t->d_thresholds, sizeof(T)), "hipMalloc thresholds");
         if (rc == FRW_OK) rc = check_hip(hipMemcpy(t->d_thresholds, T, sizeof(T), hipMemcpyHostToDevice), "copy thresholds");
-        double wt[3 * kNSample];
-        green_grad_tables(wt, wt + kNSample, wt + 2 * kNSample);
+        double wt[6 * kNSample];  // 5 x 200 doubles, then 2 x 200 u32 thresholds
+        green_grad_tables(wt, reinterpret_cast<uint32_t*>(wt + 5 * kNSample));
         if (rc == FRW_OK) rc = check_hip(hipMalloc(&t->d_wtab, sizeof(wt)), "hipMalloc weight tables");
         if (rc == FRW_OK) rc = check_hip(hipMemcpy(t->d_wtab, wt, sizeof(wt), hipMemcpyHostToDevice), "copy weight tables");
         if (rc == FRW_OK) rc = check_hip(hipMalloc(&t->d_work, sizeof(unsigned long long)), "hipMalloc work");
@@ -341,8 +342,13 @@ int frw_walk_async(const frw_tree* t, const frw_walk_params* p, const frw_walk_o
     FRW_GUARD_END
 }
 
+void frw_weight_tables(double wt[1000], uint32_t ithr[400])
+{
+    if (wt && ithr) green_grad_tables(wt, ithr);
+}
+
 int frw_walk_weighted_async(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o, double eps,
-                            double* wsum, double* wsq, double* w_out, void* stream)
+                            uint32_t flags, double* wsum, double* wsq, double* w_out, void* stream)
 {
     if (!t || !p) return set_error(FRW_E_INVALID, "null argument");
     frw_walk_params q = *p;
@@ -352,6 +358,7 @@ int frw_walk_weighted_async(const frw_tree* t, const frw_walk_params* p, const f
     if (!o || !o->counts || !o->hops || !wsum || !wsq)
         return set_error(FRW_E_INVALID, "weighted walks need counts, hops, wsum and wsq");
     if (!(eps > 0.0) || !std::isfinite(eps)) return set_error(FRW_E_INVALID, "eps must be positive");
+    if (flags & ~(uint32_t)FRW_WEIGHT_IMPORTANCE) return set_error(FRW_E_INVALID, "unknown weighted-walk flags");
     FRW_GUARD_BEGIN
     q.mode = FRW_MODE_WEIGHTED;
     WalkExtras ex;
@@ -359,11 +366,12 @@ int frw_walk_weighted_async(const frw_tree* t, const frw_walk_params* p, const f
     ex.wsq = wsq;
     ex.w_out = w_out;
     ex.eps = eps;
+    ex.wflags = flags;
     return launch_walk(t, &q, o, (hipStream_t)stream, nullptr, &ex);
     FRW_GUARD_END
 }
 
-int frw_walk_weighted(const frw_tree* t, const frw_walk_params* p, double eps, int64_t* counts_host,
+int frw_walk_weighted(const frw_tree* t, const frw_walk_params* p, double eps, uint32_t flags, int64_t* counts_host,
                       double* wsum_host, double* wsq_host, int64_t* hops)
 {
     if (!counts_host || !wsum_host || !wsq_host) return set_error(FRW_E_INVALID, "null output");
@@ -382,7 +390,7 @@ int frw_walk_weighted(const frw_tree* t, const frw_walk_params* p, double eps, i
     if (rc == FRW_OK) rc = check_hip(hipMemset(d_wq, 0, sizeof(double) * nb), "memset");
     if (rc == FRW_OK) {
         frw_walk_outputs o{d_counts, d_hops, nullptr, nullptr};
-        rc = frw_walk_weighted_async(t, p, &o, eps, d_ws, d_wq, nullptr, nullptr);
+        rc = frw_walk_weighted_async(t, p, &o, eps, flags, d_ws, d_wq, nullptr, nullptr);
     }
     if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "weighted walk");
     if (rc == FRW_OK) rc = check_err_flag(t, nullptr);
@@ -392,6 +400,110 @@ int frw_walk_weighted(const frw_tree* t, const frw_walk_params* p, double eps, i
     if (rc == FRW_OK && hops) rc = check_hip(hipMemcpy(hops, d_hops, sizeof(int64_t), hipMemcpyDeviceToHost), "copy");
     for (void* q : {(void*)d_counts, (void*)d_hops, (void*)d_ws, (void*)d_wq})
         if (q) hipFree(q);
+    return rc;
+    FRW_GUARD_END
+}
+
+int frw_walk_adaptive(const frw_tree* t, const frw_walk_params* p, double rel_err, int64_t max_walks, double eps,
+                      uint32_t wflags, int64_t* counts_host, double* wsum_host, double* wsq_host,
+                      frw_adaptive_result* r)
+{
+    if (!t || !p || !counts_host || !r) return set_error(FRW_E_INVALID, "null argument");
+    if (!(rel_err > 0.0) || !std::isfinite(rel_err)) return set_error(FRW_E_INVALID, "rel_err must be positive");
+    const bool weighted = p->mode == FRW_MODE_WEIGHTED;
+    const int64_t id_space = ((int64_t)1 << 32) - p->first_walk;
+    if (max_walks <= 0 || max_walks > id_space) max_walks = id_space;
+    frw_walk_params q = *p;
+    if (q.n_walks == 0) q.n_walks = 65536;
+    q.n_walks = std::min(q.n_walks, max_walks);
+    {
+        frw_walk_params c = q;
+        if (weighted) c.mode = FRW_MODE_EXACT;
+        const int rc = check_walk_params(t, &c);
+        if (rc != FRW_OK) return rc;
+    }
+    if (weighted && (!(eps > 0.0) || !std::isfinite(eps))) return set_error(FRW_E_INVALID, "eps must be positive");
+    FRW_GUARD_BEGIN
+    const int64_t nb = t->n_rects + 2;
+    // getAvgPathLen counts a hit as "other" unless the rect equals the master
+    // by value (randomWalk/CPU/getAvgPathLen.cpp:96)
+    std::vector<int64_t> same_ids;
+    const int4 m = t->host_rects[q.master];
+    for (int64_t i = 0; i < t->n_rects; ++i) {
+        const int4 x = t->host_rects[i];
+        if (x.x == m.x && x.y == m.y && x.z == m.z && x.w == m.w) same_ids.push_back(i);
+    }
+    int64_t *d_counts = nullptr, *d_hops = nullptr;
+    double *d_ws = nullptr, *d_wq = nullptr;
+    int rc = check_hip(hipMalloc(&d_counts, sizeof(int64_t) * nb), "hipMalloc counts");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hops, sizeof(int64_t)), "hipMalloc hops");
+    if (rc == FRW_OK && weighted) rc = check_hip(hipMalloc(&d_ws, sizeof(double) * nb), "hipMalloc wsum");
+    if (rc == FRW_OK && weighted) rc = check_hip(hipMalloc(&d_wq, sizeof(double) * nb), "hipMalloc wsq");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_counts, 0, sizeof(int64_t) * nb), "memset counts");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_hops, 0, sizeof(int64_t)), "memset hops");
+    if (rc == FRW_OK && weighted) rc = check_hip(hipMemset(d_ws, 0, sizeof(double) * nb), "memset wsum");
+    if (rc == FRW_OK && weighted) rc = check_hip(hipMemset(d_wq, 0, sizeof(double) * nb), "memset wsq");
+    if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "adaptive set-up");
+    frw_adaptive_result res{};
+    res.rel_err_3sigma = INFINITY;
+    const auto t0 = std::chrono::steady_clock::now();
+    int64_t next = q.n_walks;
+    const frw_walk_outputs o{d_counts, d_hops, nullptr, nullptr};
+    while (rc == FRW_OK) {
+        frw_walk_params c = q;
+        c.first_walk = q.first_walk + res.n_walks;
+        c.n_walks = next;
+        rc = weighted ? frw_walk_weighted_async(t, &c, &o, eps, wflags, d_ws, d_wq, nullptr, nullptr)
+                      : launch_walk(t, &c, &o, nullptr);
+        if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "adaptive round");
+        if (rc == FRW_OK) rc = check_err_flag(t, nullptr);
+        if (rc != FRW_OK) break;
+        res.n_walks += next;
+        res.rounds++;
+        const double n = (double)res.n_walks;
+        if (weighted) {
+            double sm = 0.0, sq = 0.0;
+            rc = check_hip(hipMemcpy(&sm, d_ws + q.master, sizeof(double), hipMemcpyDeviceToHost), "read wsum");
+            if (rc == FRW_OK)
+                rc = check_hip(hipMemcpy(&sq, d_wq + q.master, sizeof(double), hipMemcpyDeviceToHost), "read wsq");
+            res.value = sm / n;
+            const double var = std::max(sq / n - res.value * res.value, 0.0);
+            res.rel_err_3sigma = res.value != 0.0 ? 3.0 * std::sqrt(var / n) / std::fabs(res.value) : INFINITY;
+        } else {
+            int64_t same = 0;
+            for (int64_t id : same_ids) {
+                int64_t cm = 0;
+                if (rc == FRW_OK)
+                    rc = check_hip(hipMemcpy(&cm, d_counts + id, sizeof(int64_t), hipMemcpyDeviceToHost), "read count");
+                same += cm;
+            }
+            const double f = (n - (double)same) / n;
+            res.value = f;
+            res.rel_err_3sigma = f > 0.0 ? 3.0 * std::sqrt(f * (1.0 - f) / n) / f : INFINITY;
+        }
+        if (rc != FRW_OK) break;
+        if (res.rel_err_3sigma <= rel_err) {
+            res.converged = 1;
+            break;
+        }
+        if (res.n_walks >= max_walks) break;
+        // Yu et al. 2013 eq. 32: n_rem = n (err^2 / goal^2 - 1), with a 5 % margin
+        const double e = res.rel_err_3sigma / rel_err;
+        const double want = std::isfinite(e) ? n * (e * e - 1.0) * 1.05 : n;
+        next = std::max<int64_t>((int64_t)std::ceil(want), std::max<int64_t>(res.n_walks / 16, 1024));
+        next = std::min<int64_t>(next, max_walks - res.n_walks);
+    }
+    res.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (rc == FRW_OK)
+        rc = check_hip(hipMemcpy(counts_host, d_counts, sizeof(int64_t) * nb, hipMemcpyDeviceToHost), "copy counts");
+    if (rc == FRW_OK) rc = check_hip(hipMemcpy(&res.hops, d_hops, sizeof(int64_t), hipMemcpyDeviceToHost), "copy hops");
+    if (rc == FRW_OK && weighted && wsum_host)
+        rc = check_hip(hipMemcpy(wsum_host, d_ws, sizeof(double) * nb, hipMemcpyDeviceToHost), "copy wsum");
+    if (rc == FRW_OK && weighted && wsq_host)
+        rc = check_hip(hipMemcpy(wsq_host, d_wq, sizeof(double) * nb, hipMemcpyDeviceToHost), "copy wsq");
+    for (void* x : {(void*)d_counts, (void*)d_hops, (void*)d_ws, (void*)d_wq})
+        if (x) (void)hipFree(x);
+    if (rc == FRW_OK) *r = res;
     return rc;
     FRW_GUARD_END
 }

@@ -59,6 +59,7 @@ struct WalkLaunch {
     // weighted mode
     const double* wtab = nullptr;
     double eps = 1.0;
+    uint32_t wflags = 0;
 };
 
 // Optional extras of a single-layout walk launch.
@@ -75,6 +76,7 @@ struct WalkExtras {
     double* wsq = nullptr;
     double* w_out = nullptr;
     double eps = 1.0;
+    uint32_t wflags = 0;  // FRW_WEIGHT_* options
 };
 
 }  // namespace frw
@@ -105,7 +107,8 @@ struct frw_tree {
     int32_t* d_ids = nullptr;
     frw::Node* d_nodes = nullptr;
     uint32_t* d_thresholds = nullptr;  // Green CDF panel thresholds
-    double* d_wtab = nullptr;          // weighted estimator: gx | gy | pw, 3 x 200
+    double* d_wtab = nullptr;          // weighted estimator: gx | gy | pw | qx | qy (5 x 200), then
+                                       // the importance thresholds (2 x 200 u32)
     // walk scratch owned by the tree handle
     unsigned long long* d_work = nullptr;  // walk work-queue counter
     int* d_err = nullptr;                  // kernel error bits
@@ -121,7 +124,11 @@ int check_hip(hipError_t e, const char* what);
 
 // host tables
 void green_thresholds(uint32_t T[kNSample]);
-void green_grad_tables(double gx[kNSample], double gy[kNSample], double pw[kNSample]);
+// Weighted-estimator tables.  wt (5 x 200 doubles): gx | gy (Green-gradient
+// panel weights, unit square, source at the centre, y down) | pw (Green-CDF
+// draw probabilities) | qx | qy (draw probabilities of the importance tables);
+// ithr (2 x 200 u32): thresholds drawing panels proportional to |gx| and |gy|.
+void green_grad_tables(double* wt, uint32_t* ithr);
 
 // device build (frw_build.hip)
 int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 bounds, int leaf_cap,

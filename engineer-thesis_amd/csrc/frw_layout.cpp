@@ -163,8 +163,29 @@ void green_thresholds(uint32_t T[kNSample])
 // (u, v) per side: top (x, y), right (y, 1 - x), bottom (1 - x, 1 - y),
 // left (1 - y, x).  Panel values are midpoint densities / (50 T), T the
 // normalisation of the panel probabilities.
-void green_grad_tables(double gx[kNSample], double gy[kNSample], double pw[kNSample])
+// First-hop importance table: panel k drawn with probability q_k proportional
+// to |g[k]| (one component of the Green gradient), as integer thresholds over
+// u32 draws; q_k is the exact draw probability (threshold difference).
+static void importance_table(const double* g, uint32_t* T, double* q)
 {
+    double S = 0.0;
+    for (int k = 0; k < kNSample; ++k) S += std::fabs(g[k]);
+    double cum = 0.0;
+    for (int k = 0; k < kNSample; ++k) {
+        T[k] = (uint32_t)std::floor(cum / S * 4294967296.0);
+        cum += std::fabs(g[k]);
+    }
+    for (int k = 0; k < kNSample; ++k) {
+        const double hi = k + 1 < kNSample ? (double)T[k + 1] : 4294967296.0;
+        q[k] = (hi - (double)T[k]) / 4294967296.0;
+    }
+}
+
+void green_grad_tables(double* wt, uint32_t* ithr)
+{
+    double* gx = wt;
+    double* gy = wt + kNSample;
+    double* pw = wt + 2 * kNSample;
     const double pi = 3.14159265358979323846;
     const int per = kNSample / 4;
     double raw = 0.0;
@@ -197,6 +218,8 @@ void green_grad_tables(double gx[kNSample], double gy[kNSample], double pw[kNSam
         const double hi = k + 1 < kNSample ? (double)th[k + 1] : 4294967296.0;
         pw[k] = (hi - (double)th[k]) / 4294967296.0;
     }
+    importance_table(gx, ithr, wt + 3 * kNSample);
+    importance_table(gy, ithr + kNSample, wt + 4 * kNSample);
 }
 
 // ---------------------------------------------------------------- generator

@@ -36,4 +36,7 @@ enum WalkMode : int32_t {
     kModeRefCompat = 4,  // the reference CPU tree and its queries, defects included
 };
 
+// Weighted-walk options (FRW_WEIGHT_* in the C ABI).
+constexpr uint32_t kWeightImportance = 1u;  // first panel drawn proportional to |n . grad G|
+
 }  // namespace frw

@@ -48,9 +48,12 @@ struct WalkArgs {
     uint64_t h_cap;
     // learned proposal
     unsigned long long* l_stats;  // accepted, fallback
-    // weighted estimator: first-hop tables gx[200] | gy[200] | pw[200], permittivity
+    // weighted estimator: first-hop tables gx | gy | pw | qx | qy (200 each)
+    // followed by the importance thresholds (2 x 200 u32), permittivity,
+    // FRW_WEIGHT_* options
     const double* wtab;
     double eps;
+    uint32_t wflags;
 };
 
 constexpr int kLdsJobs = 128;                                   // job walk offsets cached in LDS
@@ -239,7 +242,21 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
                     rnd[2] = 0u;
                     rnd[3] = kWalkDomain;
                     philox4x32_10(rnd, k0, k1);
-                    const int k = sample_panel(sT, rnd[0]);
+                    // the first panel: from the Green CDF (probability p_k),
+                    // or, with importance sampling, proportional to |n . g_k|
+                    // (probability q_k; n is axis-aligned, so the table of
+                    // |gx| or of |gy|)
+                    int k;
+                    double prob;
+                    if (a.wflags & kWeightImportance) {
+                        const int axis = nx != 0.0 ? 0 : 1;
+                        const uint32_t* it = reinterpret_cast<const uint32_t*>(a.wtab + 5 * kNSample);
+                        k = sample_panel(it + axis * kNSample, rnd[0]);
+                        prob = a.wtab[(3 + axis) * kNSample + k];
+                    } else {
+                        k = sample_panel(sT, rnd[0]);
+                        prob = a.wtab[2 * kNSample + k];
+                    }
                     const double side = 2.0 * hh;
                     const double tk = ((double)(k % (kNSample / 4)) + 0.5) / (kNSample / 4) * side;
                     const double x0 = rx - hh, y0 = ry - hh;
@@ -249,8 +266,7 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
                     case 2: fpx = x0 + side - tk; fpy = y0 + side; break;
                     default: fpx = x0; fpy = y0 + side - tk; break;
                     }
-                    wgt = -a.eps * L * (nx * a.wtab[k] + ny * a.wtab[kNSample + k]) /
-                          (a.wtab[2 * kNSample + k] * 2.0 * hh);
+                    wgt = -a.eps * L * (nx * a.wtab[k] + ny * a.wtab[kNSample + k]) / (prob * 2.0 * hh);
                     first_pt = true;
                 }
             }
@@ -496,6 +512,7 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
     a.l_stats = L.l_stats;
     a.wtab = L.wtab;
     a.eps = L.eps;
+    a.wflags = L.wflags;
     MlpNet m{};
     if (L.net) m = *L.net;
     const bool st = L.traffic != nullptr;
@@ -570,6 +587,7 @@ int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outp
         L.h_cap = extra->h_cap;
         L.net = extra->net;
         L.eps = extra->eps;
+        L.wflags = extra->wflags;
         L.wtab = extra->wsum ? t->d_wtab : nullptr;
         L.l_stats = extra->l_stats;
     }

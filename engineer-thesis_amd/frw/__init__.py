@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libfrw.so")
 
 MODE_EXACT, MODE_BISECT, MODE_LEARNED, MODE_WEIGHTED, MODE_REFCOMPAT = 0, 1, 2, 3, 4
 TREE_MORTON, TREE_REFCOMPAT = 0, 1
+WEIGHT_IMPORTANCE = 1
 GEN_UNIFORM, GEN_CLUSTERED = 0, 1
 NSAMPLE = 200
 
@@ -67,6 +68,11 @@ class BatchParams(C.Structure):
                 ("use_graph", C.c_int32), ("seed", C.c_uint64), ("max_hops", C.c_int64)]
 
 
+class AdaptiveResult(C.Structure):
+    _fields_ = [("n_walks", C.c_int64), ("hops", C.c_int64), ("rounds", C.c_int32), ("converged", C.c_int32),
+                ("value", C.c_double), ("rel_err_3sigma", C.c_double), ("wall_ms", C.c_double)]
+
+
 class WalkSummary(C.Structure):
     _fields_ = [("n_walks", C.c_int64), ("hops", C.c_int64), ("n_master", C.c_int64),
                 ("n_escaped", C.c_int64), ("n_stuck", C.c_int64), ("fraction_other", C.c_double),
@@ -88,7 +94,7 @@ EXPORTS = [
     "frw_mlp_gradient", "frw_mlp_set_samples", "frw_mlp_train", "frw_walk_harvest",
     "frw_walk_learned", "frw_walk_learned_async", "frw_walk_learned_traffic",
     "frw_mlp_levels", "frw_mlp_certify", "frw_mlp_add_samples", "frw_batch_set_proposal",
-    "frw_batch_walk_learned", "frw_batch_autotune_learned", "frw_walk_weighted_async", "frw_walk_weighted",
+    "frw_batch_walk_learned", "frw_batch_autotune_learned", "frw_walk_weighted_async", "frw_walk_weighted", "frw_walk_adaptive", "frw_weight_tables",
     "frw_query_refcompat", "frw_refcompat_gauss_surface", "frw_refcompat_info",
 ]
 
@@ -142,8 +148,12 @@ def lib():
     L.frw_walk.argtypes = [vp, C.POINTER(WalkParams), vp, C.POINTER(WalkSummary)]
     L.frw_walk_traffic.argtypes = [vp, C.POINTER(WalkParams), vp]
     L.frw_walk_weighted_async.argtypes = [vp, C.POINTER(WalkParams), C.POINTER(WalkOutputs), C.c_double,
-                                          vp, vp, vp, vp]
-    L.frw_walk_weighted.argtypes = [vp, C.POINTER(WalkParams), C.c_double, vp, vp, vp, vp]
+                                          C.c_uint32, vp, vp, vp, vp]
+    L.frw_walk_weighted.argtypes = [vp, C.POINTER(WalkParams), C.c_double, C.c_uint32, vp, vp, vp, vp]
+    L.frw_weight_tables.argtypes = [vp, vp]
+    L.frw_weight_tables.restype = None
+    L.frw_walk_adaptive.argtypes = [vp, C.POINTER(WalkParams), C.c_double, i64, C.c_double, C.c_uint32, vp, vp,
+                                    vp, C.POINTER(AdaptiveResult)]
     L.frw_write_results.argtypes = [vp, vp, i64, C.c_char_p]
     L.frw_batch_create.argtypes = [vp, i32, vp, i32, C.POINTER(vp)]
     L.frw_batch_counts_size.restype = i64
@@ -193,6 +203,14 @@ def green_thresholds():
     t = np.zeros(NSAMPLE, dtype=np.uint32)
     _check(lib().frw_green_thresholds(t.ctypes.data))
     return t
+
+
+def weight_tables():
+    """(gx, gy, pw, qx, qy, tx, ty): the weighted estimator's first-hop tables."""
+    wt = np.zeros(5 * NSAMPLE, dtype=np.float64)
+    it = np.zeros(2 * NSAMPLE, dtype=np.uint32)
+    lib().frw_weight_tables(wt.ctypes.data, it.ctypes.data)
+    return tuple(wt.reshape(5, NSAMPLE)) + tuple(it.reshape(2, NSAMPLE))
 
 
 class Layout:
@@ -349,19 +367,20 @@ class Tree:
 
     def walk_weighted_async(self, master, n_walks, counts, hops, wsum, wsq, w_out=None, hit_out=None,
                             hops_out=None, seed=1, eps=1.0, job=0, first_walk=0, max_hops=0, block=0,
-                            grid=0, stream=None):
+                            grid=0, stream=None, flags=0):
         """Weighted (physical) estimator into device tensors: wsum[j] / n_walks
-        estimates the capacitance C_master,j (eps per unit length)."""
+        estimates the capacitance C_master,j (eps per unit length).  flags:
+        WEIGHT_IMPORTANCE draws the first panel proportional to |n . grad G|."""
         p = WalkParams(master=master, mode=MODE_WEIGHTED, n_walks=n_walks, first_walk=first_walk, seed=seed,
                        job=job, block=block, grid=grid, reserved=0, max_hops=max_hops)
         o = WalkOutputs(counts.data_ptr(), hops.data_ptr(),
                         hit_out.data_ptr() if hit_out is not None else None,
                         hops_out.data_ptr() if hops_out is not None else None)
-        _check(lib().frw_walk_weighted_async(self.h, C.byref(p), C.byref(o), eps, wsum.data_ptr(),
+        _check(lib().frw_walk_weighted_async(self.h, C.byref(p), C.byref(o), eps, flags, wsum.data_ptr(),
                                              wsq.data_ptr(), w_out.data_ptr() if w_out is not None else None,
                                              stream))
 
-    def walk_weighted(self, master, n_walks, seed=1, eps=1.0, job=0, first_walk=0, max_hops=0):
+    def walk_weighted(self, master, n_walks, seed=1, eps=1.0, job=0, first_walk=0, max_hops=0, flags=0):
         """Synchronous weighted walks: (counts, wsum, wsq, hops) on the host."""
         p = WalkParams(master=master, mode=MODE_WEIGHTED, n_walks=n_walks, first_walk=first_walk, seed=seed,
                        job=job, block=0, grid=0, reserved=0, max_hops=max_hops)
@@ -370,9 +389,24 @@ class Tree:
         wsum = np.zeros(nb, dtype=np.float64)
         wsq = np.zeros(nb, dtype=np.float64)
         hops = C.c_int64()
-        _check(lib().frw_walk_weighted(self.h, C.byref(p), eps, counts.ctypes.data, wsum.ctypes.data,
+        _check(lib().frw_walk_weighted(self.h, C.byref(p), eps, flags, counts.ctypes.data, wsum.ctypes.data,
                                        wsq.ctypes.data, C.byref(hops)))
         return counts, wsum, wsq, hops.value
+
+    def walk_adaptive(self, master, rel_err=0.01, mode=MODE_EXACT, first_round=0, max_walks=0, seed=1,
+                      eps=1.0, flags=0, first_walk=0, job=0, max_hops=0):
+        """Walks in rounds until the 3-sigma relative error of f (hit modes)
+        or of C_mm (MODE_WEIGHTED) reaches rel_err: (result, counts, wsum, wsq)."""
+        p = WalkParams(master=master, mode=mode, n_walks=first_round, first_walk=first_walk, seed=seed,
+                       job=job, block=0, grid=0, reserved=0, max_hops=max_hops)
+        nb = self.info().n_rects + 2
+        counts = np.zeros(nb, dtype=np.int64)
+        wsum = np.zeros(nb, dtype=np.float64)
+        wsq = np.zeros(nb, dtype=np.float64)
+        r = AdaptiveResult()
+        _check(lib().frw_walk_adaptive(self.h, C.byref(p), rel_err, max_walks, eps, flags, counts.ctypes.data,
+                                       wsum.ctypes.data, wsq.ctypes.data, C.byref(r)))
+        return r, counts, wsum, wsq
 
     def __del__(self):
         try:

@@ -334,12 +334,50 @@ int frw_batch_autotune_learned(frw_batch* b, const frw_batch_params* base, const
  * counts/hops as for frw_walk_async; wsum, wsq: device double[n_rects + 2],
  * accumulated (caller zeroes); w_out: optional device double[n_walks].
  * Sums are compensated per lane and added across lanes with FP64 atomics
- * (their order, hence the last bits, varies run to run). */
+ * (their order, hence the last bits, varies run to run).
+ * flags: FRW_WEIGHT_IMPORTANCE draws the first panel with probability q_k
+ * proportional to |n . g_k| instead of p_k (Yu et al. 2013, importance
+ * sampling of the first hop); the weight divides by q_k, so every weight of
+ * one side of G has the same magnitude and the estimate's variance drops.
+ * The expectation is unchanged. */
+/* The weighted estimator's first-hop tables, host side (no device needed):
+ * wt = gx | gy | pw | qx | qy (5 x 200 doubles: Green-gradient panel weights
+ * of the unit square, draw probabilities of the Green CDF and of the two
+ * importance tables), ithr = thresholds drawing panels proportional to |gx|
+ * then |gy| (2 x 200 u32). */
+void frw_weight_tables(double wt[1000], uint32_t ithr[400]);
+enum { FRW_WEIGHT_IMPORTANCE = 1 };
 int frw_walk_weighted_async(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o, double eps,
-                            double* wsum, double* wsq, double* w_out, void* stream);
+                            uint32_t flags, double* wsum, double* wsq, double* w_out, void* stream);
 /* Synchronous form into host arrays (counts int64[n+2], wsum/wsq double[n+2]). */
-int frw_walk_weighted(const frw_tree* t, const frw_walk_params* p, double eps, int64_t* counts_host,
-                      double* wsum_host, double* wsq_host, int64_t* hops);
+int frw_walk_weighted(const frw_tree* t, const frw_walk_params* p, double eps, uint32_t flags,
+                      int64_t* counts_host, double* wsum_host, double* wsq_host, int64_t* hops);
+
+/* Adaptive stopping (Yu et al. 2013 eq. 32, no reference counterpart: the
+ * reference runs a fixed ITER_NUM, randomWalk/CPU/getAvgPathLen.cpp:86-111):
+ * walks in rounds until the controlled estimate reaches a 3-sigma relative
+ * error of rel_err, each round sized n_rem = n (err^2 / rel_err^2 - 1) from
+ * the walks so far (at least 1/16 of them, at most what max_walks leaves).
+ * The controlled estimate is f = #(hit != master) / n with
+ * sigma^2 = f (1 - f) / n for FRW_MODE_EXACT, FRW_MODE_BISECT and
+ * FRW_MODE_REFCOMPAT, and the master's self term C_mm = wsum[m] / n with the
+ * sample variance of the weights for FRW_MODE_WEIGHTED (eps, wflags as for
+ * frw_walk_weighted).  p->n_walks is the first round (0: 65 536); rounds take
+ * consecutive walk ids from p->first_walk, so the tallies equal one launch of
+ * the total.  Outputs: counts_host int64[n+2] (required), wsum_host and
+ * wsq_host double[n+2] (weighted mode, may be NULL), *r.  Synchronous. */
+typedef struct {
+    int64_t n_walks;        /* walks run */
+    int64_t hops;
+    int32_t rounds;         /* launches */
+    int32_t converged;      /* 1: the goal was met within max_walks */
+    double value;           /* f or C_mm */
+    double rel_err_3sigma;  /* 3-sigma relative error of value */
+    double wall_ms;         /* host wall time of the rounds (launches, waits, reads) */
+} frw_adaptive_result;
+int frw_walk_adaptive(const frw_tree* t, const frw_walk_params* p, double rel_err, int64_t max_walks, double eps,
+                      uint32_t wflags, int64_t* counts_host, double* wsum_host, double* wsq_host,
+                      frw_adaptive_result* r);
 
 /* Reference-compatible mode (FRW_MODE_REFCOMPAT walks on a tree built with
  * FRW_TREE_REFCOMPAT): the reference CPU tree with its insertion behaviour

@@ -138,13 +138,15 @@ void    orc_green_grad(double gx[ORC_NSAMPLE], double gy[ORC_NSAMPLE]);
 void    orc_panel_centre(double x0, double y0, double a, int k, double* px, double* py);
 void    orc_gauss_point(const orc_box* G, uint64_t seed, uint32_t walk, uint32_t job, double* rx, double* ry,
                         double* nx, double* ny, double* L);
+void    orc_importance_table(const double g[ORC_NSAMPLE], uint32_t T[ORC_NSAMPLE], double q[ORC_NSAMPLE]);
 int     orc_walk_weighted_one(const orc_index* ix, const orc_box* G, const uint32_t T[ORC_NSAMPLE],
                               const double gx[ORC_NSAMPLE], const double gy[ORC_NSAMPLE],
-                              const double pw[ORC_NSAMPLE], uint64_t seed, uint32_t job, uint32_t walk,
+                              const double pw[ORC_NSAMPLE], const uint32_t* tx, const uint32_t* ty,
+                              const double* qx, const double* qy, uint64_t seed, uint32_t job, uint32_t walk,
                               int64_t max_hops, double eps, int64_t* hops, double* weight);
 int64_t orc_walk_weighted_many(const orc_index* ix, int master, uint64_t seed, uint32_t job,
                                int64_t first_walk, int64_t n_walks, int64_t max_hops, double eps,
-                               int32_t* hit_out, int32_t* hops_out, double* w_out);
+                               int importance, int32_t* hit_out, int32_t* hops_out, double* w_out);
 
 #ifdef __cplusplus
 }

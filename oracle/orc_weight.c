@@ -94,6 +94,25 @@ void orc_green_grad(double gx[ORC_NSAMPLE], double gy[ORC_NSAMPLE])
     }
 }
 
+/* First-hop importance table (Yu et al. 2013, importance sampling of the
+ * first hop): panel k drawn with probability q_k proportional to |g[k]|, as
+ * u32 thresholds T[k] = floor(sum_{j<k} |g_j| / S * 2^32); q_k is the exact
+ * draw probability (threshold difference). */
+void orc_importance_table(const double g[ORC_NSAMPLE], uint32_t T[ORC_NSAMPLE], double q[ORC_NSAMPLE])
+{
+    double S = 0.0;
+    for (int k = 0; k < ORC_NSAMPLE; ++k) S += fabs(g[k]);
+    double cum = 0.0;
+    for (int k = 0; k < ORC_NSAMPLE; ++k) {
+        T[k] = (uint32_t)floor(cum / S * 4294967296.0);
+        cum += fabs(g[k]);
+    }
+    for (int k = 0; k < ORC_NSAMPLE; ++k) {
+        const double hi = k + 1 < ORC_NSAMPLE ? (double)T[k + 1] : 4294967296.0;
+        q[k] = (hi - (double)T[k]) / 4294967296.0;
+    }
+}
+
 /* Centre of panel k on the square (x0, y0) - (x0 + a, y0 + a), clockwise. */
 void orc_panel_centre(double x0, double y0, double a, int k, double* px, double* py)
 {
@@ -130,9 +149,13 @@ void orc_gauss_point(const orc_box* G, uint64_t seed, uint32_t walk, uint32_t jo
     }
 }
 
+/* One weighted walk.  With importance sampling (tx/ty/qx/qy non-NULL) the
+ * first panel comes from the |gx| table when the normal is along x, else from
+ * the |gy| table, and the weight divides by that table's probability. */
 int orc_walk_weighted_one(const orc_index* ix, const orc_box* G, const uint32_t T[ORC_NSAMPLE],
                           const double gx[ORC_NSAMPLE], const double gy[ORC_NSAMPLE],
-                          const double pw[ORC_NSAMPLE], uint64_t seed, uint32_t job, uint32_t walk,
+                          const double pw[ORC_NSAMPLE], const uint32_t* tx, const uint32_t* ty,
+                          const double* qx, const double* qy, uint64_t seed, uint32_t job, uint32_t walk,
                           int64_t max_hops, double eps, int64_t* hops, double* weight)
 {
     const orc_box B = orc_index_bounds(ix);
@@ -144,10 +167,19 @@ int orc_walk_weighted_one(const orc_index* ix, const orc_box* G, const uint32_t 
     *weight = 0.0;
     if (!(h > 0.0)) { *hops = 0; return ORC_HIT_STUCK; }
     const uint32_t u0 = orc_walk_u32(seed, walk, job, 0u);
-    const int k = orc_sample_index(T, u0);
+    int k;
+    double prob;
+    if (tx) {
+        const int along_x = nx != 0.0;
+        k = orc_sample_index(along_x ? tx : ty, u0);
+        prob = along_x ? qx[k] : qy[k];
+    } else {
+        k = orc_sample_index(T, u0);
+        prob = pw[k];
+    }
     double px, py;
     orc_panel_centre(rx - h, ry - h, 2.0 * h, k, &px, &py);
-    *weight = -eps * L * (nx * gx[k] + ny * gy[k]) / (pw[k] * 2.0 * h);
+    *weight = -eps * L * (nx * gx[k] + ny * gy[k]) / (prob * 2.0 * h);
     int64_t n = 0;
     orc_box sq;
     for (;;) {
@@ -164,7 +196,7 @@ int orc_walk_weighted_one(const orc_index* ix, const orc_box* G, const uint32_t 
 
 int64_t orc_walk_weighted_many(const orc_index* ix, int master, uint64_t seed, uint32_t job,
                                int64_t first_walk, int64_t n_walks, int64_t max_hops, double eps,
-                               int32_t* hit_out, int32_t* hops_out, double* w_out)
+                               int importance, int32_t* hit_out, int32_t* hops_out, double* w_out)
 {
     uint32_t T[ORC_NSAMPLE];
     orc_green_thresholds(T);
@@ -176,13 +208,18 @@ int64_t orc_walk_weighted_many(const orc_index* ix, int master, uint64_t seed, u
         const double hi = k + 1 < ORC_NSAMPLE ? (double)T[k + 1] : 4294967296.0;
         pw[k] = (hi - (double)T[k]) / 4294967296.0;
     }
+    uint32_t tx[ORC_NSAMPLE], ty[ORC_NSAMPLE];
+    double qx[ORC_NSAMPLE], qy[ORC_NSAMPLE];
+    orc_importance_table(gx, tx, qx);
+    orc_importance_table(gy, ty, qy);
     const orc_box G = orc_gauss_from(ix, master);
     int64_t total = 0;
     for (int64_t i = 0; i < n_walks; ++i) {
         int64_t h = 0;
         double w = 0.0;
-        const int hit = orc_walk_weighted_one(ix, &G, T, gx, gy, pw, seed, job, (uint32_t)(first_walk + i),
-                                              max_hops, eps, &h, &w);
+        const int hit = orc_walk_weighted_one(ix, &G, T, gx, gy, pw, importance ? tx : NULL, importance ? ty : NULL,
+                                              importance ? qx : NULL, importance ? qy : NULL, seed, job,
+                                              (uint32_t)(first_walk + i), max_hops, eps, &h, &w);
         if (hit_out) hit_out[i] = hit;
         if (hops_out) hops_out[i] = (int32_t)h;
         if (w_out) w_out[i] = w;

@@ -90,7 +90,8 @@ def oracle():
     L.orc_green_grad.argtypes = [_D, _D]
     L.orc_walk_weighted_many.restype = C.c_int64
     L.orc_walk_weighted_many.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_uint32, C.c_int64, C.c_int64,
-                                         C.c_int64, C.c_double, _I32, _I32, _D]
+                                         C.c_int64, C.c_double, C.c_int, _I32, _I32, _D]
+    L.orc_importance_table.argtypes = [_D, _U32, _D]
     L.orc_bf16.restype = C.c_float
     L.orc_bf16.argtypes = [C.c_float]
     L.orc_mlp_set_rounding.argtypes = [C.c_int]
@@ -228,13 +229,15 @@ class Index:
                                 nthreads)
         return q1, q2, q3[:n_q3]
 
-    def walk_weighted(self, master, n_walks, seed=1, job=0, first_walk=0, max_hops=10**6, eps=1.0):
-        """(hit[n], hops[n], weight[n]) of weighted walks (Gaussian-surface start)."""
+    def walk_weighted(self, master, n_walks, seed=1, job=0, first_walk=0, max_hops=10**6, eps=1.0,
+                      importance=False):
+        """(hit[n], hops[n], weight[n]) of weighted walks (Gaussian-surface start;
+        importance: first panel proportional to |n . grad G|)."""
         hit = np.zeros(n_walks, dtype=np.int32)
         hops = np.zeros(n_walks, dtype=np.int32)
         w = np.zeros(n_walks, dtype=np.float64)
         self.L.orc_walk_weighted_many(self.h, master, seed, job, first_walk, n_walks, max_hops, eps,
-                                      _ptr(hit, _I32), _ptr(hops, _I32), _ptr(w, _D))
+                                      int(importance), _ptr(hit, _I32), _ptr(hops, _I32), _ptr(w, _D))
         return hit, hops, w
 
     def nearest(self, px, py):
@@ -399,6 +402,15 @@ def mlp_backward(params, X, t, rounding=True):
 
 def square_density(s, u, v):
     return oracle().orc_square_density(s, u, v)
+
+
+def importance_table(g):
+    """(thresholds u32[200], probabilities[200]) drawing panel k proportional to |g[k]|."""
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    T = np.zeros(NSAMPLE, dtype=np.uint32)
+    q = np.zeros(NSAMPLE, dtype=np.float64)
+    oracle().orc_importance_table(_ptr(g, _D), _ptr(T, _U32), _ptr(q, _D))
+    return T, q
 
 
 def green_grad():

@@ -18,7 +18,7 @@ from gpu_common import load_fixture, oracle_index, torch_dev
 pytestmark = pytest.mark.gpu
 
 
-def gpu_weighted(tree, master, n, seed=1, first=0, job=0, max_hops=0, eps=1.0):
+def gpu_weighted(tree, master, n, seed=1, first=0, job=0, max_hops=0, eps=1.0, flags=0):
     torch = torch_dev()
     nr = tree.info().n_rects
     counts = torch.zeros(nr + 2, dtype=torch.int64, device="cuda")
@@ -29,7 +29,7 @@ def gpu_weighted(tree, master, n, seed=1, first=0, job=0, max_hops=0, eps=1.0):
     hit = torch.full((n,), -7, dtype=torch.int32, device="cuda")
     hp = torch.full((n,), -7, dtype=torch.int32, device="cuda")
     tree.walk_weighted_async(master, n, counts, hops, wsum, wsq, w_out=w_out, hit_out=hit, hops_out=hp,
-                             seed=seed, eps=eps, first_walk=first, job=job, max_hops=max_hops)
+                             seed=seed, eps=eps, first_walk=first, job=job, max_hops=max_hops, flags=flags)
     torch.cuda.synchronize()
     return (counts.cpu().numpy(), int(hops.item()), wsum.cpu().numpy(), wsq.cpu().numpy(),
             w_out.cpu().numpy(), hit.cpu().numpy(), hp.cpu().numpy())
@@ -55,6 +55,38 @@ def test_weighted_walks_replay_oracle(name, master):
     scale = np.abs(ow).sum() + 1e-300
     assert np.abs(wsum - want).max() <= 1e-12 * scale
     assert np.abs(wsq - want_q).max() <= 1e-12 * (ow * ow).sum()
+
+
+@pytest.mark.parametrize("name,master", FIXTURES[:4])
+def test_importance_walks_replay_oracle(name, master):
+    """Importance-sampled first hop: same panels, weights and walks as the oracle."""
+    lay, rects, bounds = load_fixture(name)
+    tree = frw.Tree(lay)
+    n = 3000
+    counts, hops, wsum, wsq, w, hit, hp = gpu_weighted(tree, master, n, seed=8, flags=frw.WEIGHT_IMPORTANCE)
+    ohit, ohp, ow = oracle_index(rects, bounds, O.REFTREE_DFS).walk_weighted(master, n, seed=8, importance=True)
+    assert np.array_equal(hit, ohit), np.nonzero(hit != ohit)[0][:10]
+    assert np.array_equal(hp, ohp)
+    np.testing.assert_allclose(w, ow, rtol=1e-13, atol=0)
+    assert hops == ohp.sum()
+
+
+def test_importance_parallel_plates():
+    """Same coupling as the plain estimator within the Monte Carlo error,
+    zero-sum weights, smaller variance of the self term."""
+    lay = frw.Layout.from_rects(_plates())
+    tree = frw.Tree(lay)
+    n = 2_000_000
+    _, ws_p, wq_p, _ = tree.walk_weighted(0, n, seed=11)
+    _, ws_i, wq_i, _ = tree.walk_weighted(0, n, seed=12, flags=frw.WEIGHT_IMPORTANCE)
+    cp, ci = ws_p / n, ws_i / n
+    vp, vi = wq_p / n - cp * cp, wq_i / n - ci * ci
+    for j in (0, 1):
+        assert abs(cp[j] - ci[j]) < 4 * np.sqrt((vp[j] + vi[j]) / n), (j, cp[j], ci[j])
+    assert -25.0 < ci[1] < -20.0, ci
+    tot = ws_i.sum() / n
+    assert abs(tot) < 4 * np.sqrt(max(wq_i.sum() / n - tot * tot, 0) / n)
+    assert vi[0] < 0.9 * vp[0], (vi[0], vp[0])
 
 
 def test_weighted_seed_job_offset_and_max_hops():
@@ -112,3 +144,5 @@ def test_weighted_rejects_bad_arguments():
         tree.walk_weighted(0, 100, eps=0.0)
     with pytest.raises(frw.FrwError):
         tree.walk_weighted(len(rects), 100)
+    with pytest.raises(frw.FrwError):
+        tree.walk_weighted(0, 100, flags=8)

@@ -78,6 +78,19 @@ def test_green_table_matches_oracle(oracle_lib):
     assert np.array_equal(frw.green_thresholds(), O.green_thresholds())
 
 
+def test_weight_tables_match_oracle(oracle_lib):
+    """The weighted estimator's first-hop tables equal the oracle's to the bit."""
+    gx, gy, pw, qx, qy, tx, ty = frw.weight_tables()
+    ogx, ogy = O.green_grad()
+    assert np.array_equal(gx, ogx) and np.array_equal(gy, ogy)
+    T = O.green_thresholds().astype(np.float64)
+    assert np.array_equal(pw, (np.append(T[1:], 2.0 ** 32) - T) / 2.0 ** 32)
+    otx, oqx = O.importance_table(ogx)
+    oty, oqy = O.importance_table(ogy)
+    assert np.array_equal(tx, otx) and np.array_equal(ty, oty)
+    assert np.array_equal(qx, oqx) and np.array_equal(qy, oqy)
+
+
 def _overlaps(r):
     """Closed-interval overlap (touching counts) by sweep over x."""
     order = np.argsort(r[:, 0], kind="stable")
