@@ -252,6 +252,93 @@ def traffic_bytes(frw, tree, mlp, mode, master, walks, alpha, first):
     return b / max(h, 1), nodes / max(h, 1), rects / max(h, 1)
 
 
+def weighted_run(frw, torch, tree, master, nw, stream, flags, build_ms):
+    """Weighted estimator over nw walks (one warm-up launch on disjoint walk
+    ids first): the master's self term, its 3-sigma error and time to 1 %."""
+    nr = tree.info().n_rects
+    wc = torch.zeros(nr + 2, dtype=torch.int64, device="cuda")
+    wh = torch.zeros(1, dtype=torch.int64, device="cuda")
+    wsum = torch.zeros(nr + 2, dtype=torch.float64, device="cuda")
+    wsq = torch.zeros(nr + 2, dtype=torch.float64, device="cuda")
+    tree.walk_weighted_async(master, nw, wc, wh, wsum, wsq, seed=1, first_walk=1 << 31,
+                             stream=stream.cuda_stream, flags=flags)  # warm-up launch
+    torch.cuda.synchronize()
+    for t_ in (wc, wh, wsum, wsq):
+        t_.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tree.walk_weighted_async(master, nw, wc, wh, wsum, wsq, seed=1, first_walk=0, stream=stream.cuda_stream,
+                             flags=flags)
+    torch.cuda.synchronize()
+    wt = time.perf_counter() - t0
+    ws, wq = wsum.cpu().numpy(), wsq.cpu().numpy()
+    c_self = ws[master] / nw
+    var = max(wq[master] / nw - c_self * c_self, 0.0)
+    n1w = math.ceil(var * (300.0 / c_self) ** 2) if c_self > 0 else float("inf")
+    return {
+        "walks": nw,
+        "walks_per_s": nw / wt,
+        "walk_steps_per_s": int(wh.item()) / wt,
+        "c_self": c_self,
+        "c_self_rel_3sigma": 3.0 * math.sqrt(var / nw) / c_self if c_self > 0 else None,
+        "var_per_walk": var,
+        "c_escape": ws[nr] / nw,
+        "sum_over_bins": float(ws.sum() / nw),
+        "walks_for_1pct": n1w,
+        "time_to_1pct_s": build_ms / 1e3 + n1w / (nw / wt),
+        "units": "eps per unit length",
+    }
+
+
+def adaptive_runs(frw, tree, master, build_ms, mode):
+    """Time to 1 % (3 sigma) measured by adaptive rounds: the hit fraction f
+    in the bench's walk mode (exact unless learned was asked for) and the
+    master's self-capacitance with the importance-sampled weighted walks."""
+    res = {"goal_rel_3sigma": 0.01, "build_ms": build_ms}
+    hit_mode = frw.MODE_BISECT if mode == "bisect" else frw.MODE_EXACT
+    runs = [("fraction_other", dict(mode=hit_mode)),
+            ("c_self", dict(mode=frw.MODE_WEIGHTED, flags=frw.WEIGHT_IMPORTANCE))]
+    for name, kw in runs:
+        tree.walk_adaptive(master, rel_err=0.01, first_round=65536, seed=9, first_walk=3 << 30,
+                           max_walks=1 << 24, **kw)  # warm-up (code paths, allocations)
+        r, _, _, _ = tree.walk_adaptive(master, rel_err=0.01, first_round=65536, seed=1,
+                                        max_walks=(1 << 31) - 1, **kw)
+        res[name] = {"value": r.value, "rel_err_3sigma": r.rel_err_3sigma, "converged": bool(r.converged),
+                     "n_walks": r.n_walks, "rounds": r.rounds, "hops": r.hops, "wall_ms": r.wall_ms,
+                     "time_to_1pct_s": build_ms / 1e3 + r.wall_ms / 1e3}
+    return res
+
+
+def refcompat_run(frw, torch, lay, master, stream):
+    """The reference-compatible mode (the reference CPU tree and queries, one
+    lane per walk): the same walks as the reference's own CPU code."""
+    t0 = time.perf_counter()
+    rt = frw.Tree(lay, ref_compat=True)
+    build_s = time.perf_counter() - t0
+    nr = rt.info().n_rects
+    counts = torch.zeros(nr + 2, dtype=torch.int64, device="cuda")
+    hops = torch.zeros(1, dtype=torch.int64, device="cuda")
+    n = 1 << 14
+    while True:
+        counts.zero_()
+        hops.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rt.walk_async(master, n, counts, hops, seed=1, mode=frw.MODE_REFCOMPAT, first_walk=0,
+                      stream=stream.cuda_stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if dt > 0.5 or n >= 1 << 24:
+            break
+        n *= 4
+    c = counts.cpu().numpy()
+    return {"walks": n, "walks_per_s": n / dt, "walk_steps_per_s": int(hops.item()) / dt,
+            "fraction_other": (n - int(c[master])) / n, "host_tree_build_s": build_s,
+            "tree_nodes": rt.refcompat_info()[0],
+            "note": "the reference CPU tree (insertion behaviour, single-path queries, literal bisection) on the "
+                    "device; walk for walk the reference CPU FRW's walks for the same random stream"}
+
+
 def main():
     args = parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -358,40 +445,24 @@ def main():
             "avg_launch_ms": avg_ms,
         },
     }
-    # the physical capacitance row of the master (weighted estimator), same layout
+    # the physical capacitance row of the master (weighted estimator), same
+    # layout: plain first hop, then the importance-sampled one
     nw = min(walks, 10_000_000)
-    wc = torch.zeros(nr + 2, dtype=torch.int64, device="cuda")
-    wh = torch.zeros(1, dtype=torch.int64, device="cuda")
-    wsum = torch.zeros(nr + 2, dtype=torch.float64, device="cuda")
-    wsq = torch.zeros(nr + 2, dtype=torch.float64, device="cuda")
-    tree.walk_weighted_async(master, nw, wc, wh, wsum, wsq, seed=1, first_walk=1 << 31,
-                             stream=stream.cuda_stream)  # warm-up launch
-    torch.cuda.synchronize()
-    for t_ in (wc, wh, wsum, wsq):
-        t_.zero_()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    tree.walk_weighted_async(master, nw, wc, wh, wsum, wsq, seed=1, first_walk=0, stream=stream.cuda_stream)
-    torch.cuda.synchronize()
-    wt = time.perf_counter() - t0
-    ws, wq = wsum.cpu().numpy(), wsq.cpu().numpy()
-    c_self = ws[master] / nw
-    var = max(wq[master] / nw - c_self * c_self, 0.0)
-    n1w = math.ceil(var * (300.0 / c_self) ** 2) if c_self > 0 else float("inf")
-    out["weighted"] = {
-        "walks": nw,
-        "walks_per_s": nw / wt,
-        "walk_steps_per_s": int(wh.item()) / wt,
-        "c_self": c_self,
-        "c_self_rel_3sigma": 3.0 * math.sqrt(var / nw) / c_self if c_self > 0 else None,
-        "c_escape": ws[nr] / nw,
-        "sum_over_bins": float(ws.sum() / nw),
-        "walks_for_1pct": n1w,
-        "time_to_1pct_s": info.build_ms / 1e3 + n1w / (nw / wt),
-        "units": "eps per unit length",
-    }
-    log(f"weighted: C_self {c_self:.4g} (3 sigma {out['weighted']['c_self_rel_3sigma']:.2%}), "
-        f"{nw / wt:.3e} walks/s")
+    out["weighted"] = weighted_run(frw, torch, tree, master, nw, stream, 0, info.build_ms)
+    out["weighted"]["importance"] = weighted_run(frw, torch, tree, master, nw, stream,
+                                                 frw.WEIGHT_IMPORTANCE, info.build_ms)
+    for k, v in (("plain", out["weighted"]), ("importance", out["weighted"]["importance"])):
+        log(f"weighted ({k}): C_self {v['c_self']:.4g} (3 sigma {v['c_self_rel_3sigma']:.2%}), "
+            f"{v['walks_per_s']:.3e} walks/s")
+    # time to 1 % measured end to end: adaptive rounds (eq. 32) on the warm
+    # tree, plus the warm build
+    out["adaptive"] = adaptive_runs(frw, tree, master, info.build_ms, mode)
+    for k, v in out["adaptive"].items():
+        if isinstance(v, dict):
+            log(f"adaptive {k}: {v['n_walks']} walks in {v['rounds']} rounds, {v['time_to_1pct_s'] * 1e3:.2f} ms")
+    if nr <= 200_000:
+        out["refcompat"] = refcompat_run(frw, torch, lay, master, stream)
+        log(f"reference-compatible mode: {out['refcompat']['walks_per_s']:.3e} walks/s")
     if mode == "learned":
         out["learned"] = dict(train_info, alpha=args.alpha,
                               accept_rate=r["accepted"] / max(1, r["accepted"] + r["fallback"]))
@@ -424,6 +495,10 @@ def main():
         cb = cpu_baseline(lay.rects(), lay.bounds(), master, args.cpu_seconds)
         cb["time_to_1pct_s"] = n1 / cb.pop("walks_per_s")
         out["cpu_baseline"] = cb
+        if "refcompat" in out and cb["kind"] == "reference":
+            # the same algorithm on both sides: the reference's own CPU tree
+            # code against its device restatement
+            out["refcompat"]["vs_reference_cpu_steps_per_s"] = out["refcompat"]["walk_steps_per_s"] / cb["value"]
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
