@@ -133,6 +133,17 @@ int frw_layout_bounds(const frw_layout* l, int32_t out[4])
     return FRW_OK;
 }
 
+int frw_layout_set_bounds(frw_layout* l, const int32_t b[4])
+{
+    if (!l || !b) return set_error(FRW_E_INVALID, "null argument");
+    if (b[0] > b[2] || b[1] > b[3]) return set_error(FRW_E_RANGE, "bounds are not a box");
+    for (const int4& r : l->rects)
+        if (r.x < b[0] || r.y < b[1] || r.z > b[2] || r.w > b[3])
+            return set_error(FRW_E_RANGE, "a rect lies outside the bounds");
+    l->bounds = make_int4(b[0], b[1], b[2], b[3]);
+    return FRW_OK;
+}
+
 void frw_layout_free(frw_layout* l) { delete l; }
 
 int frw_green_thresholds(uint32_t out[FRW_NSAMPLE])
@@ -254,21 +265,41 @@ static int check_err_flag(const frw_tree* t, hipStream_t s)
     return FRW_OK;
 }
 
-int frw_query_refcompat(const frw_tree* t, const double* pts, const double* boxes, int64_t n, uint8_t* q1,
-                        int32_t* q2, double* q3, void* stream)
+static int check_walk_params(const frw_tree* t, const frw_walk_params* p);
+
+int frw_query_refcompat(const frw_tree* t, const double* pts, const double* boxes, int64_t n,
+                        const int32_t* ignore, uint8_t* q1, int32_t* q2, double* q3, void* stream)
 {
     if (!t || n < 0 || (n > 0 && ((q1 && !boxes) || ((q2 || q3) && !pts))))
         return set_error(FRW_E_INVALID, "bad query arguments");
     FRW_GUARD_BEGIN
-    return launch_ref_queries(t, pts, boxes, n, q1, q2, q3, (hipStream_t)stream);
+    return launch_ref_queries(t, pts, boxes, n, ignore, q1, q2, q3, (hipStream_t)stream);
     FRW_GUARD_END
 }
 
-int frw_refcompat_gauss_surface(const frw_tree* t, int32_t master, double out[4])
+int frw_refcompat_gauss_surface(const frw_tree* t, const int32_t rect[4], double factor, double out[4])
 {
-    if (!t || !out || master < 0 || master >= t->n_rects) return set_error(FRW_E_INVALID, "bad arguments");
+    if (!t || !rect || !out || !std::isfinite(factor))
+        return set_error(FRW_E_INVALID, "bad arguments");
     FRW_GUARD_BEGIN
-    return refcompat_gauss(t, master, out);
+    return refcompat_gauss(t, make_int4(rect[0], rect[1], rect[2], rect[3]), factor, out);
+    FRW_GUARD_END
+}
+
+int frw_refcompat_walk_from(const frw_tree* t, const double surface[4], const frw_walk_params* p,
+                            const frw_walk_outputs* o, void* stream)
+{
+    if (!t || !surface || !p) return set_error(FRW_E_INVALID, "null argument");
+    frw_walk_params q = *p;
+    q.master = 0;
+    q.mode = FRW_MODE_REFCOMPAT;
+    int rc = check_walk_params(t, &q);
+    if (rc != FRW_OK) return rc;
+    if (!o || !o->counts || !o->hops) return set_error(FRW_E_INVALID, "walk outputs need counts and hops");
+    for (int k = 0; k < 4; ++k)
+        if (!std::isfinite(surface[k])) return set_error(FRW_E_INVALID, "start square is not finite");
+    FRW_GUARD_BEGIN
+    return launch_ref_walk(t, &q, o, (hipStream_t)stream, surface);
     FRW_GUARD_END
 }
 

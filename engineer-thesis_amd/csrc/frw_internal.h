@@ -157,10 +157,12 @@ int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outp
 int build_refcompat(frw_tree* t);
 void free_refcompat(frw_tree* t);
 int refcompat_info(const frw_tree* t, int64_t out[2]);
-int refcompat_gauss(const frw_tree* t, int master, double out[4]);
-int launch_ref_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o, hipStream_t s);
-int launch_ref_queries(const frw_tree* t, const double* pts, const double* boxes, int64_t n, uint8_t* q1,
-                       int32_t* q2, double* q3, hipStream_t s);
+int refcompat_gauss(const frw_tree* t, int4 rect, double factor, double out[4]);
+// surface: start square (host double[4]); NULL = the master's Gaussian surface
+int launch_ref_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o, hipStream_t s,
+                    const double* surface = nullptr);
+int launch_ref_queries(const frw_tree* t, const double* pts, const double* boxes, int64_t n, const int32_t* ignore,
+                       uint8_t* q1, int32_t* q2, double* q3, hipStream_t s);
 
 // learned proposal views (frw_mlp.hip)
 MlpPyramid mlp_pyramid(const frw_mlp* m);

@@ -109,17 +109,6 @@ struct RefCompat {
 
 namespace {
 
-RefView host_view(const frw_tree* t)
-{
-    RefView v;
-    v.nodes = t->ref->h_nodes.data();
-    v.objs = t->ref->h_objs.data();
-    v.rects = t->host_rects.data();
-    v.bounds = t->bounds;
-    v.dist0 = t->dist0;
-    return v;
-}
-
 RefView device_view(const frw_tree* t)
 {
     RefView v;
@@ -132,7 +121,8 @@ RefView device_view(const frw_tree* t)
 }
 
 __global__ void __launch_bounds__(256) ref_walk_kernel(RefView v, const uint32_t* __restrict__ thresholds,
-                                                       DBox surf, uint64_t seed, uint32_t job,
+                                                       DBox start, const double* __restrict__ surface,
+                                                       uint64_t seed, uint32_t job,
                                                        uint32_t first_walk, int64_t n_walks, uint32_t max_hops,
                                                        int32_t n_rects, unsigned long long* counts,
                                                        unsigned long long* hops, int32_t* hit_out,
@@ -141,6 +131,8 @@ __global__ void __launch_bounds__(256) ref_walk_kernel(RefView v, const uint32_t
     __shared__ uint32_t sT[kNSample];
     for (int i = threadIdx.x; i < kNSample; i += blockDim.x) sT[i] = thresholds[i];
     __syncthreads();
+    // the start square: computed on the device (surface) or passed by value
+    const DBox surf = surface ? DBox{surface[0], surface[1], surface[2], surface[3]} : start;
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     unsigned long long my_hops = 0;
     for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n_walks;
@@ -184,13 +176,26 @@ __global__ void __launch_bounds__(256) ref_walk_kernel(RefView v, const uint32_t
     if (my_hops) atomicAdd(hops, my_hops);
 }
 
+// Tree::creatGaussianSurfFrom (quadTree.cpp:351-364), one thread.
+__global__ void ref_gauss_kernel(RefView v, int4 rect, double factor, double* out)
+{
+    const double fx = ref_gauss_factor(v, rect, factor, false);
+    const double fy = ref_gauss_factor(v, rect, factor, true);
+    const DBox s = gauss_surface(rect, fx, fy);
+    out[0] = s.x0;
+    out[1] = s.y0;
+    out[2] = s.x1;
+    out[3] = s.y1;
+}
+
 __global__ void ref_query_kernel(RefView v, const double* __restrict__ pts, const double* __restrict__ boxes,
-                                 int64_t n, uint8_t* q1, int32_t* q2, double* q3)
+                                 int64_t n, int4 ignore, bool has_ignore, uint8_t* q1, int32_t* q2, double* q3)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (q1 && boxes)
-        q1[i] = ref_q1(v, boxes[4 * i], boxes[4 * i + 1], boxes[4 * i + 2], boxes[4 * i + 3], nullptr) ? 1 : 0;
+        q1[i] = ref_q1(v, boxes[4 * i], boxes[4 * i + 1], boxes[4 * i + 2], boxes[4 * i + 3],
+                       has_ignore ? &ignore : nullptr) ? 1 : 0;
     if (pts) {
         const double px = pts[2 * i], py = pts[2 * i + 1];
         if (q2) q2[i] = ref_q2(v, px, py);
@@ -262,44 +267,59 @@ int refcompat_info(const frw_tree* t, int64_t out[2])
     return FRW_OK;
 }
 
-int refcompat_gauss(const frw_tree* t, int master, double out[4])
+// The start square into t->d_surface (stream-ordered).
+static int gauss_to_device(const frw_tree* t, int4 rect, double factor, hipStream_t s)
 {
-    if (!t->ref) return set_error(FRW_E_INVALID, "tree was built without the reference-compatible index");
-    const RefView v = host_view(t);
-    const int4 m = t->host_rects[master];
-    const double fx = ref_gauss_factor(v, m, 1.5, false);
-    const double fy = ref_gauss_factor(v, m, 1.5, true);
-    const DBox s = gauss_surface(m, fx, fy);
-    out[0] = s.x0; out[1] = s.y0; out[2] = s.x1; out[3] = s.y1;
-    return FRW_OK;
+    ref_gauss_kernel<<<1, 1, 0, s>>>(device_view(t), rect, factor, t->d_surface);
+    return check_hip(hipGetLastError(), "ref_gauss_kernel");
 }
 
-int launch_ref_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o, hipStream_t s)
+int refcompat_gauss(const frw_tree* t, int4 rect, double factor, double out[4])
+{
+    if (!t->ref) return set_error(FRW_E_INVALID, "tree was built without the reference-compatible index");
+    if (factor < 1.0) {  // the reference logs and returns the rect itself
+        out[0] = rect.x; out[1] = rect.y; out[2] = rect.z; out[3] = rect.w;
+        return FRW_OK;
+    }
+    int rc = gauss_to_device(t, rect, factor, nullptr);
+    if (rc == FRW_OK)
+        rc = check_hip(hipMemcpy(out, t->d_surface, 4 * sizeof(double), hipMemcpyDeviceToHost), "copy surface");
+    return rc;
+}
+
+int launch_ref_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o, hipStream_t s,
+                    const double* surface)
 {
     if (!t->ref) return set_error(FRW_E_INVALID, "tree was built without the reference-compatible index");
     if (p->n_walks == 0) return FRW_OK;
-    double g[4];
-    int rc = refcompat_gauss(t, p->master, g);
-    if (rc != FRW_OK) return rc;
-    const DBox surf{g[0], g[1], g[2], g[3]};
+    // the start square: the master's Gaussian surface at 1.5 (getAvgPathLen
+    // via RandomWalk, mainFunctions.cpp:70), or the caller's
+    const DBox start = surface ? DBox{surface[0], surface[1], surface[2], surface[3]} : DBox{0, 0, 0, 0};
+    if (!surface) {
+        const int rc = gauss_to_device(t, t->host_rects[p->master], 1.5, s);
+        if (rc != FRW_OK) return rc;
+    }
     int cus = 0;
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, t->device);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, t->device);
     const int64_t want = (p->n_walks + 255) / 256;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)std::max(cus, 1) * 8));
     const uint32_t max_hops = (uint32_t)(p->max_hops > 0 ? p->max_hops : 10000000);
-    ref_walk_kernel<<<grid, 256, 0, s>>>(device_view(t), t->d_thresholds, surf, p->seed, p->job,
+    ref_walk_kernel<<<grid, 256, 0, s>>>(device_view(t), t->d_thresholds, start, surface ? nullptr : t->d_surface,
+                                         p->seed, p->job,
                                          (uint32_t)p->first_walk, p->n_walks, max_hops, (int32_t)t->n_rects,
                                          (unsigned long long*)o->counts, (unsigned long long*)o->hops,
                                          o->hit_out, o->hops_out);
     return check_hip(hipGetLastError(), "ref_walk_kernel");
 }
 
-int launch_ref_queries(const frw_tree* t, const double* pts, const double* boxes, int64_t n, uint8_t* q1,
-                       int32_t* q2, double* q3, hipStream_t s)
+int launch_ref_queries(const frw_tree* t, const double* pts, const double* boxes, int64_t n, const int32_t* ignore,
+                       uint8_t* q1, int32_t* q2, double* q3, hipStream_t s)
 {
     if (!t->ref) return set_error(FRW_E_INVALID, "tree was built without the reference-compatible index");
     if (n == 0) return FRW_OK;
-    ref_query_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(device_view(t), pts, boxes, n, q1, q2, q3);
+    const int4 ig = ignore ? make_int4(ignore[0], ignore[1], ignore[2], ignore[3]) : make_int4(0, 0, 0, 0);
+    ref_query_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(device_view(t), pts, boxes, n, ig,
+                                                                 ignore != nullptr, q1, q2, q3);
     int rc = check_hip(hipGetLastError(), "ref_query_kernel");
     if (rc == FRW_OK) rc = check_hip(hipStreamSynchronize(s), "ref queries");
     return rc;

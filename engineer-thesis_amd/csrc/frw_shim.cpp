@@ -192,20 +192,25 @@ void randomWalkCudaWrapper(int n, QuadTreeManager* q, unsigned int* output, Rect
     const int64_t nr = frw_layout_size(q->layout);
     int64_t *d_counts = nullptr, *d_hops = nullptr;
     int32_t* d_hit = nullptr;
-    hipMalloc(&d_counts, sizeof(int64_t) * (nr + 2));
-    hipMalloc(&d_hops, sizeof(int64_t));
-    hipMalloc(&d_hit, sizeof(int32_t) * n);
-    hipMemset(d_counts, 0, sizeof(int64_t) * (nr + 2));
-    hipMemset(d_hops, 0, sizeof(int64_t));
+    if (hipMalloc(&d_counts, sizeof(int64_t) * (nr + 2)) != hipSuccess ||
+        hipMalloc(&d_hops, sizeof(int64_t)) != hipSuccess || hipMalloc(&d_hit, sizeof(int32_t) * n) != hipSuccess ||
+        hipMemset(d_counts, 0, sizeof(int64_t) * (nr + 2)) != hipSuccess ||
+        hipMemset(d_hops, 0, sizeof(int64_t)) != hipSuccess) {
+        std::fprintf(stderr, "randomWalkCudaWrapper: device allocation failed\n");
+        (void)hipFree(d_counts);
+        (void)hipFree(d_hops);
+        (void)hipFree(d_hit);
+        return;
+    }
     frw_walk_params p{};
     p.master = q->master;
     p.mode = FRW_MODE_EXACT;
     p.n_walks = n;
     p.seed = randomSeed;
     frw_walk_outputs o{d_counts, d_hops, d_hit, nullptr};
-    if (frw_walk_async(q->tree, &p, &o, nullptr) == FRW_OK && hipDeviceSynchronize() == hipSuccess) {
-        std::vector<int32_t> hit((size_t)n), rects((size_t)nr * 4);
-        hipMemcpy(hit.data(), d_hit, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
+    std::vector<int32_t> hit((size_t)n), rects((size_t)nr * 4);
+    if (frw_walk_async(q->tree, &p, &o, nullptr) == FRW_OK && hipDeviceSynchronize() == hipSuccess &&
+        hipMemcpy(hit.data(), d_hit, sizeof(int32_t) * n, hipMemcpyDeviceToHost) == hipSuccess) {
         frw_layout_rects(q->layout, rects.data());
         std::vector<unsigned int> flag((size_t)n);
         std::vector<RectHost> out((size_t)n);
@@ -214,14 +219,15 @@ void randomWalkCudaWrapper(int n, QuadTreeManager* q, unsigned int* output, Rect
             out[i] = hit[i] >= 0 ? to_rect(&rects[4 * (size_t)hit[i]]) : RectHost();
             flag[i] = (out[i] == master) ? 0u : 1u;
         }
-        if (output) hipMemcpy(output, flag.data(), sizeof(unsigned int) * n, hipMemcpyHostToDevice);
-        if (rectOutput) hipMemcpy(rectOutput, out.data(), sizeof(RectHost) * n, hipMemcpyHostToDevice);
+        if ((output && hipMemcpy(output, flag.data(), sizeof(unsigned int) * n, hipMemcpyHostToDevice) != hipSuccess) ||
+            (rectOutput && hipMemcpy(rectOutput, out.data(), sizeof(RectHost) * n, hipMemcpyHostToDevice) != hipSuccess))
+            std::fprintf(stderr, "randomWalkCudaWrapper: copy to the output arrays failed\n");
     } else {
         std::fprintf(stderr, "randomWalkCudaWrapper: %s\n", frw_last_error_string());
     }
-    hipFree(d_counts);
-    hipFree(d_hops);
-    hipFree(d_hit);
+    (void)hipFree(d_counts);
+    (void)hipFree(d_hops);
+    (void)hipFree(d_hit);
 }
 
 bool initCuda(int argc, char** argv)
@@ -249,4 +255,251 @@ floatingPoint countAvg(unsigned int output[], int ITER_NUM)
     floatingPoint out = 0;
     for (int i = 0; i < ITER_NUM; ++i) out += output[i];
     return out / ITER_NUM;
+}
+
+// ---------------------------------------------------------------- Tree
+namespace {
+
+// Device staging of one Tree query or walk.
+struct TreeScratch {
+    double* in = nullptr;      // box or point
+    double* q3 = nullptr;      // 4 doubles
+    int32_t* q2 = nullptr;
+    uint8_t* q1 = nullptr;
+    int64_t* counts = nullptr; // n_rects + 2
+    int64_t* hops = nullptr;
+    int32_t* hit = nullptr;
+    int32_t* hop_n = nullptr;
+};
+
+void free_scratch(TreeScratch* s)
+{
+    if (!s) return;
+    for (void* p : {(void*)s->in, (void*)s->q3, (void*)s->q2, (void*)s->q1, (void*)s->counts, (void*)s->hops,
+                    (void*)s->hit, (void*)s->hop_n})
+        if (p) (void)hipFree(p);
+    delete s;
+}
+
+bool is_int(double v) { return v == (double)(int32_t)v; }
+
+bool rect_ints(const RectHost& r, int32_t out[4])
+{
+    const double c[4] = {r.topLeft.x, r.topLeft.y, r.bottomRight.x, r.bottomRight.y};
+    for (int k = 0; k < 4; ++k) {
+        if (!is_int(c[k])) return false;
+        out[k] = (int32_t)c[k];
+    }
+    return true;
+}
+
+bool dev_ok(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return true;
+    std::fprintf(stderr, "Tree: %s: %s\n", what, hipGetErrorString(e));
+    return false;
+}
+
+}  // namespace
+
+Tree::Tree(int pLevel, int nodeSCount, RectHost const& b) : bounds(b)
+{
+    (void)pLevel;      // the root is level 0 (quadTree.cpp:5-17); sub-trees are internal here
+    (void)nodeSCount;  // a capacity hint in the reference
+}
+
+Tree::~Tree() { clear(); }
+
+bool Tree::isInBounds(point const& p)
+{
+    return p.x > bounds.topLeft.x && p.y > bounds.topLeft.y && p.x < bounds.bottomRight.x &&
+           p.y < bounds.bottomRight.y;
+}
+
+bool Tree::isInBounds(RectHost const& r)
+{
+    return r.topLeft.y >= bounds.topLeft.y && r.topLeft.x >= bounds.topLeft.x &&
+           r.bottomRight.y <= bounds.bottomRight.y && r.bottomRight.x <= bounds.bottomRight.x;
+}
+
+void Tree::clear()
+{
+    rects.clear();
+    if (tree) frw_tree_free(tree);
+    if (layout) frw_layout_free(layout);
+    free_scratch(static_cast<TreeScratch*>(scratch));
+    tree = nullptr;
+    layout = nullptr;
+    scratch = nullptr;
+    dirty = true;
+}
+
+bool Tree::insert(RectHost const& r)
+{
+    if (!isInBounds(r)) return false;
+    rects.push_back(r);
+    dirty = true;
+    return true;
+}
+
+frw_tree* Tree::deviceTree()
+{
+    if (!dirty) return tree;
+    const Layer keep = rects;
+    clear();
+    rects = keep;
+    std::vector<int32_t> r;
+    int32_t b[4];
+    if (rects.empty() || !to_ints(rects, &r) || !rect_ints(bounds, b)) {
+        std::fprintf(stderr, "Tree: needs at least one rect and integer coordinates\n");
+        return nullptr;
+    }
+    frw_tree_params tp{};
+    tp.kind = FRW_TREE_REFCOMPAT;
+    if (frw_layout_from_rects(r.data(), (int64_t)rects.size(), &layout) != FRW_OK ||
+        frw_layout_set_bounds(layout, b) != FRW_OK || frw_tree_build(layout, &tp, nullptr, &tree) != FRW_OK) {
+        std::fprintf(stderr, "Tree: %s\n", frw_last_error_string());
+        clear();
+        rects = keep;
+        return nullptr;
+    }
+    TreeScratch* s = new TreeScratch;
+    scratch = s;
+    const size_t nb = rects.size() + 2;
+    if (!dev_ok(hipMalloc(&s->in, 4 * sizeof(double)), "hipMalloc") ||
+        !dev_ok(hipMalloc(&s->q3, 4 * sizeof(double)), "hipMalloc") ||
+        !dev_ok(hipMalloc(&s->q2, sizeof(int32_t)), "hipMalloc") ||
+        !dev_ok(hipMalloc(&s->q1, sizeof(uint8_t)), "hipMalloc") ||
+        !dev_ok(hipMalloc(&s->counts, nb * sizeof(int64_t)), "hipMalloc") ||
+        !dev_ok(hipMalloc(&s->hops, sizeof(int64_t)), "hipMalloc") ||
+        !dev_ok(hipMalloc(&s->hit, sizeof(int32_t)), "hipMalloc") ||
+        !dev_ok(hipMalloc(&s->hop_n, sizeof(int32_t)), "hipMalloc") ||
+        !dev_ok(hipMemset(s->counts, 0, nb * sizeof(int64_t)), "hipMemset") ||
+        !dev_ok(hipMemset(s->hops, 0, sizeof(int64_t)), "hipMemset")) {
+        clear();
+        rects = keep;
+        return nullptr;
+    }
+    dirty = false;
+    return tree;
+}
+
+bool Tree::checkCollisions(RectHost const& r, const RectHost& ignore)
+{
+    frw_tree* t = deviceTree();
+    if (!t) return !isInBounds(r);
+    TreeScratch* s = static_cast<TreeScratch*>(scratch);
+    const double box[4] = {r.topLeft.x, r.topLeft.y, r.bottomRight.x, r.bottomRight.y};
+    int32_t ig[4];
+    const bool has_ig = rect_ints(ignore, ig);  // a non-integer rect equals no stored rect
+    uint8_t out = 0;
+    if (!dev_ok(hipMemcpy(s->in, box, sizeof(box), hipMemcpyHostToDevice), "copy box")) return true;
+    if (frw_query_refcompat(t, nullptr, s->in, 1, has_ig ? ig : nullptr, s->q1, nullptr, nullptr, nullptr) !=
+        FRW_OK) {
+        std::fprintf(stderr, "Tree: %s\n", frw_last_error_string());
+        return true;
+    }
+    if (!dev_ok(hipMemcpy(&out, s->q1, 1, hipMemcpyDeviceToHost), "read Q1")) return true;
+    return out != 0;
+}
+
+bool Tree::checkCollisons(point p, RectHost& r)
+{
+    frw_tree* t = deviceTree();
+    if (!t) return false;
+    TreeScratch* s = static_cast<TreeScratch*>(scratch);
+    const double pt[2] = {p.x, p.y};
+    int32_t id = -1;
+    if (!dev_ok(hipMemcpy(s->in, pt, sizeof(pt), hipMemcpyHostToDevice), "copy point")) return false;
+    if (frw_query_refcompat(t, s->in, nullptr, 1, nullptr, nullptr, s->q2, nullptr, nullptr) != FRW_OK) {
+        std::fprintf(stderr, "Tree: %s\n", frw_last_error_string());
+        return false;
+    }
+    if (!dev_ok(hipMemcpy(&id, s->q2, sizeof(id), hipMemcpyDeviceToHost), "read Q2")) return false;
+    if (id < 0) return false;
+    r = rects[(size_t)id];
+    return true;
+}
+
+RectHost Tree::drawBiggestSquareAtPoint(point p)
+{
+    frw_tree* t = deviceTree();
+    if (!t) return RectHost();
+    TreeScratch* s = static_cast<TreeScratch*>(scratch);
+    const double pt[2] = {p.x, p.y};
+    double sq[4];
+    if (!dev_ok(hipMemcpy(s->in, pt, sizeof(pt), hipMemcpyHostToDevice), "copy point")) return RectHost();
+    if (frw_query_refcompat(t, s->in, nullptr, 1, nullptr, nullptr, nullptr, s->q3, nullptr) != FRW_OK) {
+        std::fprintf(stderr, "Tree: %s\n", frw_last_error_string());
+        return RectHost();
+    }
+    if (!dev_ok(hipMemcpy(sq, s->q3, sizeof(sq), hipMemcpyDeviceToHost), "read Q3")) return RectHost();
+    return RectHost(point(sq[0], sq[1]), point(sq[2], sq[3]));
+}
+
+RectHost Tree::creatGaussianSurfFrom(RectHost const& r, floatingPoint const factor)
+{
+    if (factor < 1) return r;  // the reference logs and returns r (quadTree.cpp:353-357)
+    frw_tree* t = deviceTree();
+    int32_t ri[4];
+    if (!t || !rect_ints(r, ri)) return r;
+    double g[4];
+    if (frw_refcompat_gauss_surface(t, ri, factor, g) != FRW_OK) {
+        std::fprintf(stderr, "Tree: %s\n", frw_last_error_string());
+        return r;
+    }
+    return RectHost(point(g[0], g[1]), point(g[2], g[3]));
+}
+
+void Tree::walkBuffers(int64_t** counts, int64_t** hops, int32_t** hit, int32_t** hop_n)
+{
+    TreeScratch* s = static_cast<TreeScratch*>(scratch);
+    *counts = s ? s->counts : nullptr;
+    *hops = s ? s->hops : nullptr;
+    *hit = s ? s->hit : nullptr;
+    *hop_n = s ? s->hop_n : nullptr;
+}
+
+void createTree(Tree* mainTree, Layer const& layer)
+{
+    if (!mainTree) return;
+    for (const RectHost& r : layer) mainTree->insert(r);
+}
+
+int getIndex(REAL64_t intg[NSAMPLE + 1], floatingPoint rand)
+{
+    for (int i = 0; i < NSAMPLE; ++i)
+        if (intg[i] <= rand && intg[i + 1] > rand) return i;
+    return -1;
+}
+
+RectHost RandomWalk(RectHost const& R, Tree* mainTree, int& pointCount, REAL64_t intg[NSAMPLE + 1], int iterId)
+{
+    (void)intg;
+    pointCount = 0;
+    if (!mainTree || iterId < 0) return RectHost();
+    frw_tree* t = mainTree->deviceTree();
+    if (!t) return RectHost();
+    const RectHost g = mainTree->creatGaussianSurfFrom(R, 1.5);
+    const double surface[4] = {g.topLeft.x, g.topLeft.y, g.bottomRight.x, g.bottomRight.y};
+    // the device staging of this tree (counts accumulate there unread)
+    int32_t hit = -1, hops = 0;
+    int64_t* counts = nullptr;
+    int64_t* hop_sum = nullptr;
+    int32_t *d_hit = nullptr, *d_hops = nullptr;
+    mainTree->walkBuffers(&counts, &hop_sum, &d_hit, &d_hops);
+    frw_walk_params p{};
+    p.n_walks = 1;
+    p.first_walk = iterId;
+    p.seed = shim_seed();
+    frw_walk_outputs o{counts, hop_sum, d_hit, d_hops};
+    if (frw_refcompat_walk_from(t, surface, &p, &o, nullptr) != FRW_OK) {
+        std::fprintf(stderr, "RandomWalk: %s\n", frw_last_error_string());
+        return RectHost();
+    }
+    if (!dev_ok(hipMemcpy(&hit, d_hit, sizeof(hit), hipMemcpyDeviceToHost), "read hit") ||
+        !dev_ok(hipMemcpy(&hops, d_hops, sizeof(hops), hipMemcpyDeviceToHost), "read hops"))
+        return RectHost();
+    pointCount = hops;
+    return hit >= 0 ? mainTree->objects()[(size_t)hit] : RectHost();
 }

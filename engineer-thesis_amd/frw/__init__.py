@@ -95,7 +95,8 @@ EXPORTS = [
     "frw_walk_learned", "frw_walk_learned_async", "frw_walk_learned_traffic",
     "frw_mlp_levels", "frw_mlp_certify", "frw_mlp_add_samples", "frw_batch_set_proposal",
     "frw_batch_walk_learned", "frw_batch_autotune_learned", "frw_walk_weighted_async", "frw_walk_weighted", "frw_walk_adaptive", "frw_weight_tables",
-    "frw_query_refcompat", "frw_refcompat_gauss_surface", "frw_refcompat_info",
+    "frw_query_refcompat", "frw_refcompat_gauss_surface", "frw_refcompat_info", "frw_refcompat_walk_from",
+    "frw_layout_set_bounds",
 ]
 
 _lib = None
@@ -137,8 +138,10 @@ def lib():
     L.frw_tree_free.argtypes = [vp]
     L.frw_tree_free.restype = None
     L.frw_query_collide.argtypes = [vp, vp, i64, vp, vp]
-    L.frw_query_refcompat.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp]
-    L.frw_refcompat_gauss_surface.argtypes = [vp, i32, vp]
+    L.frw_query_refcompat.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp, vp]
+    L.frw_refcompat_gauss_surface.argtypes = [vp, vp, C.c_double, vp]
+    L.frw_refcompat_walk_from.argtypes = [vp, vp, C.POINTER(WalkParams), C.POINTER(WalkOutputs), vp]
+    L.frw_layout_set_bounds.argtypes = [vp, vp]
     L.frw_refcompat_info.argtypes = [vp, vp]
     L.frw_query_point.argtypes = [vp, vp, i64, vp, vp]
     L.frw_query_nearest.argtypes = [vp, vp, i64, vp, vp]
@@ -251,6 +254,11 @@ class Layout:
     def __len__(self):
         return lib().frw_layout_size(self.h)
 
+    def set_bounds(self, b):
+        """Replace the padded bounds with the caller's (every rect inside them)."""
+        arr = np.asarray(b, dtype=np.int32)
+        _check(lib().frw_layout_set_bounds(self.h, arr.ctypes.data))
+
     def rects(self):
         out = np.zeros((len(self), 4), dtype=np.int32)
         if len(out):
@@ -293,6 +301,9 @@ class Tree:
         _check(lib().frw_tree_info_get(self.h, C.byref(i)))
         return i
 
+    def layout_rect(self, index):
+        return self.layout.rects()[index]
+
     def export(self):
         i = self.info()
         nodes = np.zeros((i.n_nodes, 8), dtype=np.int32)
@@ -315,16 +326,34 @@ class Tree:
         _check(lib().frw_query_maxsquare(self.h, pts.data_ptr(), pts.shape[0], mode, out.data_ptr(),
                                          stream))
 
-    def refcompat_queries(self, pts=None, boxes=None, q1=None, q2=None, q3=None, stream=None):
-        """Reference-compatible Q1 (boxes), Q2 and Q3 (points) into device tensors."""
+    def refcompat_queries(self, pts=None, boxes=None, q1=None, q2=None, q3=None, stream=None, ignore=None):
+        """Reference-compatible Q1 (boxes, skipping a rect equal to `ignore`),
+        Q2 and Q3 (points) into device tensors."""
         n = pts.shape[0] if pts is not None else boxes.shape[0]
         ptr = lambda x: x.data_ptr() if x is not None else None
-        _check(lib().frw_query_refcompat(self.h, ptr(pts), ptr(boxes), n, ptr(q1), ptr(q2), ptr(q3), stream))
+        ig = np.asarray(ignore, dtype=np.int32) if ignore is not None else None
+        _check(lib().frw_query_refcompat(self.h, ptr(pts), ptr(boxes), n,
+                                         ig.ctypes.data if ig is not None else None,
+                                         ptr(q1), ptr(q2), ptr(q3), stream))
 
-    def refcompat_gauss_surface(self, master):
+    def refcompat_gauss_surface(self, master=None, rect=None, factor=1.5):
+        """Tree::creatGaussianSurfFrom(rect, factor) on the reference tree (rect = the
+        master's by default)."""
+        r = np.asarray(self.layout_rect(master) if rect is None else rect, dtype=np.int32)
         out = np.zeros(4, dtype=np.float64)
-        _check(lib().frw_refcompat_gauss_surface(self.h, master, out.ctypes.data))
+        _check(lib().frw_refcompat_gauss_surface(self.h, r.ctypes.data, factor, out.ctypes.data))
         return tuple(out)
+
+    def refcompat_walk_from(self, surface, n_walks, counts, hops, hit_out=None, hops_out=None, seed=1,
+                            job=0, first_walk=0, max_hops=0, stream=None):
+        """Reference-compatible walks from an explicit start square (device outputs)."""
+        sq = np.asarray(surface, dtype=np.float64)
+        p = WalkParams(master=0, mode=MODE_REFCOMPAT, n_walks=n_walks, first_walk=first_walk, seed=seed,
+                       job=job, block=0, grid=0, reserved=0, max_hops=max_hops)
+        o = WalkOutputs(counts.data_ptr(), hops.data_ptr(),
+                        hit_out.data_ptr() if hit_out is not None else None,
+                        hops_out.data_ptr() if hops_out is not None else None)
+        _check(lib().frw_refcompat_walk_from(self.h, sq.ctypes.data, C.byref(p), C.byref(o), stream))
 
     def refcompat_info(self):
         out = np.zeros(2, dtype=np.int64)

@@ -84,6 +84,10 @@ int64_t frw_layout_size(const frw_layout* l);
 int frw_layout_rects(const frw_layout* l, int32_t* xyxy_out);
 /* Padded absorbing bounds = Parser::getLayerSize (Parser.cpp:51-87). */
 int frw_layout_bounds(const frw_layout* l, int32_t out[4]);
+/* Replaces the bounds with the caller's, as the reference's
+ * Tree(level, count, bounds) takes them (randomWalk/CPU/quadTree.h:51,
+ * getAvgPathLen.cpp:63); every rect must lie inside them (inclusive). */
+int frw_layout_set_bounds(frw_layout* l, const int32_t b[4]);
 void frw_layout_free(frw_layout* l);
 
 /* Panel thresholds of the Green CDF: T[i] = round(intg[i] * 2^32), the table
@@ -386,12 +390,25 @@ int frw_walk_adaptive(const frw_tree* t, const frw_walk_params* p, double rel_er
  * (quadTree.cpp:240-262), Q3 the literal bisection (quadTree.cpp:264-312) —
  * so walks reproduce the reference CPU FRW (randomWalk/CPU/mainFunctions.cpp:
  * 48-90) walk for walk for the same random stream.  One lane per walk.
- * Query hook: q1[i] for boxes[i] (x0,y0,x1,y1), q2[i] / q3[i] (4 doubles)
- * for pts[i]; any output may be NULL.  Synchronous. */
-int frw_query_refcompat(const frw_tree* t, const double* pts, const double* boxes, int64_t n, uint8_t* q1,
-                        int32_t* q2, double* q3, void* stream);
-/* The reference's Gaussian surface of `master` on its own tree (quadTree.cpp:315-364). */
-int frw_refcompat_gauss_surface(const frw_tree* t, int32_t master, double out[4]);
+ * Query hook: q1[i] for boxes[i] (x0,y0,x1,y1), ignoring a rect equal to
+ * *ignore (host int32[4], may be NULL) as Tree::checkCollisions(r, ignore)
+ * does (quadTree.cpp:148); q2[i] / q3[i] (4 doubles) for pts[i]; device
+ * arrays; any output may be NULL.  Synchronous. */
+int frw_query_refcompat(const frw_tree* t, const double* pts, const double* boxes, int64_t n,
+                        const int32_t* ignore, uint8_t* q1, int32_t* q2, double* q3, void* stream);
+/* Tree::creatGaussianSurfFrom(r, factor) (quadTree.cpp:351-364) on the
+ * reference tree, computed on the device: rect (host int32[4]) scaled per axis
+ * by the 10-step factor search (quadTree.cpp:315-349) from `factor`, the rect
+ * itself ignored by value; a factor below 1 gives the rect itself, as in the
+ * reference.  out: host double[4].  Synchronous. */
+int frw_refcompat_gauss_surface(const frw_tree* t, const int32_t rect[4], double factor, double out[4]);
+/* RandomWalk(R, tree, pointCount, intg, iterId) (randomWalk/CPU/
+ * mainFunctions.cpp:48-90) from an explicit start square (host double[4],
+ * normally frw_refcompat_gauss_surface of R at 1.5): p->n_walks walks with
+ * ids from p->first_walk (p->seed, p->job, p->max_hops as for frw_walk; master
+ * and mode are not used) into the outputs of frw_walk_async. */
+int frw_refcompat_walk_from(const frw_tree* t, const double surface[4], const frw_walk_params* p,
+                            const frw_walk_outputs* o, void* stream);
 /* {nodes, object entries} of the reference-compatible tree. */
 int frw_refcompat_info(const frw_tree* t, int64_t out[2]);
 

@@ -17,6 +17,8 @@
 #ifndef FRW_RANDOMWALK_H
 #define FRW_RANDOMWALK_H
 
+#include <stdint.h>
+
 #include <string>
 #include <vector>
 
@@ -58,6 +60,63 @@ private:
     std::vector<RectHost> bounds;
     std::string delimiter;
 };
+
+typedef double REAL64_t;
+#ifndef NSAMPLE
+#define NSAMPLE 200  // panels per square (randomWalk/defines.h:25)
+#endif
+
+struct frw_layout;
+struct frw_tree;
+
+// randomWalk/CPU/quadTree.h:31-77: the reference's CPU quadtree, served by the
+// device reference-compatible index (the reference tree rebuilt with its
+// insertion behaviour, its single-path queries and literal bisection).
+// insert() collects rects in call order; the device index is built from them
+// at the first query after an insert.  Every query is one device call.
+// Differences: coordinates must be integers (the library's rect type);
+// insert() returns whether the rect lies inside the bounds (the reference
+// also returns false for a rect its level cap drops, quadTree.cpp:139-145).
+class Tree {
+public:
+    Tree(int pLevel, int nodeSCount, RectHost const& bounds);
+    ~Tree();
+    Tree(const Tree&) = delete;
+    Tree& operator=(const Tree&) = delete;
+    RectHost getBounds() { return bounds; }
+    bool isInBounds(point const& p);       // strict (quadTree.cpp:49-59)
+    bool isInBounds(RectHost const& r);    // inclusive (quadTree.cpp:38-47)
+    void clear();
+    bool insert(RectHost const& r);
+    bool checkCollisions(RectHost const& r, const RectHost& ignore = RectHost());
+    bool checkCollisons(point p, RectHost& r);
+    RectHost drawBiggestSquareAtPoint(point p);
+    RectHost creatGaussianSurfFrom(RectHost const& r, floatingPoint const factor);
+    // not in the reference class: the device index (built on demand, NULL on
+    // error), the stored rects, and the device buffers RandomWalk writes
+    frw_tree* deviceTree();
+    const Layer& objects() const { return rects; }
+    void walkBuffers(int64_t** counts, int64_t** hops, int32_t** hit, int32_t** hop_n);
+
+private:
+    RectHost bounds;
+    Layer rects;
+    frw_layout* layout = nullptr;
+    frw_tree* tree = nullptr;
+    void* scratch = nullptr;  // device staging for one query
+    bool dirty = true;
+};
+
+// randomWalk/CPU/mainFunctions.h:15-17 (mainFunctions.cpp:33-90).  RandomWalk
+// runs walk iterId (Philox stream keyed by FRW_SEED) from the Gaussian surface
+// of R at 1.5 on the device and returns the rect it ended on (RectHost() when
+// it escaped); pointCount = its hops.  intg is accepted for the signature:
+// the walk samples the library's Green table (the same CDF re-derived from its
+// series).  getIndex is the reference's linear CDF scan (-1 when rand lies
+// outside the table).
+void createTree(Tree* mainTree, Layer const& layer);
+RectHost RandomWalk(RectHost const& R, Tree* mainTree, int& pointCount, REAL64_t intg[NSAMPLE + 1], int iterId);
+int getIndex(REAL64_t intg[NSAMPLE + 1], floatingPoint rand);
 
 // Opaque device tree + master (randomWalk/Cuda/d_quadtree.h:36-47).
 struct QuadTreeManager;

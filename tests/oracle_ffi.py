@@ -103,6 +103,9 @@ def oracle():
     L.orc_mlp_certify.argtypes = [_D, C.c_int, C.POINTER(Box), C.c_double, C.c_double, C.c_double]
     L.orc_mlp_backward.restype = C.c_float
     L.orc_mlp_backward.argtypes = [_F32, _F32, _F32, C.c_int, _F32]
+    L.orc_walk_one.restype = C.c_int
+    L.orc_walk_one.argtypes = [C.c_void_p, C.POINTER(Box), _U32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64,
+                               _I64]
     L.orc_walk_many.restype = C.c_int64
     L.orc_walk_many.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_uint32, C.c_int64,
                                 C.c_int64, C.c_int64, _I64, _I32, _I32, C.c_int]
@@ -262,6 +265,14 @@ class Index:
                                      _ptr(counts, _I64), _ptr(hit, _I32), _ptr(hops, _I32),
                                      nthreads)
         return counts, total, hit, hops
+
+    def walk_one_from(self, surface, T, seed, job, walk, max_hops=10**6):
+        """(hit, hops) of one walk from an explicit start square."""
+        T = np.ascontiguousarray(T, dtype=np.uint32)
+        h = C.c_int64()
+        hit = self.L.orc_walk_one(self.h, C.byref(Box(*surface)), _ptr(T, _U32), seed, job, walk, max_hops,
+                                  C.byref(h))
+        return hit, h.value
 
 
 class RefTree:

@@ -49,5 +49,12 @@ def test_shim_symbols_exported():
                 "randomWalkCudaWrapper(int, QuadTreeManager*, unsigned int*, RectHost*, unsigned long long)",
                 "freeQuadTreeManager(QuadTreeManager*)", "countAvg(unsigned int*, int)",
                 "Parser::parse(std::__cxx11::basic_string<char, std::char_traits<char>, std::allocator<char> > const&)",
-                "initCuda(int, char**)"]:
+                "initCuda(int, char**)",
+                "createTree(Tree*, std::vector<RectHost, std::allocator<RectHost> > const&)",
+                "RandomWalk(RectHost const&, Tree*, int&, double*, int)", "getIndex(double*, double)",
+                "Tree::Tree(int, int, RectHost const&)", "Tree::insert(RectHost const&)",
+                "Tree::checkCollisions(RectHost const&, RectHost const&)",
+                "Tree::checkCollisons(point, RectHost&)", "Tree::drawBiggestSquareAtPoint(point)",
+                "Tree::creatGaussianSurfFrom(RectHost const&, double)", "Tree::isInBounds(point const&)",
+                "Tree::isInBounds(RectHost const&)", "Tree::clear()"]:
         assert sym in out, sym

@@ -70,10 +70,17 @@ def test_refcompat_queries(name):
     q2 = torch.zeros(n, dtype=torch.int32, device="cuda")
     q3 = torch.zeros((n, 4), dtype=torch.float64, device="cuda")
     tree.refcompat_queries(to_dev(torch, pts), to_dev(torch, boxes), q1, q2, q3)
-    oq1, oq2, oq3 = oracle_index(rects, bounds, O.REFTREE).queries_many(pts, boxes, n_q3=n)
+    ix = oracle_index(rects, bounds, O.REFTREE)
+    oq1, oq2, oq3 = ix.queries_many(pts, boxes, n_q3=n)
     assert np.array_equal(q1.cpu().numpy(), oq1)
     assert np.array_equal(q2.cpu().numpy(), oq2)
     assert np.array_equal(q3.cpu().numpy(), oq3)
+    # Q1 with a rect ignored by value (the Gaussian-surface search's form)
+    ig = rects[len(rects) // 2]
+    sel = slice(0, 400)
+    tree.refcompat_queries(boxes=to_dev(torch, boxes[sel]), q1=q1[sel], ignore=ig)
+    want = [ix.q1(tuple(b), ignore=tuple(float(v) for v in ig)) for b in boxes[sel]]
+    assert np.array_equal(q1[sel].cpu().numpy(), np.array(want, dtype=np.uint8))
     # the single-path queries differ from the full search somewhere on the
     # larger fixtures; a mode that answered the full search would fail here
     if len(rects) >= 1000:
@@ -88,6 +95,8 @@ def test_refcompat_gauss_surface(name, master):
     ix = oracle_index(rects, bounds, O.REFTREE)
     for m in sorted({master, 0, len(rects) // 2, len(rects) - 1}):
         assert tree.refcompat_gauss_surface(m) == ix.gauss_from(m), m
+    # factor below one: the reference returns the rect itself
+    assert tree.refcompat_gauss_surface(master, factor=0.5) == tuple(float(v) for v in rects[master])
 
 
 @pytest.mark.parametrize("name,master", FIXTURES)
@@ -101,6 +110,39 @@ def test_refcompat_walks_replay_oracle(name, master):
     assert np.array_equal(hp, ohp)
     assert hops == ohops
     assert np.array_equal(counts, oc)
+
+
+def test_refcompat_walk_from_explicit_square():
+    """Walks from the caller's start square equal the master's walks when the
+    square is the master's Gaussian surface, and the oracle's walks from the
+    same square otherwise."""
+    torch = torch_dev()
+    lay, rects, bounds = load_fixture("test100.txt")
+    tree = frw.Tree(lay, ref_compat=True)
+    ix = oracle_index(rects, bounds, O.REFTREE)
+    nr, n = len(rects), 1500
+    c0, h0, hit0, hp0 = gpu_ref_walks(tree, 3, n, seed=2)
+
+    def from_square(sq):
+        counts = torch.zeros(nr + 2, dtype=torch.int64, device="cuda")
+        hops = torch.zeros(1, dtype=torch.int64, device="cuda")
+        hit = torch.full((n,), -7, dtype=torch.int32, device="cuda")
+        hp = torch.full((n,), -7, dtype=torch.int32, device="cuda")
+        tree.refcompat_walk_from(sq, n, counts, hops, hit, hp, seed=2)
+        torch.cuda.synchronize()
+        return counts.cpu().numpy(), hit.cpu().numpy(), hp.cpu().numpy()
+
+    c1, hit1, hp1 = from_square(tree.refcompat_gauss_surface(3))
+    assert np.array_equal(hit1, hit0) and np.array_equal(hp1, hp0) and np.array_equal(c1, c0)
+    # a square of the caller's: the oracle walk from it (its 'master' only
+    # names the start surface, taken here from a rect with that surface)
+    r = rects[7]
+    sq = tuple(float(v) for v in (r[0] - 3, r[1] - 3, r[2] + 3, r[3] + 3))
+    c2, hit2, hp2 = from_square(sq)
+    T = O.green_thresholds()
+    for i in range(0, n, 97):
+        oh, ohops = ix.walk_one_from(sq, T, 2, 0, i)
+        assert hit2[i] == oh and hp2[i] == ohops, i
 
 
 def test_refcompat_seed_job_offset_and_max_hops():
