@@ -445,6 +445,13 @@ def main():
             "avg_launch_ms": avg_ms,
         },
     }
+    # measured read ceilings: a 2 MiB buffer stays in each XCD's L2, a 4 GiB
+    # one streams from HBM; the walk's records come mostly from L1/L2
+    l2_gbs = frw.probe_read_bandwidth(2 << 20, 20000)
+    hbm_gbs = frw.probe_read_bandwidth(4 << 30, 4)
+    out["roofline"]["measured_read_gbs"] = {"l2_resident_2MiB": l2_gbs, "hbm_4GiB": hbm_gbs}
+    out["roofline"]["frac_of_measured_l2_read"] = achieved / l2_gbs
+    log(f"read ceilings: L2-resident {l2_gbs:.0f} GB/s, HBM {hbm_gbs:.0f} GB/s")
     # the physical capacitance row of the master (weighted estimator), same
     # layout: plain first hop, then the importance-sampled one
     nw = min(walks, 10_000_000)

@@ -247,14 +247,19 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
                                       Counters* cnt, int hint = -1, int* arg = nullptr)
 {
     // node pruning and ordering use integer lower bounds (an integer lb is
-    // below best exactly when it is below ceil(best)); rect distances are exact
+    // below best exactly when it is below ceil(best)); rect distances are exact.
+    // When p lies in a rect (best <= 0) the search also settles ties the way
+    // the point query does (deepest containment, then lowest original id), so
+    // the arg is the point query's answer: nodes with lb == ceil(best) are
+    // still visited then (lim = ceil(best) + 1) and an equal distance with a
+    // lower original id replaces the arg.
     const int fx = floor_int(px), cx = ceil_int(px), fy = floor_int(py), cy = ceil_int(py);
     double best = __builtin_inf();
-    int best_c = 2147483647;
+    int lim = 2147483647;
     int besti = -1;
     if (hint >= 0) {
         best = linf(t.rects[hint], px, py);
-        best_c = ceil_int(best);
+        lim = ceil_int(best) + (best <= 0.0 ? 1 : 0);
         besti = hint;
         if (cnt) cnt->rects++;
     }
@@ -264,14 +269,14 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
         const int ni = st.at(--sp);
         const Node nd = t.nodes[ni];
         if (cnt) cnt->nodes++;
-        if (!(linf_lb(nd.box, fx, cx, fy, cy) < best_c)) continue;
+        if (!(linf_lb(nd.box, fx, cx, fy, cy) < lim)) continue;
         if (cnt) cnt->rects += (uint32_t)(nd.rend - nd.rbeg);
         if (cnt && nd.first_child >= 0) cnt->nodes += 4u;
         for (int r = nd.rbeg; r < nd.rend; ++r) {
             const double d = linf(t.rects[r], px, py);
-            if (d < best) {
+            if (d < best || (d == best && d <= 0.0 && t.ids[r] < t.ids[besti])) {
                 best = d;
-                best_c = ceil_int(d);
+                lim = ceil_int(d) + (d <= 0.0 ? 1 : 0);
                 besti = r;
             }
         }
@@ -286,8 +291,8 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
                 if (lb[c] < lb[near]) near = c;
 #pragma unroll
             for (int c = 0; c < 4; ++c)
-                if (c != near && lb[c] < best_c) st.at(sp++) = nd.first_child + c;
-            if (lb[near] < best_c) st.at(sp++) = nd.first_child + near;
+                if (c != near && lb[c] < lim) st.at(sp++) = nd.first_child + c;
+            if (lb[near] < lim) st.at(sp++) = nd.first_child + near;
         }
     }
     if (arg) *arg = besti;

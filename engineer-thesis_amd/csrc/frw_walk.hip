@@ -339,7 +339,8 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
                     const double ds = nearest_rect(t, st, px, py, cp, hint, &arg);
                     hint = arg;
                     if (ds <= 0.0) {
-                        result = q2_point(t, st, px, py, cp);
+                        // p lies in a rect: the search's arg is the point query's answer
+                        result = t.ids[arg];
                     } else {
                         if (kHarvest) {
                             const unsigned long long slot = atomicAdd(a.h_count, 1ull);

@@ -94,7 +94,7 @@ EXPORTS = [
     "frw_mlp_gradient", "frw_mlp_set_samples", "frw_mlp_train", "frw_walk_harvest",
     "frw_walk_learned", "frw_walk_learned_async", "frw_walk_learned_traffic",
     "frw_mlp_levels", "frw_mlp_certify", "frw_mlp_add_samples", "frw_batch_set_proposal",
-    "frw_batch_walk_learned", "frw_batch_autotune_learned", "frw_walk_weighted_async", "frw_walk_weighted", "frw_walk_adaptive", "frw_weight_tables",
+    "frw_batch_walk_learned", "frw_batch_autotune_learned", "frw_walk_weighted_async", "frw_walk_weighted", "frw_walk_adaptive", "frw_weight_tables", "frw_probe_read_bandwidth",
     "frw_query_refcompat", "frw_refcompat_gauss_surface", "frw_refcompat_info", "frw_refcompat_walk_from",
     "frw_layout_set_bounds",
 ]
@@ -153,6 +153,7 @@ def lib():
     L.frw_walk_weighted_async.argtypes = [vp, C.POINTER(WalkParams), C.POINTER(WalkOutputs), C.c_double,
                                           C.c_uint32, vp, vp, vp, vp]
     L.frw_walk_weighted.argtypes = [vp, C.POINTER(WalkParams), C.c_double, C.c_uint32, vp, vp, vp, vp]
+    L.frw_probe_read_bandwidth.argtypes = [i64, i32, C.POINTER(C.c_double)]
     L.frw_weight_tables.argtypes = [vp, vp]
     L.frw_weight_tables.restype = None
     L.frw_walk_adaptive.argtypes = [vp, C.POINTER(WalkParams), C.c_double, i64, C.c_double, C.c_uint32, vp, vp,
@@ -206,6 +207,13 @@ def green_thresholds():
     t = np.zeros(NSAMPLE, dtype=np.uint32)
     _check(lib().frw_green_thresholds(t.ctypes.data))
     return t
+
+
+def probe_read_bandwidth(nbytes, repeats):
+    """GB/s of repeated read sweeps over a device buffer of nbytes."""
+    g = C.c_double()
+    _check(lib().frw_probe_read_bandwidth(nbytes, repeats, C.byref(g)))
+    return g.value
 
 
 def weight_tables():

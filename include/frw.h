@@ -418,6 +418,13 @@ int frw_refcompat_info(const frw_tree* t, int64_t out[2]);
  * used to state the algorithmic bytes per hop. */
 int frw_walk_traffic(const frw_tree* t, const frw_walk_params* p, int64_t out[3]);
 
+/* Measurement hook (no reference counterpart): read bandwidth in GB/s of a
+ * buffer (the largest power of two of 1 KiB chunks within `bytes`) read
+ * `repeats` times over by every CU in scattered, coalesced 1 KiB chunks,
+ * after one warm-up pass.  A buffer within one XCD's L2 (4 MiB) gives the
+ * L2 read ceiling, one well beyond the 256 MiB Infinity Cache the HBM one. */
+int frw_probe_read_bandwidth(int64_t bytes, int32_t repeats, double* gbps);
+
 /* Writes <path> in the reference result format (getAvgPathLen.cpp:13-33):
  * "x0 y0 x1 y1: P%" per hit rect and the escaped percentage. */
 int frw_write_results(const frw_layout* l, const int64_t* counts_host, int64_t n_walks,

@@ -284,7 +284,7 @@ int64_t hs_walk_hinted(void* h, int master, int mode, uint64_t seed, uint32_t jo
                 const double ds = nearest_rect(v, st, px, py, traffic ? &cnt : nullptr, use_hint ? hint : -1, &arg);
                 hint = arg;
                 if (ds <= 0.0)
-                    result = q2_point(v, st, px, py, traffic ? &cnt : nullptr);
+                    result = v.ids[arg];
                 else
                     sq = biggest_square<false>(v, st, px, py, ds, nullptr, traffic ? &cnt : nullptr, &err);
             } else {
