@@ -238,17 +238,18 @@ def walks_for_1pct(f):
 
 
 def traffic_bytes(frw, tree, mlp, mode, master, walks, alpha, first):
-    """Algorithmic bytes per hop from an instrumented run: 32 B per node record
-    and 16 B per rect record the traversals read; learned mode adds 16 B per
-    pyramid probe and 64 B per feature window (8 rows x two words)."""
+    """Algorithmic bytes per hop from an instrumented run: 32 B per node record,
+    16 B per rect record and 4 B per start-table entry the searches read;
+    learned mode adds 16 B per pyramid probe and 64 B per feature window
+    (8 rows x two words)."""
     if mode == "learned":
-        nodes, rects, h, acc, fb = frw.walk_learned_traffic(tree, mlp, master, walks, seed=1,
-                                                            alpha=alpha, first_walk=first)
-        b = 32.0 * nodes + 16.0 * rects + 16.0 * (acc + fb) + 64.0 * h
+        nodes, rects, h, acc, fb, tabs = frw.walk_learned_traffic(tree, mlp, master, walks, seed=1,
+                                                                  alpha=alpha, first_walk=first)
+        b = 32.0 * nodes + 16.0 * rects + 4.0 * tabs + 16.0 * (acc + fb) + 64.0 * h
     else:
         m = frw.MODE_EXACT if mode == "exact" else frw.MODE_BISECT
-        nodes, rects, h = tree.walk_traffic(master, walks, seed=1, mode=m, first_walk=first)
-        b = 32.0 * nodes + 16.0 * rects
+        nodes, rects, h, tabs = tree.walk_traffic(master, walks, seed=1, mode=m, first_walk=first)
+        b = 32.0 * nodes + 16.0 * rects + 4.0 * tabs
     return b / max(h, 1), nodes / max(h, 1), rects / max(h, 1)
 
 

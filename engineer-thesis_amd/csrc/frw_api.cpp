@@ -162,6 +162,7 @@ void frw_tree_free(frw_tree* t)
     if (t->d_rects) hipFree(t->d_rects);
     if (t->d_ids) hipFree(t->d_ids);
     if (t->d_nodes) hipFree(t->d_nodes);
+    if (t->d_start) hipFree(t->d_start);
     if (t->d_thresholds) hipFree(t->d_thresholds);
     if (t->d_wtab) hipFree(t->d_wtab);
     if (t->d_work) hipFree(t->d_work);
@@ -199,6 +200,7 @@ int frw_tree_build(const frw_layout* l, const frw_tree_params* p, void* stream, 
     int k = 0;
     while (std::ldexp(1.0, k) <= 2.0 * cmax) ++k;
     t->margin = std::ldexp(1.0, k - 48);
+    for (const int4& r : l->rects) t->ext2_max = std::max({t->ext2_max, r.z - r.x, r.w - r.y});
     int rc = build_tree(t, l->rects, l->bounds, cap, s);
     if (rc == FRW_OK) {
         uint32_t T[kNSample];
@@ -595,7 +597,7 @@ int frw_walk(const frw_tree* t, const frw_walk_params* p, int64_t* counts_host, 
     FRW_GUARD_END
 }
 
-int frw_walk_traffic(const frw_tree* t, const frw_walk_params* p, int64_t out[3])
+int frw_walk_traffic(const frw_tree* t, const frw_walk_params* p, int64_t out[4])
 {
     int rc = check_walk_params(t, p);
     if (rc != FRW_OK) return rc;
@@ -605,18 +607,24 @@ int frw_walk_traffic(const frw_tree* t, const frw_walk_params* p, int64_t out[3]
     int64_t *d_counts = nullptr, *d_hops = nullptr, *d_tr = nullptr;
     rc = check_hip(hipMalloc(&d_counts, sizeof(int64_t) * nb), "hipMalloc counts");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hops, sizeof(int64_t)), "hipMalloc hops");
-    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_tr, 2 * sizeof(int64_t)), "hipMalloc traffic");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_tr, 3 * sizeof(int64_t)), "hipMalloc traffic");
     if (rc == FRW_OK) rc = check_hip(hipMemset(d_counts, 0, sizeof(int64_t) * nb), "memset");
     if (rc == FRW_OK) rc = check_hip(hipMemset(d_hops, 0, sizeof(int64_t)), "memset");
-    if (rc == FRW_OK) rc = check_hip(hipMemset(d_tr, 0, 2 * sizeof(int64_t)), "memset");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_tr, 0, 3 * sizeof(int64_t)), "memset");
     if (rc == FRW_OK) {
         frw_walk_outputs o{d_counts, d_hops, nullptr, nullptr};
         rc = launch_walk(t, p, &o, nullptr, (unsigned long long*)d_tr);
     }
     if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "walk");
     if (rc == FRW_OK) rc = check_err_flag(t, nullptr);
-    if (rc == FRW_OK) rc = check_hip(hipMemcpy(out, d_tr, 2 * sizeof(int64_t), hipMemcpyDeviceToHost), "copy");
+    int64_t tr[3] = {0, 0, 0};
+    if (rc == FRW_OK) rc = check_hip(hipMemcpy(tr, d_tr, 3 * sizeof(int64_t), hipMemcpyDeviceToHost), "copy");
     if (rc == FRW_OK) rc = check_hip(hipMemcpy(out + 2, d_hops, sizeof(int64_t), hipMemcpyDeviceToHost), "copy");
+    if (rc == FRW_OK) {
+        out[0] = tr[0];
+        out[1] = tr[1];
+        out[3] = tr[2];
+    }
     if (d_counts) hipFree(d_counts);
     if (d_hops) hipFree(d_hops);
     if (d_tr) hipFree(d_tr);

@@ -222,6 +222,17 @@ struct DevBuf {
 
 inline int blocks_for(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
+// One thread per start-table entry (all levels): the node of its cell.
+__global__ void start_table(const Node* __restrict__ nodes, int levels, int64_t entries, int32_t* __restrict__ tab)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= entries) return;
+    int l = 0;
+    while (l + 1 < levels && start_offset(l + 1) <= i) ++l;
+    const int64_t j = i - start_offset(l);
+    tab[i] = start_cell_node(nodes, l, (uint32_t)(j & (((int64_t)1 << l) - 1)), (uint32_t)(j >> l));
+}
+
 }  // namespace
 
 int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 B, int leaf_cap, hipStream_t s)
@@ -334,6 +345,15 @@ int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 B, int leaf_cap
     FRW_TRY(hipGetLastError());
     FRW_TRY(hipMalloc(&t->d_nodes, sizeof(Node) * n_nodes));
     FRW_TRY(hipMemcpyAsync(t->d_nodes, nodes.p, sizeof(Node) * n_nodes, hipMemcpyDeviceToDevice, s));
+    {
+        // start table of the nearest-rect search, levels 0 .. min(depth, kStartLevels) - 1
+        const int levels = std::min(depth, kStartLevels);
+        const int64_t entries = start_offset(levels);
+        FRW_TRY(hipMalloc(&t->d_start, sizeof(int32_t) * entries));
+        start_table<<<(unsigned)((entries + 255) / 256), 256, 0, s>>>(t->d_nodes, levels, entries, t->d_start);
+        FRW_TRY(hipGetLastError());
+        t->start_levels = levels;
+    }
     hipEventRecord(e1, s);
     FRW_TRY(hipStreamSynchronize(s));
     {

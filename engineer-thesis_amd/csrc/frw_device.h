@@ -186,13 +186,40 @@ struct DevTree {
     int32_t stack_cap;     // per-lane traversal stack entries
     double margin;         // ambiguity band of the Q3 schedule emulation
     double dist0;          // 0.05 * max(W, H), first Q3 step
+    // start table of the nearest-rect search (start_levels = 0: none): for
+    // Morton cell levels 0..start_levels-1, level l at entries
+    // [(4^l - 1) / 3, (4^(l+1) - 1) / 3), row-major (y, x), the node whose
+    // subtree holds every rect centred in the cell, -1 when none is
+    const int32_t* start_tab;
+    int32_t start_levels;
+    int32_t morton_bits;   // K: doubled centre coordinates have K bits per axis
+    int32_t ext2_max;      // largest rect width or height (doubled half-extent)
 };
+
+__host__ __device__ inline int64_t start_offset(int l) { return (((int64_t)1 << (2 * l)) - 1) / 3; }
+
+// The node a start-table entry holds: descend from the root along the cell's
+// quadrants (child c = x bit | y bit << 1) until level l or a leaf; -1 when
+// that node holds no rect (an empty quadrant).
+__host__ __device__ inline int32_t start_cell_node(const Node* nodes, int l, uint32_t cx, uint32_t cy)
+{
+    int ni = 0;
+    for (int lv = 0; lv < l; ++lv) {
+        const Node nd = nodes[ni];
+        if (nd.first_child < 0) break;
+        const int b = l - 1 - lv;
+        ni = nd.first_child + (int)(((cx >> b) & 1u) | (((cy >> b) & 1u) << 1));
+    }
+    const Node nd = nodes[ni];
+    return (nd.first_child < 0 && nd.rbeg == nd.rend) ? -1 : ni;
+}
 
 // Optional per-lane traffic counters: node records (32 B) and rect records
 // (16 B) read by the traversals.
 struct Counters {
     uint32_t nodes;
     uint32_t rects;
+    uint32_t tabs;   // start-table entries (4 B)
 };
 
 // Per-lane traversal stack in LDS: stack[slot * stride + lane_in_block].
@@ -264,7 +291,47 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
         if (cnt) cnt->rects++;
     }
     int sp = 0;
-    st.at(sp++) = 0;
+    if (hint >= 0 && t.start_levels > 0) {
+        // every rect within best of p has its doubled centre within
+        // w = 2 best + ext2_max of 2 (p - corner) on each axis, so the
+        // subtrees of the (at most 4) cells of that window at the deepest
+        // level where it spans at most 2 cells per axis hold all of them
+        const double w = 2.0 * best + (double)t.ext2_max;
+        const double ux = 2.0 * (px - (double)t.bounds.x), uy = 2.0 * (py - (double)t.bounds.y);
+        const int64_t kmax = ((int64_t)1 << t.morton_bits) - 1;
+        auto clampk = [&](double v) -> int64_t {
+            return v <= 0.0 ? 0 : (v >= (double)kmax ? kmax : (int64_t)v);
+        };
+        const int64_t x0 = clampk(floor(ux - w) - 1.0), x1 = clampk(ceil(ux + w) + 1.0);
+        const int64_t y0 = clampk(floor(uy - w) - 1.0), y1 = clampk(ceil(uy + w) + 1.0);
+        const int64_t span = (x1 - x0 > y1 - y0 ? x1 - x0 : y1 - y0) + 1;
+        int s = 0;
+        while (((int64_t)1 << s) < span) ++s;
+        int l = t.morton_bits - s;
+        if (l < 0) l = 0;
+        if (l > t.start_levels - 1) l = t.start_levels - 1;
+        const int shift = t.morton_bits - l;
+        const int32_t* tab = t.start_tab + start_offset(l);
+        const int64_t cx0 = x0 >> shift, cx1 = x1 >> shift, cy0 = y0 >> shift, cy1 = y1 >> shift;
+        const int64_t pcx = clampk(floor(ux)) >> shift, pcy = clampk(floor(uy)) >> shift;
+        const int own = tab[(pcy << l) + pcx];
+        if (cnt) cnt->tabs++;
+        int cand[3];
+        int nc = 0;
+        for (int k = 0; k < 4; ++k) {
+            const int64_t qx = (k & 1) ? cx1 : cx0, qy = (k & 2) ? cy1 : cy0;
+            if (((k & 1) && cx1 == cx0) || ((k & 2) && cy1 == cy0) || (qx == pcx && qy == pcy)) continue;
+            const int ni = tab[(qy << l) + qx];
+            if (cnt) cnt->tabs++;
+            bool dup = ni < 0 || ni == own;
+            for (int j = 0; j < nc; ++j) dup = dup || cand[j] == ni;
+            if (!dup) cand[nc++] = ni;
+        }
+        for (int j = 0; j < nc; ++j) st.at(sp++) = cand[j];
+        if (own >= 0) st.at(sp++) = own;  // p's cell first
+    } else {
+        st.at(sp++) = 0;
+    }
     while (sp > 0) {
         const int ni = st.at(--sp);
         const Node nd = t.nodes[ni];

@@ -106,6 +106,9 @@ struct frw_tree {
     int4* d_rects = nullptr;       // tree order
     int32_t* d_ids = nullptr;
     frw::Node* d_nodes = nullptr;
+    int32_t* d_start = nullptr;        // start table of the nearest-rect search
+    int32_t start_levels = 0;
+    int32_t ext2_max = 0;              // largest rect width or height
     uint32_t* d_thresholds = nullptr;  // Green CDF panel thresholds
     double* d_wtab = nullptr;          // weighted estimator: gx | gy | pw | qx | qy (5 x 200), then
                                        // the importance thresholds (2 x 200 u32)

@@ -923,7 +923,7 @@ int frw_walk_learned(const frw_tree* t, const frw_mlp* m, const frw_walk_params*
 }
 
 int frw_walk_learned_traffic(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p, float alpha,
-                             int64_t out[5])
+                             int64_t out[6])
 {
     int rc = check_learned(t, m, p, alpha);
     if (rc != FRW_OK) return rc;
@@ -932,16 +932,22 @@ int frw_walk_learned_traffic(const frw_tree* t, const frw_mlp* m, const frw_walk
     int64_t *d_counts = nullptr, *d_hops = nullptr, *d_tr = nullptr;
     rc = check_hip(hipMalloc(&d_counts, sizeof(int64_t) * nb), "hipMalloc counts");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hops, sizeof(int64_t)), "hipMalloc hops");
-    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_tr, 2 * sizeof(int64_t)), "hipMalloc traffic");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_tr, 3 * sizeof(int64_t)), "hipMalloc traffic");
     if (rc == FRW_OK) rc = check_hip(hipMemset(d_counts, 0, sizeof(int64_t) * nb), "memset");
     if (rc == FRW_OK) rc = check_hip(hipMemset(d_hops, 0, sizeof(int64_t)), "memset");
-    if (rc == FRW_OK) rc = check_hip(hipMemset(d_tr, 0, 2 * sizeof(int64_t)), "memset");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_tr, 0, 3 * sizeof(int64_t)), "memset");
     if (rc == FRW_OK) {
         frw_walk_outputs o{d_counts, d_hops, nullptr, nullptr};
         rc = learned_sync(t, m, p, &o, alpha, (unsigned long long*)d_tr, out + 3);
     }
-    if (rc == FRW_OK) rc = check_hip(hipMemcpy(out, d_tr, 2 * sizeof(int64_t), hipMemcpyDeviceToHost), "copy");
+    int64_t tr[3] = {0, 0, 0};
+    if (rc == FRW_OK) rc = check_hip(hipMemcpy(tr, d_tr, 3 * sizeof(int64_t), hipMemcpyDeviceToHost), "copy");
     if (rc == FRW_OK) rc = check_hip(hipMemcpy(out + 2, d_hops, sizeof(int64_t), hipMemcpyDeviceToHost), "copy");
+    if (rc == FRW_OK) {
+        out[0] = tr[0];
+        out[1] = tr[1];
+        out[5] = tr[2];
+    }
     if (d_counts) hipFree(d_counts);
     if (d_hops) hipFree(d_hops);
     if (d_tr) hipFree(d_tr);

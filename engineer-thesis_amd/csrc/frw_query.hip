@@ -124,6 +124,10 @@ DevTree dev_view(const frw_tree* t)
     v.stack_cap = 3 * t->depth + 2;
     v.margin = t->margin;
     v.dist0 = t->dist0;
+    v.start_tab = t->d_start;
+    v.start_levels = t->d_start ? t->start_levels : 0;
+    v.morton_bits = t->morton_bits;
+    v.ext2_max = t->ext2_max;
     return v;
 }
 

@@ -9,6 +9,7 @@ namespace frw {
 constexpr int kNSample = 200;        // panels per square (randomWalk/defines.h:25)
 constexpr int kMaxBisect = 4096;     // safety cap on one biggest-square search
 constexpr int kWaveSize = 64;
+constexpr int kStartLevels = 11;     // Morton levels 0..10 in the nearest-search start table (5.6 MB)
 
 // Error bits reported by kernels.
 constexpr int kErrBisect = 1;        // biggest-square search exceeded kMaxBisect

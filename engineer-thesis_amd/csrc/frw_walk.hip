@@ -34,7 +34,7 @@ struct WalkArgs {
     const uint32_t* thresholds;
     unsigned long long* work;
     int* err;
-    unsigned long long* traffic;  // kStats: node records, rect records
+    unsigned long long* traffic;  // kStats: node records, rect records, start-table entries
     uint64_t seed;
     uint64_t total_walks;
     uint32_t max_hops;
@@ -144,9 +144,9 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
     double wgt = 0.0, fpx = 0.0, fpy = 0.0;
     bool first_pt = false, start_stuck = false;
     double wm_s = 0.0, wm_c = 0.0, wm_q = 0.0, we_s = 0.0, we_c = 0.0, we_q = 0.0;
-    Counters cnt{0u, 0u};
+    Counters cnt{0u, 0u, 0u};
     Counters* cp = kStats ? &cnt : nullptr;
-    unsigned long long my_nodes = 0, my_rects = 0, my_acc = 0, my_fb = 0;
+    unsigned long long my_nodes = 0, my_rects = 0, my_tabs = 0, my_acc = 0, my_fb = 0;
 
     auto flush = [&]() {
         if (my_master) atomicAdd(&jobs[j].counts[mid], (unsigned long long)my_master);
@@ -362,7 +362,8 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
             if (kStats) {
                 my_nodes += cnt.nodes;
                 my_rects += cnt.rects;
-                cnt.nodes = cnt.rects = 0u;
+                my_tabs += cnt.tabs;
+                cnt.nodes = cnt.rects = cnt.tabs = 0u;
             }
             hop++;
             if (result == -3 && hop >= a.max_hops) result = -2;
@@ -420,10 +421,12 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
         for (int off = 32; off >= 1; off >>= 1) {
             my_nodes += __shfl_xor(my_nodes, off);
             my_rects += __shfl_xor(my_rects, off);
+            my_tabs += __shfl_xor(my_tabs, off);
         }
         if (lane == 0) {
             atomicAdd(&a.traffic[0], my_nodes);
             atomicAdd(&a.traffic[1], my_rects);
+            atomicAdd(&a.traffic[2], my_tabs);
         }
     }
     if (kMode == kModeLearned) {

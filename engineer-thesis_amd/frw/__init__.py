@@ -385,10 +385,11 @@ class Tree:
 
     def walk_traffic(self, master, n_walks, seed=1, mode=MODE_EXACT, job=0, first_walk=0,
                      max_hops=0, block=0, grid=0):
-        """Instrumented walk: (node records read, rect records read, hops)."""
+        """Instrumented walk: (node records read, rect records read, hops,
+        start-table entries read)."""
         p = WalkParams(master=master, mode=mode, n_walks=n_walks, first_walk=first_walk, seed=seed,
                        job=job, block=block, grid=grid, reserved=0, max_hops=max_hops)
-        out = np.zeros(3, dtype=np.int64)
+        out = np.zeros(4, dtype=np.int64)
         _check(lib().frw_walk_traffic(self.h, C.byref(p), out.ctypes.data))
         return tuple(int(v) for v in out)
 
@@ -643,9 +644,10 @@ def walk_learned_async(tree, mlp, master, n_walks, counts, hops, stats=None, hit
 
 
 def walk_learned_traffic(tree, mlp, master, n_walks, seed=1, alpha=0.8, first_walk=0, job=0):
-    """Instrumented learned walks: (node records, rect records, hops, accepted, fallback)."""
+    """Instrumented learned walks: (node records, rect records, hops, accepted, fallback,
+    start-table entries)."""
     p = WalkParams(master=master, mode=MODE_LEARNED, n_walks=n_walks, first_walk=first_walk,
                    seed=seed, job=job, block=0, grid=0, reserved=0, max_hops=0)
-    out = np.zeros(5, dtype=np.int64)
+    out = np.zeros(6, dtype=np.int64)
     _check(lib().frw_walk_learned_traffic(tree.h, mlp.h, C.byref(p), alpha, out.ctypes.data))
     return tuple(int(v) for v in out)

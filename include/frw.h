@@ -309,9 +309,10 @@ int frw_walk_learned(const frw_tree* t, const frw_mlp* m, const frw_walk_params*
 int frw_walk_learned_async(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p,
                            const frw_walk_outputs* o, float alpha, uint64_t* stats_dev, void* stream);
 /* Measurement hook: learned walks with traffic counters compiled in.
- * out = {node records read, rect records read, hops, accepted, fallbacks}. */
+ * out = {node records read, rect records read, hops, accepted, fallbacks,
+ * start-table entries read}. */
 int frw_walk_learned_traffic(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p, float alpha,
-                             int64_t out[5]);
+                             int64_t out[6]);
 
 /* Learned walks of a batch.  frw_batch_set_proposal gives each tree of the
  * batch its occupancy pyramid (per_tree[i] must have been created for tree
@@ -414,9 +415,10 @@ int frw_refcompat_info(const frw_tree* t, int64_t out[2]);
 
 /* Measurement hook (no reference counterpart): the same walks as frw_walk
  * with per-hop traffic counters compiled in.  out = {node records read
- * (32 B each), rect records read (16 B each), hops}.  Slower than frw_walk;
- * used to state the algorithmic bytes per hop. */
-int frw_walk_traffic(const frw_tree* t, const frw_walk_params* p, int64_t out[3]);
+ * (32 B each), rect records read (16 B each), hops, start-table entries read
+ * (4 B each)}.  Slower than frw_walk; used to state the algorithmic bytes per
+ * hop. */
+int frw_walk_traffic(const frw_tree* t, const frw_walk_params* p, int64_t out[4]);
 
 /* Measurement hook (no reference counterpart): read bandwidth in GB/s of a
  * buffer (the largest power of two of 1 KiB chunks within `bytes`) read

@@ -23,10 +23,11 @@ uint64_t spread(uint32_t v)
 
 struct HostTree {
     int4 bounds;
-    int K = 0, depth = 0;
+    int K = 0, depth = 0, ext2_max = 0, start_levels = 0;
     std::vector<int4> rects;
     std::vector<int32_t> ids;
     std::vector<Node> nodes;
+    std::vector<int32_t> start;
     double margin = 0, dist0 = 0;
 };
 
@@ -127,6 +128,14 @@ void build(HostTree& t, const int32_t* xyxy, int n, const int32_t* B, int cap)
     int k = 0;
     while (std::ldexp(1.0, k) <= 2.0 * cmax) ++k;
     t.margin = std::ldexp(1.0, k - 48);
+    for (int i = 0; i < n; ++i)
+        t.ext2_max = std::max({t.ext2_max, xyxy[4 * i + 2] - xyxy[4 * i], xyxy[4 * i + 3] - xyxy[4 * i + 1]});
+    t.start_levels = std::min(t.depth, kStartLevels);
+    t.start.assign((size_t)start_offset(t.start_levels), -1);
+    for (int l = 0; l < t.start_levels; ++l)
+        for (uint32_t cy = 0; cy < (1u << l); ++cy)
+            for (uint32_t cx = 0; cx < (1u << l); ++cx)
+                t.start[(size_t)(start_offset(l) + ((int64_t)cy << l) + cx)] = start_cell_node(t.nodes.data(), l, cx, cy);
 }
 
 DevTree view(const HostTree& t)
@@ -142,6 +151,10 @@ DevTree view(const HostTree& t)
     v.stack_cap = 3 * t.depth + 2;
     v.margin = t.margin;
     v.dist0 = t.dist0;
+    v.start_tab = t.start.data();
+    v.start_levels = t.start_levels;
+    v.morton_bits = t.K;
+    v.ext2_max = t.ext2_max;
     return v;
 }
 
@@ -257,7 +270,7 @@ int64_t hs_walk_hinted(void* h, int master, int mode, uint64_t seed, uint32_t jo
     hs_gauss(h, master, surf);
     int err = 0;
     int64_t total = 0;
-    Counters cnt{0, 0};
+    Counters cnt{0, 0, 0};
     for (int64_t w = 0; w < n_walks; ++w) {
         DBox sq{surf[0], surf[1], surf[2], surf[3]};
         uint32_t rnd[4] = {0, 0, 0, 0};
@@ -302,6 +315,7 @@ int64_t hs_walk_hinted(void* h, int master, int mode, uint64_t seed, uint32_t jo
     if (traffic) {
         traffic[0] = cnt.nodes;
         traffic[1] = cnt.rects;
+        traffic[2] = cnt.tabs;
     }
     return err ? -1 : total;
 }

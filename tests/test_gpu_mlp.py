@@ -289,8 +289,8 @@ def test_learned_walk_distribution_and_gain(setup, trained):
     tl = frw.walk_learned_traffic(tree, mlp2, 0, m, seed=23)
     te = tree.walk_traffic(0, m, seed=23)
     # tree records of the fallback searches + one 16-byte table probe per hop
-    bytes_l = 32 * tl[0] + 16 * tl[1] + 16 * (tl[3] + tl[4])
-    bytes_e = 32 * te[0] + 16 * te[1]
+    bytes_l = 32 * tl[0] + 16 * tl[1] + 16 * (tl[3] + tl[4]) + 4 * tl[5]
+    bytes_e = 32 * te[0] + 16 * te[1] + 4 * te[3]
     print(f"\nlearned: hops/walk {tl[2] / m:.2f} accept {tl[3] / max(1, tl[2]):.3f} "
           f"bytes/walk {bytes_l / m:.0f}; exact: hops/walk {te[2] / m:.2f} bytes/walk {bytes_e / m:.0f}")
     assert bytes_l < bytes_e
@@ -302,10 +302,10 @@ def test_learned_alpha_sweep(setup, trained):
     mlp2, first, last = trained
     print(f"\ntraining loss: first epoch {first:.4f}, last {last:.4f}")
     te = tree.walk_traffic(0, 20_000, seed=23)
-    print(f"exact: hops/walk {te[2] / 20_000:.2f} bytes/walk {(32 * te[0] + 16 * te[1]) / 20_000:.0f}")
+    print(f"exact: hops/walk {te[2] / 20_000:.2f} bytes/walk {(32 * te[0] + 16 * te[1] + 4 * te[3]) / 20_000:.0f}")
     for alpha in (0.5, 0.6, 0.7, 0.8, 0.9, 1.0):
         tl = frw.walk_learned_traffic(tree, mlp2, 0, 20_000, seed=23, alpha=alpha)
-        b = 32 * tl[0] + 16 * tl[1] + 16 * (tl[3] + tl[4])
+        b = 32 * tl[0] + 16 * tl[1] + 16 * (tl[3] + tl[4]) + 4 * tl[5]
         print(f"alpha {alpha}: hops/walk {tl[2] / 20_000:.2f} accept {tl[3] / max(1, tl[2]):.3f} "
               f"bytes/walk {b / 20_000:.0f}")
         assert tl[3] + tl[4] == tl[2]

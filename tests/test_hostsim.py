@@ -83,17 +83,17 @@ class HostTree:
         return hit, hp, tot
 
     def walk_counted(self, master, n, mode=0, seed=1, job=0, first=0, use_hint=True, max_hops=10 ** 7):
-        """(node records read, rect records read, hops) of the same walks, the
-        exact mode seeding each search with the previous hop's nearest rect
-        as the walk kernel does."""
+        """(node records read, rect records read, hops, start-table entries
+        read) of the same walks, the exact mode seeding each search with the
+        previous hop's nearest rect as the walk kernel does."""
         T = frw.green_thresholds()
         hit = np.zeros(n, np.int32)
         hp = np.zeros(n, np.int32)
-        tr = np.zeros(2, np.int64)
+        tr = np.zeros(3, np.int64)
         tot = self.L.hs_walk_hinted(self.h, master, mode, seed, job, first, n, max_hops, T.ctypes.data,
                                     hit.ctypes.data, hp.ctypes.data, tr.ctypes.data, int(use_hint))
         assert tot >= 0, "bisection error flag"
-        return int(tr[0]), int(tr[1]), int(tot)
+        return int(tr[0]), int(tr[1]), int(tot), int(tr[2])
 
 
 @pytest.mark.parametrize("name,master", FIXTURES)
