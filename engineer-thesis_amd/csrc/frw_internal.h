@@ -27,6 +27,8 @@ struct WalkJob {
     uint32_t first_walk;           // Philox walk id of the job's first walk
     uint32_t job;                  // Philox counter word 1
     int32_t master_id;             // original index of the master (its hits are counted per lane)
+    int32_t start_hint;            // sorted index of a rect at the master's centre (prologue): the
+                                   // first hop's nearest-rect search starts from its distance
     double* wsum;                  // weighted mode: per-bin sums of weights and of squared weights
     double* wsq;
     double* w_out;                 // weighted mode: optional per-walk weights

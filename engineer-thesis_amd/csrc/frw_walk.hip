@@ -99,6 +99,13 @@ __global__ void __launch_bounds__(64) walk_prologue(WalkJob* jobs, int n_jobs, u
     jb.surface[1] = s.y0;
     jb.surface[2] = s.x1;
     jb.surface[3] = s.y1;
+    // a rect at the master's centre (the master, or one overlapping it): its
+    // distance bounds the first hop's search, as the previous hop's nearest
+    // rect bounds the later ones
+    int arg = -1;
+    (void)nearest_rect(jb.tree, st, ((double)jb.master.x + (double)jb.master.z) / 2.0,
+                       ((double)jb.master.y + (double)jb.master.w) / 2.0, nullptr, -1, &arg);
+    jb.start_hint = arg;
 }
 
 template <int kMode, bool kStats, bool kHarvest>
@@ -197,7 +204,6 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
         if (got) {
             active = true;
             hop = 0;
-            hint = -1;
             const int nj = a.n_jobs == 1 ? 0 : find_job(jobs, s_begin, a.n_jobs, g);
             if (nj != j) {
                 if (my_hops) atomicAdd(jobs[j].hops, my_hops);
@@ -209,6 +215,7 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
                 if (kMode == kModeLearned) pyr = jobs[j].pyr;
             }
             w = g - jobs[j].walk_begin;
+            hint = jobs[j].start_hint;
             sq = DBox{jobs[j].surface[0], jobs[j].surface[1], jobs[j].surface[2], jobs[j].surface[3]};
             if (kMode == kModeWeighted) {
                 // start: a uniform point r on the Gaussian surface G (outward
@@ -230,7 +237,7 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
                 } else {
                     rx = sq.x0; ry = sq.y1 - (tp - 2.0 * gw - gh); nx = -1.0; ny = 0.0;
                 }
-                double hh = nearest_rect(t, st, rx, ry, cp);
+                double hh = nearest_rect(t, st, rx, ry, cp, hint);
                 hh = fmin(hh, fmin(fmin(rx - (double)t.bounds.x, (double)t.bounds.z - rx),
                                    fmin(ry - (double)t.bounds.y, (double)t.bounds.w - ry)));
                 wgt = 0.0;

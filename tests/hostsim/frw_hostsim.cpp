@@ -268,6 +268,15 @@ int64_t hs_walk_hinted(void* h, int master, int mode, uint64_t seed, uint32_t jo
     LaneStack st{stack.data(), 1};
     double surf[4];
     hs_gauss(h, master, surf);
+    // the walk kernel's first-hop hint: a rect at the master's centre
+    int start_hint = -1;
+    if (use_hint) {
+        int4 m = make_int4(0, 0, 0, 0);
+        for (size_t i = 0; i < t->ids.size(); ++i)
+            if (t->ids[i] == master) m = t->rects[i];
+        (void)nearest_rect(v, st, ((double)m.x + (double)m.z) / 2.0, ((double)m.y + (double)m.w) / 2.0, nullptr,
+                           -1, &start_hint);
+    }
     int err = 0;
     int64_t total = 0;
     Counters cnt{0, 0, 0};
@@ -276,7 +285,7 @@ int64_t hs_walk_hinted(void* h, int master, int mode, uint64_t seed, uint32_t jo
         uint32_t rnd[4] = {0, 0, 0, 0};
         uint32_t hop = 0;
         int result = -3;
-        int hint = -1;
+        int hint = start_hint;
         while (result == -3) {
             if ((hop & 3u) == 0u) {
                 rnd[0] = first_walk + (uint32_t)w;
