@@ -423,18 +423,24 @@ __host__ __device__ inline DBox biggest_square(const DevTree& t, const LaneStack
     // of h (one rounding per coordinate update, per h update and per
     // difference; coordinates and h below 2^k), so a decision taken on h
     // with the band 8 m = 2^(k-45) > m + that error is the decision of the
-    // four-extent test below; inside the band the four-extent test runs.
-    // (the thresholds' own rounding, below 2^(k-53), fits in the band's slack)
-    const double h_hit = dstar + 8.0 * t.margin, h_free = dstar - 8.0 * t.margin;
+    // exact tests below; inside the band the exact tests run.  The box meets
+    // a rect when its half-extent reaches d* and leaves the bounds when it
+    // exceeds the distance dB from p to the bounds, so one pair of thresholds
+    // on min(d*, dB) decides both tests.  (the thresholds' own rounding,
+    // below 2^(k-53), fits in the band's slack)
+    const double dB = fmin(fmin(px - (double)t.bounds.x, (double)t.bounds.z - px),
+                           fmin(py - (double)t.bounds.y, (double)t.bounds.w - py));
+    const double dmin = fmin(dstar, dB);
+    const double h_hit = dmin + 8.0 * t.margin, h_free = dmin - 8.0 * t.margin;
     double h = 1.0;
     int step = 0;
     auto collides = [&](double a, double c, double b, double e) -> bool {
+        if (!kLiteral && step < 60) {
+            if (h >= h_hit) return true;
+            if (h < h_free) return false;
+        }
         if (!in_bounds_box(t.bounds, a, c, b, e)) return true;
         if (!kLiteral) {
-            if (step < 60) {
-                if (h >= h_hit) return true;
-                if (h < h_free) return false;
-            }
             const double lo = fmin(fmin(px - a, b - px), fmin(py - c, e - py));
             const double hi = fmax(fmax(px - a, b - px), fmax(py - c, e - py));
             if (lo >= dstar + t.margin) return true;
@@ -459,16 +465,15 @@ __host__ __device__ inline DBox biggest_square(const DevTree& t, const LaneStack
             if (err) raise_error(err, kErrBisect);
             break;
         }
-        if (c) {
-            if (dist > MIN_DIST) dist /= 2;
-            max_reached = true;
-            o.x0 += dist; o.y0 += dist; o.x1 -= dist; o.y1 -= dist;
-            h -= dist;
-        } else {
-            if (max_reached) dist /= 2;
-            o.x0 -= dist; o.y0 -= dist; o.x1 += dist; o.y1 += dist;
-            h += dist;
-        }
+        // shrink by dist on a collision, grow otherwise; dist halves first
+        // (after a collision while above MIN_DIST, after a free step once a
+        // collision has been seen).  a + (-d) and a - (-d) round as a - d and
+        // a + d, so the signed step gives the same coordinates branch-free.
+        if (c ? dist > MIN_DIST : max_reached) dist /= 2;
+        max_reached = max_reached || c;
+        const double sd = c ? dist : -dist;
+        o.x0 += sd; o.y0 += sd; o.x1 -= sd; o.y1 -= sd;
+        h -= sd;
     }
     return o;
 }
