@@ -314,18 +314,29 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
         const int32_t* tab = t.start_tab + start_offset(l);
         const int64_t cx0 = x0 >> shift, cx1 = x1 >> shift, cy0 = y0 >> shift, cy1 = y1 >> shift;
         const int64_t pcx = clampk(floor(ux)) >> shift, pcy = clampk(floor(uy)) >> shift;
-        const int own = tab[(pcy << l) + pcx];
-        if (cnt) cnt->tabs++;
-        int cand[3];
-        int nc = 0;
+        // p's cell, then the window's other cells; all entries loaded at once
+        int64_t pos[4];
+        int np = 0;
+        pos[np++] = (pcy << l) + pcx;
+#pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int64_t qx = (k & 1) ? cx1 : cx0, qy = (k & 2) ? cy1 : cy0;
             if (((k & 1) && cx1 == cx0) || ((k & 2) && cy1 == cy0) || (qx == pcx && qy == pcy)) continue;
-            const int ni = tab[(qy << l) + qx];
-            if (cnt) cnt->tabs++;
-            bool dup = ni < 0 || ni == own;
-            for (int j = 0; j < nc; ++j) dup = dup || cand[j] == ni;
-            if (!dup) cand[nc++] = ni;
+            pos[np++] = (qy << l) + qx;
+        }
+        int32_t e[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) e[k] = tab[pos[k < np ? k : 0]];
+        if (cnt) cnt->tabs += (uint32_t)np;
+        const int own = e[0];
+        int cand[3];
+        int nc = 0;
+#pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            if (k >= np) break;
+            bool dup = e[k] < 0 || e[k] == own;
+            for (int j = 0; j < nc; ++j) dup = dup || cand[j] == e[k];
+            if (!dup) cand[nc++] = e[k];
         }
         for (int j = 0; j < nc; ++j) st.at(sp++) = cand[j];
         if (own >= 0) st.at(sp++) = own;  // p's cell first
@@ -339,12 +350,22 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
         if (!(linf_lb(nd.box, fx, cx, fy, cy) < lim)) continue;
         if (cnt) cnt->rects += (uint32_t)(nd.rend - nd.rbeg);
         if (cnt && nd.first_child >= 0) cnt->nodes += 4u;
-        for (int r = nd.rbeg; r < nd.rend; ++r) {
-            const double d = linf(t.rects[r], px, py);
-            if (d < best || (d == best && d <= 0.0 && t.ids[r] < t.ids[besti])) {
-                best = d;
-                lim = ceil_int(d) + (d <= 0.0 ? 1 : 0);
-                besti = r;
+        // the leaf's rects four at a time: four loads in flight, then the
+        // distances in order (the tail re-reads the last rect and skips it)
+        for (int r0 = nd.rbeg; r0 < nd.rend; r0 += 4) {
+            int4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = t.rects[r0 + u < nd.rend ? r0 + u : nd.rend - 1];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int r = r0 + u;
+                if (r >= nd.rend) break;
+                const double d = linf(q[u], px, py);
+                if (d < best || (d == best && d <= 0.0 && t.ids[r] < t.ids[besti])) {
+                    best = d;
+                    lim = ceil_int(d) + (d <= 0.0 ? 1 : 0);
+                    besti = r;
+                }
             }
         }
         if (nd.first_child >= 0) {
