@@ -269,9 +269,11 @@ __host__ __device__ inline bool q1_box(const DevTree& t, const LaneStack& st, do
 //
 // `hint` (a sorted rect index, or -1) seeds the bound with that rect's
 // distance, typically the rect nearest to the walker's previous point; *arg
-// (optional) receives the sorted index of a nearest rect.
+// (optional) receives the sorted index of a nearest rect.  With hint_rect
+// the hint's coordinates come from (and the nearest rect's go back to) the
+// caller's registers instead of a load.
 __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack& st, double px, double py,
-                                      Counters* cnt, int hint = -1, int* arg = nullptr)
+                                      Counters* cnt, int hint = -1, int* arg = nullptr, int4* hint_rect = nullptr)
 {
     // node pruning and ordering use integer lower bounds (an integer lb is
     // below best exactly when it is below ceil(best)); rect distances are exact.
@@ -284,11 +286,17 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
     double best = __builtin_inf();
     int lim = 2147483647;
     int besti = -1;
+    int4 bestr = make_int4(0, 0, 0, 0);
     if (hint >= 0) {
-        best = linf(t.rects[hint], px, py);
+        if (hint_rect) {
+            bestr = *hint_rect;
+        } else {
+            bestr = t.rects[hint];
+            if (cnt) cnt->rects++;
+        }
+        best = linf(bestr, px, py);
         lim = ceil_int(best) + (best <= 0.0 ? 1 : 0);
         besti = hint;
-        if (cnt) cnt->rects++;
     }
     int sp = 0;
     if (hint >= 0 && t.start_levels > 0) {
@@ -365,6 +373,7 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
                     best = d;
                     lim = ceil_int(d) + (d <= 0.0 ? 1 : 0);
                     besti = r;
+                    bestr = q[u];
                 }
             }
         }
@@ -384,6 +393,7 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
         }
     }
     if (arg) *arg = besti;
+    if (hint_rect) *hint_rect = bestr;
     return best;
 }
 

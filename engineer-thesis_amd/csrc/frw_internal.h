@@ -29,6 +29,7 @@ struct WalkJob {
     int32_t master_id;             // original index of the master (its hits are counted per lane)
     int32_t start_hint;            // sorted index of a rect at the master's centre (prologue): the
                                    // first hop's nearest-rect search starts from its distance
+    int4 start_rect;               // that rect
     double* wsum;                  // weighted mode: per-bin sums of weights and of squared weights
     double* wsq;
     double* w_out;                 // weighted mode: optional per-walk weights

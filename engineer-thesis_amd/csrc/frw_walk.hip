@@ -106,6 +106,7 @@ __global__ void __launch_bounds__(64) walk_prologue(WalkJob* jobs, int n_jobs, u
     (void)nearest_rect(jb.tree, st, ((double)jb.master.x + (double)jb.master.z) / 2.0,
                        ((double)jb.master.y + (double)jb.master.w) / 2.0, nullptr, -1, &arg);
     jb.start_hint = arg;
+    jb.start_rect = arg >= 0 ? jb.tree.rects[arg] : make_int4(0, 0, 0, 0);
 }
 
 template <int kMode, bool kStats, bool kHarvest>
@@ -133,6 +134,7 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
     uint64_t w = 0;  // walk index within the job
     uint32_t hop = 0;
     int hint = -1;  // sorted index of the rect nearest to the walker's previous point
+    int4 hint_r = make_int4(0, 0, 0, 0);  // and its coordinates
     DevTree t = jobs[0].tree;
     int mid = jobs[0].master_id;
     MlpPyramid pyr;
@@ -216,6 +218,7 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
             }
             w = g - jobs[j].walk_begin;
             hint = jobs[j].start_hint;
+            hint_r = jobs[j].start_rect;
             sq = DBox{jobs[j].surface[0], jobs[j].surface[1], jobs[j].surface[2], jobs[j].surface[3]};
             if (kMode == kModeWeighted) {
                 // start: a uniform point r on the Gaussian surface G (outward
@@ -237,7 +240,9 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
                 } else {
                     rx = sq.x0; ry = sq.y1 - (tp - 2.0 * gw - gh); nx = -1.0; ny = 0.0;
                 }
-                double hh = nearest_rect(t, st, rx, ry, cp, hint);
+                int ha = -1;
+                double hh = nearest_rect(t, st, rx, ry, cp, hint, &ha, &hint_r);
+                hint = ha;
                 hh = fmin(hh, fmin(fmin(rx - (double)t.bounds.x, (double)t.bounds.z - rx),
                                    fmin(ry - (double)t.bounds.y, (double)t.bounds.w - ry)));
                 wgt = 0.0;
@@ -343,7 +348,7 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ j
             if (!proposed) {
                 if (kMode != kModeBisect) {
                     int arg = -1;
-                    const double ds = nearest_rect(t, st, px, py, cp, hint, &arg);
+                    const double ds = nearest_rect(t, st, px, py, cp, hint, &arg, &hint_r);
                     hint = arg;
                     if (ds <= 0.0) {
                         // p lies in a rect: the search's arg is the point query's answer

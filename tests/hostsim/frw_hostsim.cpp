@@ -270,12 +270,14 @@ int64_t hs_walk_hinted(void* h, int master, int mode, uint64_t seed, uint32_t jo
     hs_gauss(h, master, surf);
     // the walk kernel's first-hop hint: a rect at the master's centre
     int start_hint = -1;
+    int4 start_rect = make_int4(0, 0, 0, 0);
     if (use_hint) {
         int4 m = make_int4(0, 0, 0, 0);
         for (size_t i = 0; i < t->ids.size(); ++i)
             if (t->ids[i] == master) m = t->rects[i];
         (void)nearest_rect(v, st, ((double)m.x + (double)m.z) / 2.0, ((double)m.y + (double)m.w) / 2.0, nullptr,
                            -1, &start_hint);
+        start_rect = t->rects[start_hint];
     }
     int err = 0;
     int64_t total = 0;
@@ -286,6 +288,7 @@ int64_t hs_walk_hinted(void* h, int master, int mode, uint64_t seed, uint32_t jo
         uint32_t hop = 0;
         int result = -3;
         int hint = start_hint;
+        int4 hint_r = start_rect;
         while (result == -3) {
             if ((hop & 3u) == 0u) {
                 rnd[0] = first_walk + (uint32_t)w;
@@ -303,7 +306,8 @@ int64_t hs_walk_hinted(void* h, int master, int mode, uint64_t seed, uint32_t jo
             }
             if (mode == kModeExact) {
                 int arg = -1;
-                const double ds = nearest_rect(v, st, px, py, traffic ? &cnt : nullptr, use_hint ? hint : -1, &arg);
+                const double ds = nearest_rect(v, st, px, py, traffic ? &cnt : nullptr, use_hint ? hint : -1, &arg,
+                                               use_hint ? &hint_r : nullptr);
                 hint = arg;
                 if (ds <= 0.0)
                     result = v.ids[arg];
