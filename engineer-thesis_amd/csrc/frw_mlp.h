@@ -92,12 +92,29 @@ __host__ __device__ inline uint16_t f2bf(float x)
 
 __host__ __device__ inline float bf2f(uint16_t b) { return __builtin_bit_cast(float, (uint32_t)b << 16); }
 
-__device__ inline float gelu_f(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// erf by Abramowitz & Stegun 7.1.26 (Hastings' rational approximation,
+// |error| <= 1.5e-7 in exact arithmetic, well inside a bf16 activation's
+// rounding): erf(x) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2)
+// with t = 1 / (1 + p x) for x >= 0, odd in x.  One reciprocal and one
+// exponential instead of a full-accuracy erf.
+__device__ inline float erf_as(float x)
+{
+    const float ax = fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(0.3275911f, ax, 1.0f));
+    float poly = __builtin_fmaf(t, 1.061405429f, -1.453152027f);
+    poly = __builtin_fmaf(t, poly, 1.421413741f);
+    poly = __builtin_fmaf(t, poly, -0.284496736f);
+    poly = __builtin_fmaf(t, poly, 0.254829592f);
+    const float e = __builtin_fmaf(-t * poly, __expf(-ax * ax), 1.0f);
+    return copysignf(e, x);
+}
+
+__device__ inline float gelu_f(float z) { return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f)); }
 
 __device__ inline float gelu_grad_f(float z)
 {
-    const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
-    const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
+    const float cdf = 0.5f * (1.0f + erf_as(z * 0.70710678118654752f));
+    const float pdf = 0.39894228040143268f * __expf(-0.5f * z * z);
     return cdf + z * pdf;
 }
 
@@ -191,9 +208,9 @@ __device__ inline void wave_sync()
 }
 
 // One hidden layer for the wave's 64 rows: out = bf16(gelu(in * W^T + b)).
-// Four 32x32 output tiles, one at a time (one 16-register accumulator live),
-// K/8 MFMA steps each.  Lane l feeds row/column l%32 and k-quad l/32; its 16
-// results sit at rows 8*(r/4) + 4*(l/32) + r%4 of column l%32.
+// Computed transposed (W * in^T, weights as the MFMA's A operand): each lane
+// then holds four consecutive output neurons of one row per group, stored
+// as one packed 8-byte write.
 template <int K>
 __device__ inline void mlp_layer(const uint16_t* in, uint16_t* out, const uint16_t* W, const float* bias,
                                  int lane)
@@ -202,21 +219,23 @@ __device__ inline void mlp_layer(const uint16_t* in, uint16_t* out, const uint16
 #pragma unroll 1
     for (int t = 0; t < 4; ++t) {
         const int mi = t >> 1, nj = t & 1;
-        const uint16_t* arow = in + (mi * 32 + r32) * kMlpStride + 4 * q;
-        const uint16_t* brow = W + (nj * 32 + r32) * K + 4 * q;
+        const uint16_t* xrow = in + (mi * 32 + r32) * kMlpStride + 4 * q;
+        const uint16_t* wrow = W + (nj * 32 + r32) * K + 4 * q;
         f32x16 acc = {0};
-#pragma unroll
+#pragma unroll 3
         for (int kb = 0; kb < K / 8; ++kb) {
-            const bf16x4 a = *(const bf16x4*)(arow + kb * 8);
-            const bf16x4 b = *(const bf16x4*)(brow + kb * 8);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+            const bf16x4 x = *(const bf16x4*)(xrow + kb * 8);
+            const bf16x4 w = *(const bf16x4*)(wrow + kb * 8);
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w, x, acc, 0, 0, 0);
         }
-        const int col = nj * 32 + r32;
-        const float bb = bias[col];
+        uint16_t* orow = out + (mi * 32 + r32) * kMlpStride;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int row = mi * 32 + 8 * (r >> 2) + 4 * q + (r & 3);
-            out[row * kMlpStride + col] = f2bf(gelu_f(acc[r] + bb));
+        for (int g = 0; g < 4; ++g) {
+            const int n0 = nj * 32 + 8 * g + 4 * q;
+            bf16x4 v;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = (short)f2bf(gelu_f(acc[4 * g + u] + bias[n0 + u]));
+            *(bf16x4*)(orow + n0) = v;
         }
     }
 }

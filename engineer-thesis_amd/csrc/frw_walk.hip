@@ -110,7 +110,10 @@ __global__ void __launch_bounds__(64) walk_prologue(WalkJob* jobs, int n_jobs, u
 }
 
 template <int kMode, bool kStats, bool kHarvest>
-__global__ void __launch_bounds__(256) walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
+// At least three waves per SIMD: the learned instantiation then fits 168
+// VGPRs without spills (the others need fewer).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
 {
     extern __shared__ int32_t stack_mem[];
     __shared__ uint32_t sT[kNSample];
