@@ -159,16 +159,16 @@ void frw_tree_free(frw_tree* t)
 {
     if (!t) return;
     free_refcompat(t);
-    if (t->d_rects) hipFree(t->d_rects);
-    if (t->d_ids) hipFree(t->d_ids);
-    if (t->d_nodes) hipFree(t->d_nodes);
-    if (t->d_start) hipFree(t->d_start);
-    if (t->d_thresholds) hipFree(t->d_thresholds);
-    if (t->d_wtab) hipFree(t->d_wtab);
-    if (t->d_work) hipFree(t->d_work);
-    if (t->d_err) hipFree(t->d_err);
-    if (t->d_surface) hipFree(t->d_surface);
-    if (t->d_job) hipFree(t->d_job);
+    if (t->d_rects) (void)hipFree(t->d_rects);
+    if (t->d_ids) (void)hipFree(t->d_ids);
+    if (t->d_nodes) (void)hipFree(t->d_nodes);
+    if (t->d_start) (void)hipFree(t->d_start);
+    if (t->d_thresholds) (void)hipFree(t->d_thresholds);
+    if (t->d_wtab) (void)hipFree(t->d_wtab);
+    if (t->d_work) (void)hipFree(t->d_work);
+    if (t->d_err) (void)hipFree(t->d_err);
+    if (t->d_surface) (void)hipFree(t->d_surface);
+    if (t->d_job) (void)hipFree(t->d_job);
     delete t;
 }
 
@@ -182,8 +182,11 @@ int frw_tree_build(const frw_layout* l, const frw_tree_params* p, void* stream, 
     if (kind != FRW_TREE_MORTON && kind != FRW_TREE_REFCOMPAT) return set_error(FRW_E_INVALID, "unknown tree kind");
     FRW_GUARD_BEGIN
     hipStream_t s = (hipStream_t)stream;
+    int dev = 0;
+    int rc0 = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (rc0 != FRW_OK) return rc0;
     frw_tree* t = new frw_tree;
-    hipGetDevice(&t->device);
+    t->device = dev;
     t->bounds = l->bounds;
     t->n_rects = (int64_t)l->rects.size();
     t->leaf_cap = cap;
@@ -432,7 +435,7 @@ int frw_walk_weighted(const frw_tree* t, const frw_walk_params* p, double eps, u
     if (rc == FRW_OK) rc = check_hip(hipMemcpy(wsq_host, d_wq, sizeof(double) * nb, hipMemcpyDeviceToHost), "copy");
     if (rc == FRW_OK && hops) rc = check_hip(hipMemcpy(hops, d_hops, sizeof(int64_t), hipMemcpyDeviceToHost), "copy");
     for (void* q : {(void*)d_counts, (void*)d_hops, (void*)d_ws, (void*)d_wq})
-        if (q) hipFree(q);
+        if (q) (void)hipFree(q);
     return rc;
     FRW_GUARD_END
 }
@@ -557,23 +560,23 @@ int frw_walk(const frw_tree* t, const frw_walk_params* p, int64_t* counts_host, 
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hops, sizeof(int64_t)), "hipMalloc hops");
     if (rc == FRW_OK) rc = check_hip(hipMemset(d_counts, 0, sizeof(int64_t) * nb), "memset counts");
     if (rc == FRW_OK) rc = check_hip(hipMemset(d_hops, 0, sizeof(int64_t)), "memset hops");
+    if (rc == FRW_OK) rc = check_hip(hipEventCreate(&e0), "hipEventCreate");
+    if (rc == FRW_OK) rc = check_hip(hipEventCreate(&e1), "hipEventCreate");
+    if (rc == FRW_OK) rc = check_hip(hipEventRecord(e0, nullptr), "hipEventRecord");
     if (rc == FRW_OK) {
-        hipEventCreate(&e0);
-        hipEventCreate(&e1);
         frw_walk_outputs o{d_counts, d_hops, nullptr, nullptr};
-        hipEventRecord(e0, nullptr);
         rc = launch_walk(t, p, &o, nullptr);
-        hipEventRecord(e1, nullptr);
     }
+    if (rc == FRW_OK) rc = check_hip(hipEventRecord(e1, nullptr), "hipEventRecord");
     if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "walk");
     if (rc == FRW_OK) rc = check_err_flag(t, nullptr);
     if (rc == FRW_OK) rc = check_hip(hipMemcpy(counts.data(), d_counts, sizeof(int64_t) * nb, hipMemcpyDeviceToHost), "copy counts");
     if (rc == FRW_OK) rc = check_hip(hipMemcpy(&hops, d_hops, sizeof(int64_t), hipMemcpyDeviceToHost), "copy hops");
-    if (rc == FRW_OK) hipEventElapsedTime(&ms, e0, e1);
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    if (d_counts) hipFree(d_counts);
-    if (d_hops) hipFree(d_hops);
+    if (rc == FRW_OK) rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (d_counts) (void)hipFree(d_counts);
+    if (d_hops) (void)hipFree(d_hops);
     if (rc != FRW_OK) return rc;
     if (counts_host) std::memcpy(counts_host, counts.data(), sizeof(int64_t) * nb);
     if (summary) {
@@ -625,9 +628,9 @@ int frw_walk_traffic(const frw_tree* t, const frw_walk_params* p, int64_t out[4]
         out[1] = tr[1];
         out[3] = tr[2];
     }
-    if (d_counts) hipFree(d_counts);
-    if (d_hops) hipFree(d_hops);
-    if (d_tr) hipFree(d_tr);
+    if (d_counts) (void)hipFree(d_counts);
+    if (d_hops) (void)hipFree(d_hops);
+    if (d_tr) (void)hipFree(d_tr);
     return rc;
     FRW_GUARD_END
 }

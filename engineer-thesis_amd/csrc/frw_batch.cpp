@@ -39,8 +39,8 @@ namespace {
 
 void drop_graph(frw_batch* b)
 {
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
 }
@@ -111,8 +111,11 @@ int frw_batch_create(const frw_tree* const* trees, int32_t n_trees, const frw_ba
     if (!trees || n_trees <= 0 || !jobs || n_jobs <= 0 || !out)
         return set_error(FRW_E_INVALID, "bad batch arguments");
     try {
+        int dev = 0;
+        const int rc0 = check_hip(hipGetDevice(&dev), "hipGetDevice");
+        if (rc0 != FRW_OK) return rc0;
         frw_batch* b = new frw_batch;
-        hipGetDevice(&b->device);
+        b->device = dev;
         b->trees.assign(trees, trees + n_trees);
         b->pyrs.assign(n_trees, MlpPyramid{});
         b->jobs.assign(jobs, jobs + n_jobs);
@@ -202,7 +205,7 @@ int batch_walk(frw_batch* b, const frw_batch_params* p, const frw_mlp* net, floa
         hipGraph_t g = nullptr;
         rc = check_hip(hipStreamEndCapture(b->capture, &g), "end capture");
         if (lrc != FRW_OK) {
-            if (g) hipGraphDestroy(g);
+            if (g) (void)hipGraphDestroy(g);
             return lrc;
         }
         if (rc != FRW_OK) return rc;
@@ -237,13 +240,13 @@ int autotune(frw_batch* b, const frw_batch_params* base, frw_batch_params* best,
     int rc = check_hip(hipMalloc(&d_counts, sizeof(int64_t) * b->counts_size), "hipMalloc counts");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hops, sizeof(int64_t) * b->jobs.size()), "hipMalloc hops");
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
+    if (rc == FRW_OK) rc = check_hip(hipEventCreate(&e0), "hipEventCreate");
+    if (rc == FRW_OK) rc = check_hip(hipEventCreate(&e1), "hipEventCreate");
     double best_rate = -1.0;
     frw_batch_params cand = *base;
     cand.use_graph = 0;
     int cus = 0;
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
+    if (rc == FRW_OK) rc = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device), "CUs");
     const int blocks[] = {64, 128, 192, 256};
     const int waves_per_cu[] = {4, 8, 16, 32};
     for (int bi = 0; bi < 4 && rc == FRW_OK; ++bi) {
@@ -254,17 +257,19 @@ int autotune(frw_batch* b, const frw_batch_params* base, frw_batch_params* best,
             int blk = 0, grd = 0;
             size_t lds = 0;
             if (plan_walk(L, &blk, &grd, &lds) != FRW_OK) continue;  // does not fit: skip
-            hipMemsetAsync(d_counts, 0, sizeof(int64_t) * b->counts_size, nullptr);
-            hipMemsetAsync(d_hops, 0, sizeof(int64_t) * b->jobs.size(), nullptr);
-            hipEventRecord(e0, nullptr);
-            rc = run(&cand, d_counts, d_hops);
-            hipEventRecord(e1, nullptr);
+            rc = check_hip(hipMemsetAsync(d_counts, 0, sizeof(int64_t) * b->counts_size, nullptr), "memset counts");
+            if (rc == FRW_OK) rc = check_hip(hipMemsetAsync(d_hops, 0, sizeof(int64_t) * b->jobs.size(), nullptr), "memset hops");
+            if (rc == FRW_OK) rc = check_hip(hipEventRecord(e0, nullptr), "hipEventRecord");
+            if (rc == FRW_OK) rc = run(&cand, d_counts, d_hops);
+            if (rc == FRW_OK) rc = check_hip(hipEventRecord(e1, nullptr), "hipEventRecord");
             if (rc != FRW_OK) break;
             if ((rc = check_hip(hipDeviceSynchronize(), "autotune run")) != FRW_OK) break;
             float ms = 0.f;
-            hipEventElapsedTime(&ms, e0, e1);
+            if ((rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) != FRW_OK) break;
             std::vector<int64_t> h(b->jobs.size());
-            hipMemcpy(h.data(), d_hops, sizeof(int64_t) * h.size(), hipMemcpyDeviceToHost);
+            if ((rc = check_hip(hipMemcpy(h.data(), d_hops, sizeof(int64_t) * h.size(), hipMemcpyDeviceToHost),
+                                "copy hops")) != FRW_OK)
+                break;
             int64_t tot = 0;
             for (int64_t v : h) tot += v;
             const double rate = tot / (ms * 1e-3);
@@ -274,10 +279,10 @@ int autotune(frw_batch* b, const frw_batch_params* base, frw_batch_params* best,
             }
         }
     }
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    if (d_counts) hipFree(d_counts);
-    if (d_hops) hipFree(d_hops);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (d_counts) (void)hipFree(d_counts);
+    if (d_hops) (void)hipFree(d_hops);
     b->up_counts = b->up_hops = nullptr;  // the tuning buffers are gone
     if (rc != FRW_OK) return rc;
     if (best_rate < 0) return set_error(FRW_E_RANGE, "no launch shape fits");
@@ -361,11 +366,11 @@ void frw_batch_free(frw_batch* b)
 {
     if (!b) return;
     drop_graph(b);
-    if (b->capture) hipStreamDestroy(b->capture);
-    if (b->d_jobs) hipFree(b->d_jobs);
-    if (b->d_work) hipFree(b->d_work);
-    if (b->d_err) hipFree(b->d_err);
-    if (b->d_thresholds) hipFree(b->d_thresholds);
+    if (b->capture) (void)hipStreamDestroy(b->capture);
+    if (b->d_jobs) (void)hipFree(b->d_jobs);
+    if (b->d_work) (void)hipFree(b->d_work);
+    if (b->d_err) (void)hipFree(b->d_err);
+    if (b->d_thresholds) (void)hipFree(b->d_thresholds);
     delete b;
 }
 

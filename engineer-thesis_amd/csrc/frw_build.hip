@@ -205,8 +205,8 @@ struct DevBuf {
             if (hipMemcpyAsync(q, p, keep_n * sizeof(T), hipMemcpyDeviceToDevice, s) != hipSuccess)
                 return set_error(FRW_E_DEVICE, "copy failed in tree build");
         if (p) {
-            hipStreamSynchronize(s);
-            hipFree(p);
+            if (hipStreamSynchronize(s) != hipSuccess) return set_error(FRW_E_DEVICE, "sync failed in tree build");
+            (void)hipFree(p);
         }
         p = q;
         cap = nc;
@@ -214,7 +214,7 @@ struct DevBuf {
     }
     void release()
     {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
     }
@@ -246,12 +246,15 @@ int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 B, int leaf_cap
     K += 1;  // doubled centre coordinates take one more bit per axis
     t->morton_bits = K;
 
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventRecord(e0, s);
-
-    int rc = FRW_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = check_hip(hipEventCreate(&e0), "hipEventCreate");
+    if (rc == FRW_OK) rc = check_hip(hipEventCreate(&e1), "hipEventCreate");
+    if (rc == FRW_OK) rc = check_hip(hipEventRecord(e0, s), "hipEventRecord");
+    if (rc != FRW_OK) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        return rc;
+    }
     int4* d_in = nullptr;
     uint64_t *d_keys = nullptr, *d_keys_sorted = nullptr;
     int32_t* d_vals = nullptr;
@@ -315,8 +318,9 @@ int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 B, int leaf_cap
         FRW_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, need, internal.p, group.p, nf, s));
         if (need > tmp_bytes || !d_tmp) {
             if (d_tmp) {
-                hipStreamSynchronize(s);
-                hipFree(d_tmp);
+                FRW_TRY(hipStreamSynchronize(s));
+                (void)hipFree(d_tmp);
+                d_tmp = nullptr;
             }
             tmp_bytes = std::max<size_t>(need, 16);
             FRW_TRY(hipMalloc(&d_tmp, tmp_bytes));
@@ -354,11 +358,11 @@ int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 B, int leaf_cap
         FRW_TRY(hipGetLastError());
         t->start_levels = levels;
     }
-    hipEventRecord(e1, s);
+    FRW_TRY(hipEventRecord(e1, s));
     FRW_TRY(hipStreamSynchronize(s));
     {
         float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
+        FRW_TRY(hipEventElapsedTime(&ms, e0, e1));
         t->build_ms = ms;
     }
     t->n_nodes = n_nodes;
@@ -367,12 +371,12 @@ int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 B, int leaf_cap
 done:
 #undef FRW_TRY
 #undef FRW_TRYRC
-    hipStreamSynchronize(s);
-    if (d_in) hipFree(d_in);
-    if (d_keys) hipFree(d_keys);
-    if (d_keys_sorted) hipFree(d_keys_sorted);
-    if (d_vals) hipFree(d_vals);
-    if (d_tmp) hipFree(d_tmp);
+    (void)hipStreamSynchronize(s);  // cleanup: the buffers below are freed either way
+    if (d_in) (void)hipFree(d_in);
+    if (d_keys) (void)hipFree(d_keys);
+    if (d_keys_sorted) (void)hipFree(d_keys_sorted);
+    if (d_vals) (void)hipFree(d_vals);
+    if (d_tmp) (void)hipFree(d_tmp);
     fa.release();
     fb.release();
     internal.release();
@@ -380,8 +384,8 @@ done:
     own_end.release();
     runs.release();
     nodes.release();
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return rc;
 }
 

@@ -419,9 +419,9 @@ namespace {
 int ensure_ws(frw_mlp* m, int64_t B)
 {
     if (m->ws_batch >= B) return FRW_OK;
-    hipFree(m->d_Z1); hipFree(m->d_Z2); hipFree(m->d_Z3); hipFree(m->d_F);
-    hipFree(m->d_A1); hipFree(m->d_A2); hipFree(m->d_A3); hipFree(m->d_D); hipFree(m->d_Xb);
-    hipFree(m->d_tb); hipFree(m->d_idx); hipFree(m->d_dy); hipFree(m->d_lossv);
+    (void)hipFree(m->d_Z1); (void)hipFree(m->d_Z2); (void)hipFree(m->d_Z3); (void)hipFree(m->d_F);
+    (void)hipFree(m->d_A1); (void)hipFree(m->d_A2); (void)hipFree(m->d_A3); (void)hipFree(m->d_D); (void)hipFree(m->d_Xb);
+    (void)hipFree(m->d_tb); (void)hipFree(m->d_idx); (void)hipFree(m->d_dy); (void)hipFree(m->d_lossv);
     const size_t h = (size_t)B * kMlpHid;
     int rc = FRW_OK;
     rc = rc ? rc : check_hip(hipMalloc(&m->d_Z1, h * 4), "mlp ws");
@@ -616,7 +616,7 @@ void frw_mlp_free(frw_mlp* m)
                     (void*)m->d_Z2, (void*)m->d_Z3, (void*)m->d_F, (void*)m->d_A1, (void*)m->d_A2,
                     (void*)m->d_A3, (void*)m->d_D, (void*)m->d_Xb, (void*)m->d_tb, (void*)m->d_loss,
                     (void*)m->d_idx, (void*)m->d_dy, (void*)m->d_lossv})
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     delete m;
 }
 
@@ -678,7 +678,7 @@ int frw_mlp_features(const frw_mlp* m, const double* pts, const double* h, int64
     features_kernel<<<blocks(n, 256), 256, 0, s>>>(mlp_pyramid(m), pts, h, n, X, nullptr, nullptr, nullptr);
     widen_bf16<<<blocks(n * kMlpInPad, 256), 256, 0, s>>>(X, feats, n * kMlpInPad);
     rc = check_hip(hipStreamSynchronize(s), "features");
-    hipFree(X);
+    (void)hipFree(X);
     return rc;
 }
 
@@ -693,7 +693,7 @@ int frw_mlp_forward(const frw_mlp* m, const float* X, int64_t n, float* y, void*
     narrow_bf16<<<blocks(n * kMlpInPad, 256), 256, 0, s>>>(X, Xb, n * kMlpInPad);
     forward_waves<<<blocks(n, 64), 64, 0, s>>>(mlp_net(m, 1.0f), Xb, n, y);
     rc = check_hip(hipStreamSynchronize(s), "mlp forward");
-    hipFree(Xb);
+    (void)hipFree(Xb);
     return rc;
 }
 
@@ -706,13 +706,13 @@ int frw_mlp_gradient(frw_mlp* m, const float* X, const float* t, int64_t n, floa
     rc = check_hip(hipMalloc(&Xb, (size_t)n * kMlpInPad * 2), "hipMalloc gradient");
     if (rc != FRW_OK) return rc;
     narrow_bf16<<<blocks(n * kMlpInPad, 256), 256>>>(X, Xb, n * kMlpInPad);
-    hipMemset(m->d_grad, 0, kMlpParams * 4);
-    hipMemset(m->d_loss, 0, 4);
-    rc = gradient(m, Xb, t, (int)n, nullptr);
+    rc = check_hip(hipMemset(m->d_grad, 0, kMlpParams * 4), "memset grad");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(m->d_loss, 0, 4), "memset loss");
+    if (rc == FRW_OK) rc = gradient(m, Xb, t, (int)n, nullptr);
     if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "mlp gradient");
     if (rc == FRW_OK) rc = check_hip(hipMemcpy(grad, m->d_grad, kMlpParams * 4, hipMemcpyDeviceToHost), "copy grad");
     if (rc == FRW_OK && loss) rc = check_hip(hipMemcpy(loss, m->d_loss, 4, hipMemcpyDeviceToHost), "copy loss");
-    hipFree(Xb);
+    (void)hipFree(Xb);
     return rc;
 }
 
@@ -736,12 +736,12 @@ int frw_mlp_add_samples(frw_mlp* m, const frw_mlp* features_of, const double* pt
         rc = check_hip(hipDeviceSynchronize(), "features");
     }
     if (rc != FRW_OK) {
-        if (X) hipFree(X);
-        if (T) hipFree(T);
+        if (X) (void)hipFree(X);
+        if (T) (void)hipFree(T);
         return rc;
     }
-    hipFree(m->d_X);
-    hipFree(m->d_t);
+    (void)hipFree(m->d_X);
+    (void)hipFree(m->d_t);
     m->d_X = X;
     m->d_t = T;
     m->n_samples = tot;
@@ -751,8 +751,8 @@ int frw_mlp_add_samples(frw_mlp* m, const frw_mlp* features_of, const double* pt
 int frw_mlp_set_samples(frw_mlp* m, const double* pts, const double* h, const double* dstar, int64_t n)
 {
     if (!m || !pts || !h || !dstar || n <= 0) return set_error(FRW_E_INVALID, "bad arguments");
-    hipFree(m->d_X);
-    hipFree(m->d_t);
+    (void)hipFree(m->d_X);
+    (void)hipFree(m->d_t);
     m->d_X = nullptr;
     m->d_t = nullptr;
     m->n_samples = 0;
@@ -785,9 +785,9 @@ int frw_mlp_train(frw_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t 
         loss_n = 0;
         for (int64_t b0 = 0; b0 + batch <= n && rc == FRW_OK; b0 += batch) {
             gather_rows<<<batch, 128>>>(m->d_X, m->d_t, d_perm + b0, batch, m->d_Xb, m->d_tb);
-            hipMemsetAsync(m->d_grad, 0, kMlpParams * 4, nullptr);
-            hipMemsetAsync(m->d_loss, 0, 4, nullptr);
-            rc = gradient(m, m->d_Xb, m->d_tb, batch, nullptr);
+            rc = check_hip(hipMemsetAsync(m->d_grad, 0, kMlpParams * 4, nullptr), "memset grad");
+            if (rc == FRW_OK) rc = check_hip(hipMemsetAsync(m->d_loss, 0, 4, nullptr), "memset loss");
+            if (rc == FRW_OK) rc = gradient(m, m->d_Xb, m->d_tb, batch, nullptr);
             if (rc != FRW_OK) break;
             m->adam_step++;
             const float b1c = 1.0f - std::pow(0.9f, (float)m->adam_step);
@@ -803,7 +803,7 @@ int frw_mlp_train(frw_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t 
             }
         }
     }
-    if (d_perm) hipFree(d_perm);
+    if (d_perm) (void)hipFree(d_perm);
     if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "train");
     if (rc == FRW_OK && last_loss) *last_loss = loss_n ? loss_sum / loss_n : 0.f;
     return rc;
@@ -844,9 +844,9 @@ int frw_walk_harvest(const frw_tree* t, const frw_walk_params* p, double* pts, d
     unsigned long long got = 0;
     if (rc == FRW_OK) rc = check_hip(hipMemcpy(&got, d_n, 8, hipMemcpyDeviceToHost), "copy count");
     if (rc == FRW_OK && n_out) *n_out = (int64_t)std::min<unsigned long long>(got, (unsigned long long)capacity);
-    hipFree(d_counts);
-    hipFree(d_hops);
-    hipFree(d_n);
+    (void)hipFree(d_counts);
+    (void)hipFree(d_hops);
+    (void)hipFree(d_n);
     return rc;
 }
 
@@ -887,7 +887,7 @@ int learned_sync(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p, 
         stats[0] = (int64_t)st[0];
         stats[1] = (int64_t)st[1];
     }
-    if (d_st) hipFree(d_st);
+    if (d_st) (void)hipFree(d_st);
     int err = 0;
     if (rc == FRW_OK) rc = check_hip(hipMemcpy(&err, t->d_err, sizeof(int), hipMemcpyDeviceToHost), "err flag");
     if (rc == FRW_OK && err) rc = set_error(FRW_E_KERNEL, "kernel error flag " + std::to_string(err));
@@ -948,9 +948,9 @@ int frw_walk_learned_traffic(const frw_tree* t, const frw_mlp* m, const frw_walk
         out[1] = tr[1];
         out[5] = tr[2];
     }
-    if (d_counts) hipFree(d_counts);
-    if (d_hops) hipFree(d_hops);
-    if (d_tr) hipFree(d_tr);
+    if (d_counts) (void)hipFree(d_counts);
+    if (d_hops) (void)hipFree(d_hops);
+    if (d_tr) (void)hipFree(d_tr);
     return rc;
 }
 

@@ -252,9 +252,9 @@ int build_refcompat(frw_tree* t)
 void free_refcompat(frw_tree* t)
 {
     if (!t->ref) return;
-    if (t->ref->d_nodes) hipFree(t->ref->d_nodes);
-    if (t->ref->d_objs) hipFree(t->ref->d_objs);
-    if (t->ref->d_rects) hipFree(t->ref->d_rects);
+    if (t->ref->d_nodes) (void)hipFree(t->ref->d_nodes);
+    if (t->ref->d_objs) (void)hipFree(t->ref->d_objs);
+    if (t->ref->d_rects) (void)hipFree(t->ref->d_rects);
     delete t->ref;
     t->ref = nullptr;
 }

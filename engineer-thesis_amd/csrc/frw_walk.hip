@@ -477,9 +477,10 @@ int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds)
     const size_t per_wave = L.mode == kModeLearned ? (size_t)kMlpLdsPerWave : 0;
     const size_t fixed = kNSample * 4 + kLdsJobs * 8;
     int lds_block = 0, lds_cu = 0, cus = 0;
-    hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, L.device);
-    hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, L.device);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, L.device);
+    if (hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, L.device) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, L.device) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, L.device) != hipSuccess)
+        return set_error(FRW_E_DEVICE, "device attributes unavailable");
     if (lds_block <= 0) lds_block = 64 * 1024;
     if (lds_cu <= 0) lds_cu = lds_block;
     auto need = [&](int bb) { return (size_t)bb * per_lane + (size_t)(bb / kWaveSize) * per_wave; };
