@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -65,48 +66,109 @@ int parse_rect(const char* s, int4& out)
 
 }  // namespace
 
+namespace {
+
+// What one line of the file means to the reader.
+struct LineEvent {
+    int32_t kind;    // 0 rect, 1 layer delimiter, 2 line too long (ends the read), 3 malformed rect
+    int64_t line;    // line number within the chunk (1-based)
+    int4 r;
+};
+
+// Lines of buf[beg, end) (beg at a line start; end at a line start or the
+// end of the file), in order.
+void scan_chunk(const char* buf, size_t beg, size_t end, std::vector<LineEvent>* ev, int64_t* n_lines)
+{
+    char line[kMaxLine];
+    int64_t ln = 0;
+    size_t p = beg;
+    while (p < end) {
+        size_t q = p;
+        while (q < end && buf[q] != '\n') ++q;
+        const size_t len = q - p;
+        ++ln;
+        if (len >= (size_t)kMaxLine) {  // the 50-byte getline buffer overflows: the read ends
+            ev->push_back(LineEvent{2, ln, make_int4(0, 0, 0, 0)});
+            break;
+        }
+        std::memcpy(line, buf + p, len);
+        line[len] = 0;
+        if (std::strstr(line, "<<")) {
+            ev->push_back(LineEvent{1, ln, make_int4(0, 0, 0, 0)});
+        } else if (std::strstr(line, "rect")) {
+            LineEvent e{0, ln, make_int4(0, 0, 0, 0)};
+            if (parse_rect(line, e.r) != FRW_OK) e.kind = 3;
+            ev->push_back(e);
+        }
+        p = q + 1;
+    }
+    *n_lines = ln;
+}
+
+}  // namespace
+
+// The file is read whole and its lines classified in parallel chunks (split at
+// line starts), then the events are applied in file order, so the result is
+// the sequential reader's: same layers, same rects, same stopping line.
 int read_layout(const char* path, int layer, std::vector<int4>* rects, int* n_layers)
 {
     FILE* f = std::fopen(path, "rb");
     if (!f) return set_error(FRW_E_IO, std::string("cannot open layout file ") + path);
-    char line[kMaxLine];
-    bool first = true;
-    int cur = 0, err = FRW_OK, line_no = 0;
+    std::vector<char> buf;
+    {
+        char tmp[1 << 16];
+        size_t got;
+        while ((got = std::fread(tmp, 1, sizeof(tmp), f)) > 0) buf.insert(buf.end(), tmp, tmp + got);
+        const bool bad = std::ferror(f) != 0;
+        std::fclose(f);
+        if (bad) return set_error(FRW_E_IO, std::string("cannot read layout file ") + path);
+    }
+    const size_t n = buf.size();
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int T = (int)std::max<size_t>(1, std::min<size_t>({(size_t)std::min(hw, 16u), n >> 20}));
+    std::vector<size_t> cut(T + 1, n);
+    cut[0] = 0;
+    for (int t = 1; t < T; ++t) {
+        size_t c = std::max(cut[t - 1], n * (size_t)t / (size_t)T);
+        while (c < n && c > 0 && buf[c - 1] != '\n') ++c;  // start of a line
+        cut[t] = c;
+    }
+    std::vector<std::vector<LineEvent>> ev(T);
+    std::vector<int64_t> lines(T, 0);
+    if (T == 1) {
+        scan_chunk(buf.data(), 0, n, &ev[0], &lines[0]);
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < T; ++t)
+            th.emplace_back(scan_chunk, buf.data(), cut[t], cut[t + 1], &ev[t], &lines[t]);
+        for (std::thread& x : th) x.join();
+    }
+    bool first = true, stop = false;
+    int cur = 0;
+    int64_t base = 0;
     std::vector<int4> acc;
-    for (;;) {
-        int len = 0, ch = 0;
-        bool got = false, fail = false;
-        for (;;) {
-            ch = std::fgetc(f);
-            if (ch == EOF) break;
-            got = true;
-            if (ch == '\n') break;
-            if (len == kMaxLine - 1) { fail = true; break; }
-            line[len++] = (char)ch;
-        }
-        line[len] = 0;
-        if (fail || (!got && ch == EOF)) break;
-        ++line_no;
-        if (std::strstr(line, "<<")) {
-            if (first) { first = false; continue; }
-            if (cur == layer && rects) *rects = acc;
-            acc.clear();
-            ++cur;
-            continue;
-        }
-        if (std::strstr(line, "rect")) {
-            int4 r;
-            if (parse_rect(line, r) != FRW_OK) {
-                err = set_error(FRW_E_PARSE, "malformed rect line " + std::to_string(line_no) +
-                                                 " in " + path);
+    for (int t = 0; t < T && !stop; ++t) {
+        for (const LineEvent& e : ev[t]) {
+            if (e.kind == 2) {
+                stop = true;
                 break;
             }
-            if (cur == layer) acc.push_back(r);
+            if (e.kind == 3)
+                return set_error(FRW_E_PARSE, "malformed rect line " + std::to_string(base + e.line) + " in " + path);
+            if (e.kind == 1) {
+                if (first) {
+                    first = false;
+                    continue;
+                }
+                if (cur == layer && rects) *rects = acc;
+                acc.clear();
+                ++cur;
+                continue;
+            }
+            if (cur == layer) acc.push_back(e.r);
         }
-        if (ch == EOF) break;
+        base += lines[t];
     }
-    std::fclose(f);
-    if (err != FRW_OK) return err;
     if (n_layers) *n_layers = cur;
     if (layer < 0 || layer >= cur)
         return set_error(FRW_E_LAYER, "layer " + std::to_string(layer) + " not in " + path);
