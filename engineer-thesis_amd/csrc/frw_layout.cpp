@@ -125,7 +125,7 @@ int read_layout(const char* path, int layer, std::vector<int4>* rects, int* n_la
     }
     const size_t n = buf.size();
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const int T = (int)std::max<size_t>(1, std::min<size_t>({(size_t)std::min(hw, 16u), n >> 20}));
+    const int T = (int)std::max<size_t>(1, std::min<size_t>({(size_t)std::min(hw, 16u), n >> 18}));
     std::vector<size_t> cut(T + 1, n);
     cut[0] = 0;
     for (int t = 1; t < T; ++t) {

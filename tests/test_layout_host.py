@@ -143,3 +143,41 @@ def test_results_file(tmp_path):
     assert lines[1] == "Ile scieżek wyszło poza granice warstwy: 50%"
     assert lines[2] == "0 0 10 10: 37.5%"
     assert lines[3] == "1e+06 0 1.00001e+06 10: 12.5%"   # std::ostream default formatting
+
+
+def test_reader_parallel_chunks_match_oracle(tmp_path, oracle_lib):
+    """Files above 1 MB are classified in parallel chunks; every layer, the
+    stopping line and the errors must equal the sequential rules (oracle)."""
+    rng = np.random.default_rng(8)
+    parts = ["header line\n<< start >>\n"]
+    for layer in range(5):
+        for i in range(12_000):
+            x, y = rng.integers(0, 10 ** 6, 2)
+            w, h = rng.integers(1, 500, 2)
+            kind = rng.integers(0, 20)
+            if kind == 0:
+                parts.append(f"  comment {i}\n")
+            elif kind == 1:
+                parts.append(f"direct {x} {y} {x + w} {y + h}\r\n")
+            else:
+                parts.append(f"rect {x} {y} {x + w} {y + h}\n")
+        parts.append(f"<< layer {layer} >>\n")
+    text = "".join(parts)
+    assert len(text) > 1 << 20  # several 256 KB chunks
+    p = _write(tmp_path, text, "big.txt")
+    n_layers = frw.Layout.layer_count(p)
+    assert n_layers == O.parse(p, 0)[2] == 5
+    for layer in range(n_layers):
+        n, r, _ = O.parse(p, layer)
+        assert np.array_equal(frw.Layout.load(p, layer).rects().astype(np.float64), r), layer
+    # a line of 50 characters late in the file ends the read there
+    cut = text.index("<< layer 3 >>")
+    p2 = _write(tmp_path, text[:cut] + "x" * 50 + "\n" + text[cut:], "long.txt")
+    assert frw.Layout.layer_count(p2) == O.parse(p2, 0)[2] == 3
+    # a malformed rect in the last chunk is reported with its line number
+    cut = text.index("<< layer 4 >>")
+    p3 = _write(tmp_path, text[:cut] + "rect 1 2 3\n" + text[cut:], "bad.txt")
+    with pytest.raises(frw.FrwError) as e:
+        frw.Layout.load(p3)
+    assert e.value.code == frw.E_PARSE
+    assert f"line {text[:cut].count(chr(10)) + 1} " in str(e.value)
