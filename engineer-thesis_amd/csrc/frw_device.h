@@ -175,6 +175,22 @@ __host__ __device__ inline void raise_error(int* err, int bit)
 }
 
 // ---------------------------------------------------------------- tree view
+// Loads from the tree's arrays as global-memory loads: the arrays live in
+// device memory, and a global load (unlike the generic "flat" load the
+// compiler emits for a pointer it loaded from memory) waits on the vector
+// memory counter only, not on the LDS traversal stack's.  The pointer goes
+// through an integer: a plain cast to the global address space is folded
+// back into the generic form.
+template <class T>
+__host__ __device__ inline T gload(const T* p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(1))) T*)(uintptr_t)p;
+#else
+    return *p;
+#endif
+}
+
 struct DevTree {
     const int4* rects;     // sorted by Morton key, 16 B each
     const int32_t* ids;    // original index of each sorted rect
@@ -242,10 +258,10 @@ __host__ __device__ inline bool q1_box(const DevTree& t, const LaneStack& st, do
     st.at(sp++) = 0;
     while (sp > 0) {
         const int ni = st.at(--sp);
-        const Node nd = t.nodes[ni];
+        const Node nd = gload(t.nodes + ni);
         if (cnt) cnt->nodes += nd.first_child >= 0 ? 5u : 1u;
         for (int r = nd.rbeg; r < nd.rend; ++r) {
-            const int4 rc = t.rects[r];
+            const int4 rc = gload(t.rects + r);
             if (cnt) cnt->rects++;
             if (box_hits_rect(rc, x0, y0, x1, y1) && !(kIgnore && same_rect(rc, ignore)))
                 return true;
@@ -253,7 +269,7 @@ __host__ __device__ inline bool q1_box(const DevTree& t, const LaneStack& st, do
         if (nd.first_child >= 0) {
 #pragma unroll
             for (int c = 3; c >= 0; --c) {
-                const int4 cb = t.nodes[nd.first_child + c].box;
+                const int4 cb = gload(&t.nodes[nd.first_child + c].box);
                 if (box_hits_rect(cb, x0, y0, x1, y1)) st.at(sp++) = nd.first_child + c;
             }
         }
@@ -291,7 +307,7 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
         if (hint_rect) {
             bestr = *hint_rect;
         } else {
-            bestr = t.rects[hint];
+            bestr = gload(t.rects + hint);
             if (cnt) cnt->rects++;
         }
         best = linf(bestr, px, py);
@@ -334,7 +350,7 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
         }
         int32_t e[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) e[k] = tab[pos[k < np ? k : 0]];
+        for (int k = 0; k < 4; ++k) e[k] = gload(tab + pos[k < np ? k : 0]);
         if (cnt) cnt->tabs += (uint32_t)np;
         const int own = e[0];
         int cand[3];
@@ -353,7 +369,7 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
     }
     while (sp > 0) {
         const int ni = st.at(--sp);
-        const Node nd = t.nodes[ni];
+        const Node nd = gload(t.nodes + ni);
         if (cnt) cnt->nodes++;
         if (!(linf_lb(nd.box, fx, cx, fy, cy) < lim)) continue;
         if (cnt) cnt->rects += (uint32_t)(nd.rend - nd.rbeg);
@@ -363,13 +379,13 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
         for (int r0 = nd.rbeg; r0 < nd.rend; r0 += 4) {
             int4 q[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) q[u] = t.rects[r0 + u < nd.rend ? r0 + u : nd.rend - 1];
+            for (int u = 0; u < 4; ++u) q[u] = gload(t.rects + (r0 + u < nd.rend ? r0 + u : nd.rend - 1));
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int r = r0 + u;
                 if (r >= nd.rend) break;
                 const double d = linf(q[u], px, py);
-                if (d < best || (d == best && d <= 0.0 && t.ids[r] < t.ids[besti])) {
+                if (d < best || (d == best && d <= 0.0 && gload(t.ids + r) < gload(t.ids + besti))) {
                     best = d;
                     lim = ceil_int(d) + (d <= 0.0 ? 1 : 0);
                     besti = r;
@@ -380,7 +396,7 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
         if (nd.first_child >= 0) {
             int lb[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) lb[c] = linf_lb(t.nodes[nd.first_child + c].box, fx, cx, fy, cy);
+            for (int c = 0; c < 4; ++c) lb[c] = linf_lb(gload(&t.nodes[nd.first_child + c].box), fx, cx, fy, cy);
             // the nearest child is pushed last (popped first); the others in order
             int near = 0;
 #pragma unroll
@@ -409,15 +425,15 @@ __host__ __device__ inline int q2_point(const DevTree& t, const LaneStack& st, d
     st.at(sp++) = 0;
     while (sp > 0) {
         const int ni = st.at(--sp);
-        const Node nd = t.nodes[ni];
+        const Node nd = gload(t.nodes + ni);
         if (cnt) {
             cnt->nodes += nd.first_child >= 0 ? 5u : 1u;
             cnt->rects += (uint32_t)(nd.rend - nd.rbeg);
         }
         for (int r = nd.rbeg; r < nd.rend; ++r) {
-            const double d = linf(t.rects[r], px, py);
+            const double d = linf(gload(t.rects + r), px, py);
             if (d <= 0.0) {
-                const int id = t.ids[r];
+                const int id = gload(t.ids + r);
                 if (best_id < 0 || d < best || (d == best && id < best_id)) {
                     best = d;
                     best_id = id;
@@ -427,7 +443,7 @@ __host__ __device__ inline int q2_point(const DevTree& t, const LaneStack& st, d
         if (nd.first_child >= 0) {
 #pragma unroll
             for (int c = 3; c >= 0; --c)
-                if (linf(t.nodes[nd.first_child + c].box, px, py) <= 0.0)
+                if (linf(gload(&t.nodes[nd.first_child + c].box), px, py) <= 0.0)
                     st.at(sp++) = nd.first_child + c;
         }
     }

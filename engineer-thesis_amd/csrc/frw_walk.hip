@@ -355,7 +355,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
                     hint = arg;
                     if (ds <= 0.0) {
                         // p lies in a rect: the search's arg is the point query's answer
-                        result = t.ids[arg];
+                        result = gload(t.ids + arg);
                     } else {
                         if (kHarvest) {
                             const unsigned long long slot = atomicAdd(a.h_count, 1ull);
