@@ -57,7 +57,7 @@ struct WalkArgs {
 };
 
 constexpr int kLdsJobs = 128;                                   // job walk offsets cached in LDS
-constexpr int kChunk = 4;                                       // walk indices a lane takes per queue visit
+constexpr int kWaveChunk = 256;                                 // walk indices a wave claims per queue visit
 constexpr uint32_t kGausDomain = 0x47415553u;                   // 'GAUS': Gaussian-surface start points
 
 // Compensated (Kahan) accumulation; -ffp-contract=off keeps it exact.
@@ -149,7 +149,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
     // added to the tallies when the lane changes job or finishes (these two
     // bins take most walks; one atomic per walk on them would serialise)
     uint32_t my_master = 0, my_escaped = 0;
-    uint64_t nxt = 0, end = 0;  // walk indices this lane has taken from the queue
+    uint64_t wnext = 0, wend = 0;  // the wave's unclaimed walk indices [wnext, wend), equal in every lane
     // weighted estimator: the walk's weight, its preset first point, and
     // per-lane compensated sums (and sums of squares) for the master and
     // escape bins
@@ -181,29 +181,37 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
         const bool need = !active && !done;
         uint64_t g = 0;
         bool got = false;
-        if (need && nxt < end) {
-            g = nxt++;
-            got = true;
-        }
-        // lanes without a local index take kChunk more from the queue: one
-        // atomic per wave for all of them
-        const unsigned long long mask = __ballot(need && !got);
+        // lanes without a walk take the next indices of the wave's range in
+        // lane order; when the range runs short one lane claims kWaveChunk
+        // more from the queue (one atomic on the shared counter per 256
+        // walks: a claim per few walks serialised the kernel on that address)
+        const unsigned long long mask = __ballot(need);
         if (mask) {
-            const int leader = __ffsll((long long)mask) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(a.work, (unsigned long long)__popcll(mask) * kChunk);
-            base = __shfl(base, leader);
-            if (need && !got) {
+            const uint64_t n_need = (uint64_t)__popcll(mask);
+            const uint64_t avail = wend - wnext;
+            const bool refill = avail < n_need;
+            uint64_t base = 0;
+            if (refill) {
+                const int leader = __ffsll((long long)mask) - 1;
+                if (lane == leader) base = atomicAdd(a.work, (unsigned long long)kWaveChunk);
+                base = __shfl(base, leader);
+            }
+            if (need) {
                 const unsigned long long below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
-                const uint64_t g0 = base + (uint64_t)__popcll(below) * kChunk;
+                const uint64_t rank = (uint64_t)__popcll(below);
+                const uint64_t g0 = rank < avail ? wnext + rank : base + (rank - avail);
                 if (g0 < a.total_walks) {
                     g = g0;
                     got = true;
-                    nxt = g0 + 1;
-                    end = g0 + kChunk < a.total_walks ? g0 + kChunk : a.total_walks;
                 } else {
                     done = true;
                 }
+            }
+            if (refill) {
+                wnext = base + (n_need - avail);
+                wend = base + kWaveChunk;
+            } else {
+                wnext += n_need;
             }
         }
         if (got) {
