@@ -467,11 +467,39 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
 // Writes the single job of a plain walk into the tree's job slot.
 __global__ void set_job(WalkJob* slot, WalkJob jb) { *slot = jb; }
 
+// Launch shape when the caller gives none: the block size (256, 192, 128 or
+// 64 lanes) that keeps the most lanes resident per CU for this kernel's
+// registers and its LDS stack, and one such full set of blocks per CU.
+struct AutoShape {
+    bool on = false;
+    size_t per_lane = 0, per_wave = 0, fixed = 0, lds_block = 0;
+    int cus = 1;
+    uint64_t total = 0;
+};
+
 template <int kMode, bool kStats, bool kHarvest>
-void launch_kernel(int grid, int block, size_t lds, hipStream_t s, const WalkJob* jobs, const WalkArgs& a,
-                   const MlpNet& m)
+void launch_kernel(int grid, int block, size_t lds, const AutoShape& as, hipStream_t s, const WalkJob* jobs,
+                   const WalkArgs& a, const MlpNet& m)
 {
-    walk_kernel<kMode, kStats, kHarvest><<<grid, block, lds, s>>>(jobs, a, m);
+    auto kern = walk_kernel<kMode, kStats, kHarvest>;
+    if (as.on) {
+        int best = 0;
+        for (int b : {256, 192, 128, 64}) {
+            const size_t l = (size_t)b * as.per_lane + (size_t)(b / kWaveSize) * as.per_wave;
+            if (l + as.fixed > as.lds_block) continue;
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, b, l) != hipSuccess || nb <= 0) continue;
+            if (nb * b > best) {
+                best = nb * b;
+                block = b;
+                lds = l;
+                grid = nb * as.cus;
+            }
+        }
+        const uint64_t want = (as.total + block - 1) / block;
+        if (want < (uint64_t)grid) grid = (int)std::max<uint64_t>(1, want);
+    }
+    kern<<<grid, block, lds, s>>>(jobs, a, m);
 }
 
 }  // namespace
@@ -519,6 +547,20 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
     size_t lds = 0;
     int rc = plan_walk(L, &block, &grid, &lds);
     if (rc != FRW_OK) return rc;
+    AutoShape as;
+    if (L.block == 0 && L.grid == 0) {
+        int lds_block = 0, cus = 0;
+        if (hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, L.device) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, L.device) != hipSuccess)
+            return set_error(FRW_E_DEVICE, "device attributes unavailable");
+        as.on = true;
+        as.per_lane = (size_t)(3 * L.max_depth + 2) * sizeof(int32_t);
+        as.per_wave = L.mode == kModeLearned ? (size_t)kMlpLdsPerWave : 0;
+        as.fixed = kNSample * 4 + kLdsJobs * 8;
+        as.lds_block = (size_t)lds_block;
+        as.cus = std::max(cus, 1);
+        as.total = L.total_walks;
+    }
     const size_t plds = (size_t)64 * (3 * L.max_depth + 2) * sizeof(int32_t);
     walk_prologue<<<(L.n_jobs + 63) / 64, 64, plds, s>>>(d_jobs, L.n_jobs, L.work);
     if ((rc = check_hip(hipGetLastError(), "walk_prologue")) != FRW_OK) return rc;
@@ -546,20 +588,20 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
     const bool st = L.traffic != nullptr;
     if (L.mode == kModeLearned) {
         if (!L.net) return set_error(FRW_E_INVALID, "learned mode needs a network");
-        if (st) launch_kernel<kModeLearned, true, false>(grid, block, lds, s, d_jobs, a, m);
-        else launch_kernel<kModeLearned, false, false>(grid, block, lds, s, d_jobs, a, m);
+        if (st) launch_kernel<kModeLearned, true, false>(grid, block, lds, as, s, d_jobs, a, m);
+        else launch_kernel<kModeLearned, false, false>(grid, block, lds, as, s, d_jobs, a, m);
     } else if (L.mode == kModeBisect) {
-        if (st) launch_kernel<kModeBisect, true, false>(grid, block, lds, s, d_jobs, a, m);
-        else launch_kernel<kModeBisect, false, false>(grid, block, lds, s, d_jobs, a, m);
+        if (st) launch_kernel<kModeBisect, true, false>(grid, block, lds, as, s, d_jobs, a, m);
+        else launch_kernel<kModeBisect, false, false>(grid, block, lds, as, s, d_jobs, a, m);
     } else if (L.mode == kModeWeighted) {
         if (!L.wtab) return set_error(FRW_E_INVALID, "weighted mode needs its tables");
-        if (st) launch_kernel<kModeWeighted, true, false>(grid, block, lds, s, d_jobs, a, m);
-        else launch_kernel<kModeWeighted, false, false>(grid, block, lds, s, d_jobs, a, m);
+        if (st) launch_kernel<kModeWeighted, true, false>(grid, block, lds, as, s, d_jobs, a, m);
+        else launch_kernel<kModeWeighted, false, false>(grid, block, lds, as, s, d_jobs, a, m);
     } else if (L.h_count) {
-        launch_kernel<kModeExact, false, true>(grid, block, lds, s, d_jobs, a, m);
+        launch_kernel<kModeExact, false, true>(grid, block, lds, as, s, d_jobs, a, m);
     } else {
-        if (st) launch_kernel<kModeExact, true, false>(grid, block, lds, s, d_jobs, a, m);
-        else launch_kernel<kModeExact, false, false>(grid, block, lds, s, d_jobs, a, m);
+        if (st) launch_kernel<kModeExact, true, false>(grid, block, lds, as, s, d_jobs, a, m);
+        else launch_kernel<kModeExact, false, false>(grid, block, lds, as, s, d_jobs, a, m);
     }
     return check_hip(hipGetLastError(), "walk_kernel");
 }
