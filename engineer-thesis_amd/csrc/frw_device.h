@@ -336,33 +336,24 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
         if (l > t.start_levels - 1) l = t.start_levels - 1;
         const int shift = t.morton_bits - l;
         const int32_t* tab = t.start_tab + start_offset(l);
-        const int64_t cx0 = x0 >> shift, cx1 = x1 >> shift, cy0 = y0 >> shift, cy1 = y1 >> shift;
-        const int64_t pcx = clampk(floor(ux)) >> shift, pcy = clampk(floor(uy)) >> shift;
-        // p's cell, then the window's other cells; all entries loaded at once
-        int64_t pos[4];
-        int np = 0;
-        pos[np++] = (pcy << l) + pcx;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int64_t qx = (k & 1) ? cx1 : cx0, qy = (k & 2) ? cy1 : cy0;
-            if (((k & 1) && cx1 == cx0) || ((k & 2) && cy1 == cy0) || (qx == pcx && qy == pcy)) continue;
-            pos[np++] = (qy << l) + qx;
-        }
-        int32_t e[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) e[k] = gload(tab + pos[k < np ? k : 0]);
-        if (cnt) cnt->tabs += (uint32_t)np;
-        const int own = e[0];
-        int cand[3];
-        int nc = 0;
-#pragma unroll
-        for (int k = 1; k < 4; ++k) {
-            if (k >= np) break;
-            bool dup = e[k] < 0 || e[k] == own;
-            for (int j = 0; j < nc; ++j) dup = dup || cand[j] == e[k];
-            if (!dup) cand[nc++] = e[k];
-        }
-        for (int j = 0; j < nc; ++j) st.at(sp++) = cand[j];
+        // the window covers at most 2 x 2 cells (cx1 - cx0 <= 1, cy1 - cy0 <= 1)
+        // and p's own cell is one of its corners.  The four corner entries are
+        // loaded at once (a repeated corner re-reads the same entry) and kept
+        // in registers: no per-lane array, which would live in scratch memory.
+        const int cx0 = (int)(x0 >> shift), cx1 = (int)(x1 >> shift);
+        const int cy0 = (int)(y0 >> shift), cy1 = (int)(y1 >> shift);
+        const bool ox = (int)(clampk(floor(ux)) >> shift) != cx0, oy = (int)(clampk(floor(uy)) >> shift) != cy0;
+        const int32_t e0 = gload(tab + ((cy0 << l) + cx0)), e1 = gload(tab + ((cy0 << l) + cx1));
+        const int32_t e2 = gload(tab + ((cy1 << l) + cx0)), e3 = gload(tab + ((cy1 << l) + cx1));
+        if (cnt) cnt->tabs += (uint32_t)((cx1 != cx0 ? 2 : 1) * (cy1 != cy0 ? 2 : 1));
+        const int32_t own = oy ? (ox ? e3 : e2) : (ox ? e1 : e0);
+        // the other cells' nodes in corner order, each once (a repeated corner
+        // equals an earlier entry, p's cell equals own), then p's own cell,
+        // which is popped first
+        if (e0 >= 0 && e0 != own) st.at(sp++) = e0;
+        if (e1 >= 0 && e1 != own && e1 != e0) st.at(sp++) = e1;
+        if (e2 >= 0 && e2 != own && e2 != e0 && e2 != e1) st.at(sp++) = e2;
+        if (e3 >= 0 && e3 != own && e3 != e0 && e3 != e1 && e3 != e2) st.at(sp++) = e3;
         if (own >= 0) st.at(sp++) = own;  // p's cell first
     } else {
         st.at(sp++) = 0;
