@@ -18,6 +18,7 @@ struct frw_batch {
     int64_t counts_size = 0;
     uint64_t total_walks = 0;
     int max_depth = 1;
+    int max_nodes = 0;
     WalkJob* d_jobs = nullptr;
     unsigned long long* d_work = nullptr;
     int* d_err = nullptr;
@@ -84,6 +85,7 @@ WalkLaunch launch_of(const frw_batch* b, const frw_batch_params* p)
     L.n_jobs = (int)b->jobs.size();
     L.total_walks = b->total_walks;
     L.max_depth = b->max_depth;
+    L.max_nodes = b->max_nodes;
     L.mode = p->mode;
     L.block = p->block;
     L.grid = p->grid;
@@ -139,6 +141,7 @@ int frw_batch_create(const frw_tree* const* trees, int32_t n_trees, const frw_ba
             b->counts_size += t->n_rects + 2;
             b->total_walks += (uint64_t)q.n_walks;
             b->max_depth = std::max(b->max_depth, (int)t->depth);
+            b->max_nodes = std::max(b->max_nodes, (int)t->n_nodes);
         }
         uint32_t T[kNSample];
         green_thresholds(T);

@@ -199,7 +199,7 @@ struct DevTree {
     int32_t n_rects;
     int32_t n_nodes;
     int32_t depth;         // number of levels
-    int32_t stack_cap;     // per-lane traversal stack entries
+    int32_t stack_cap;     // per-lane traversal stack entries (of the stack layout in use)
     double margin;         // ambiguity band of the Q3 schedule emulation
     double dist0;          // 0.05 * max(W, H), first Q3 step
     // start table of the nearest-rect search (start_levels = 0: none): for
@@ -238,26 +238,93 @@ struct Counters {
     uint32_t tabs;   // start-table entries (4 B)
 };
 
-// Per-lane traversal stack in LDS: stack[slot * stride + lane_in_block].
-struct LaneStack {
-    int32_t* base;
-    int stride;
+// Per-lane traversal stacks in LDS: slot i of a lane at base[i * stride].
+// Two layouts with one interface and one visiting order (a node's children:
+// `lead` first when pushed, then the others highest index first):
+//  * PlainStack holds one node per entry: 3 depth + 2 entries for a tree of
+//    `depth` levels (three pending siblings per level, plus the start nodes);
+//  * RunStack holds one run of pending siblings per entry, packed as
+//    (first sibling << 6) | (lead << 4) | (mask of siblings left): depth + 6
+//    entries, for trees of fewer than kMaxTreeNodes nodes, at a few more
+//    instructions per pop.
+// The walk kernel takes RunStack where PlainStack's LDS would limit the
+// resident waves.  Built with FRW_STACK_CHECK (host builds), every slot
+// access is checked against cap and an overflow aborts.
+constexpr int kMaxTreeNodes = 1 << 25;
+
+#ifdef FRW_STACK_CHECK
+#define FRW_STACK_FIELDS                                             \
+    int32_t* base;                                                   \
+    int stride;                                                      \
+    int cap;                                                         \
+    __host__ __device__ int32_t& at(int slot) const                  \
+    {                                                                \
+        if (slot < 0 || slot >= cap) __builtin_trap();               \
+        return base[slot * stride];                                  \
+    }
+#else
+#define FRW_STACK_FIELDS                                             \
+    int32_t* base;                                                   \
+    int stride;                                                      \
     __host__ __device__ int32_t& at(int slot) const { return base[slot * stride]; }
+#endif
+
+struct PlainStack {
+    FRW_STACK_FIELDS
+    __host__ __device__ static constexpr int entries(int depth) { return 3 * depth + 2; }
+    __host__ __device__ void push_node(int& sp, int node) const { at(sp++) = node; }
+    __host__ __device__ void push_children(int& sp, int first_child, uint32_t mask, int lead) const
+    {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (c != lead && ((mask >> c) & 1u)) at(sp++) = first_child + c;
+        if ((mask >> lead) & 1u) at(sp++) = first_child + lead;
+    }
+    __host__ __device__ int pop(int& sp) const { return at(--sp); }
+    // children offered one at a time (first_child + c in the order 1, 2, 3,
+    // 0: popped 0, 3, 2, 1), then flushed
+    __host__ __device__ void offer(int& sp, uint32_t&, int first_child, int c) const { at(sp++) = first_child + c; }
+    __host__ __device__ void flush(int&, uint32_t, int) const {}
 };
+
+struct RunStack {
+    FRW_STACK_FIELDS
+    __host__ __device__ static constexpr int entries(int depth) { return depth + 6; }
+    __host__ __device__ void push_node(int& sp, int node) const { at(sp++) = (int32_t)(((uint32_t)node << 6) | 1u); }
+    __host__ __device__ void push_children(int& sp, int first_child, uint32_t mask, int lead) const
+    {
+        if (mask) at(sp++) = (int32_t)(((uint32_t)first_child << 6) | ((uint32_t)lead << 4) | mask);
+    }
+    __host__ __device__ int pop(int& sp) const
+    {
+        const uint32_t e = (uint32_t)at(sp - 1);
+        const uint32_t m = e & 15u, f = (e >> 4) & 3u;
+        const uint32_t c = ((m >> f) & 1u) ? f : 31u - (uint32_t)__builtin_clz(m);
+        const uint32_t rest = m ^ (1u << c);
+        at(sp - 1) = (int32_t)(e ^ (1u << c));  // dead once the run is spent
+        sp -= rest == 0u ? 1 : 0;
+        return (int)((e >> 6) + c);
+    }
+    __host__ __device__ void offer(int&, uint32_t& run, int, int c) const { run |= 1u << c; }
+    __host__ __device__ void flush(int& sp, uint32_t run, int first_child) const { push_children(sp, first_child, run, 0); }
+};
+#undef FRW_STACK_FIELDS
+
+using LaneStack = PlainStack;
 
 // Q1 (checkCollisions, randomWalk/CPU/quadTree.cpp:148-198 with the full
 // depth-first search of randomWalk/Cuda/d_quadTree.cu:136-193): is the closed
 // box outside the root bounds, or does it intersect any rect (other than
 // `ignore`, by value)?
-template <bool kIgnore>
-__host__ __device__ inline bool q1_box(const DevTree& t, const LaneStack& st, double x0, double y0,
+template <bool kIgnore, class Stack>
+__host__ __device__ inline bool q1_box(const DevTree& t, const Stack& st, double x0, double y0,
                               double x1, double y1, int4 ignore, Counters* cnt)
 {
     if (!in_bounds_box(t.bounds, x0, y0, x1, y1)) return true;
     int sp = 0;
-    st.at(sp++) = 0;
+    st.push_node(sp, 0);
     while (sp > 0) {
-        const int ni = st.at(--sp);
+        const int ni = st.pop(sp);
         const Node nd = gload(t.nodes + ni);
         if (cnt) cnt->nodes += nd.first_child >= 0 ? 5u : 1u;
         for (int r = nd.rbeg; r < nd.rend; ++r) {
@@ -267,11 +334,14 @@ __host__ __device__ inline bool q1_box(const DevTree& t, const LaneStack& st, do
                 return true;
         }
         if (nd.first_child >= 0) {
+            uint32_t run = 0;
 #pragma unroll
-            for (int c = 3; c >= 0; --c) {
+            for (int k = 1; k <= 4; ++k) {
+                const int c = k & 3;
                 const int4 cb = gload(&t.nodes[nd.first_child + c].box);
-                if (box_hits_rect(cb, x0, y0, x1, y1)) st.at(sp++) = nd.first_child + c;
+                if (box_hits_rect(cb, x0, y0, x1, y1)) st.offer(sp, run, nd.first_child, c);
             }
+            st.flush(sp, run, nd.first_child);
         }
     }
     return false;
@@ -288,7 +358,8 @@ __host__ __device__ inline bool q1_box(const DevTree& t, const LaneStack& st, do
 // (optional) receives the sorted index of a nearest rect.  With hint_rect
 // the hint's coordinates come from (and the nearest rect's go back to) the
 // caller's registers instead of a load.
-__host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack& st, double px, double py,
+template <class Stack>
+__host__ __device__ inline double nearest_rect(const DevTree& t, const Stack& st, double px, double py,
                                       Counters* cnt, int hint = -1, int* arg = nullptr, int4* hint_rect = nullptr)
 {
     // node pruning and ordering use integer lower bounds (an integer lb is
@@ -350,16 +421,16 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
         // the other cells' nodes in corner order, each once (a repeated corner
         // equals an earlier entry, p's cell equals own), then p's own cell,
         // which is popped first
-        if (e0 >= 0 && e0 != own) st.at(sp++) = e0;
-        if (e1 >= 0 && e1 != own && e1 != e0) st.at(sp++) = e1;
-        if (e2 >= 0 && e2 != own && e2 != e0 && e2 != e1) st.at(sp++) = e2;
-        if (e3 >= 0 && e3 != own && e3 != e0 && e3 != e1 && e3 != e2) st.at(sp++) = e3;
-        if (own >= 0) st.at(sp++) = own;  // p's cell first
+        if (e0 >= 0 && e0 != own) st.push_node(sp, e0);
+        if (e1 >= 0 && e1 != own && e1 != e0) st.push_node(sp, e1);
+        if (e2 >= 0 && e2 != own && e2 != e0 && e2 != e1) st.push_node(sp, e2);
+        if (e3 >= 0 && e3 != own && e3 != e0 && e3 != e1 && e3 != e2) st.push_node(sp, e3);
+        if (own >= 0) st.push_node(sp, own);  // p's cell first
     } else {
-        st.at(sp++) = 0;
+        st.push_node(sp, 0);
     }
     while (sp > 0) {
-        const int ni = st.at(--sp);
+        const int ni = st.pop(sp);
         const Node nd = gload(t.nodes + ni);
         if (cnt) cnt->nodes++;
         if (!(linf_lb(nd.box, fx, cx, fy, cy) < lim)) continue;
@@ -388,15 +459,16 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
             int lb[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) lb[c] = linf_lb(gload(&t.nodes[nd.first_child + c].box), fx, cx, fy, cy);
-            // the nearest child is pushed last (popped first); the others in order
+            // the nearest child first, then the others
             int near = 0;
+            uint32_t mask = 0;
 #pragma unroll
             for (int c = 1; c < 4; ++c)
                 if (lb[c] < lb[near]) near = c;
 #pragma unroll
             for (int c = 0; c < 4; ++c)
-                if (c != near && lb[c] < lim) st.at(sp++) = nd.first_child + c;
-            if (lb[near] < lim) st.at(sp++) = nd.first_child + near;
+                if (lb[c] < lim) mask |= 1u << c;
+            st.push_children(sp, nd.first_child, mask, near);
         }
     }
     if (arg) *arg = besti;
@@ -407,15 +479,16 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const LaneStack
 // Q2 (checkCollisons(point), randomWalk/CPU/quadTree.cpp:240-262, answered
 // exactly): original index of the rect containing p (closed), deepest
 // containment first, then lowest index; -1 if none.
-__host__ __device__ inline int q2_point(const DevTree& t, const LaneStack& st, double px, double py,
+template <class Stack>
+__host__ __device__ inline int q2_point(const DevTree& t, const Stack& st, double px, double py,
                                Counters* cnt)
 {
     double best = 0.0;
     int best_id = -1;
     int sp = 0;
-    st.at(sp++) = 0;
+    st.push_node(sp, 0);
     while (sp > 0) {
-        const int ni = st.at(--sp);
+        const int ni = st.pop(sp);
         const Node nd = gload(t.nodes + ni);
         if (cnt) {
             cnt->nodes += nd.first_child >= 0 ? 5u : 1u;
@@ -432,10 +505,14 @@ __host__ __device__ inline int q2_point(const DevTree& t, const LaneStack& st, d
             }
         }
         if (nd.first_child >= 0) {
+            uint32_t run = 0;
 #pragma unroll
-            for (int c = 3; c >= 0; --c)
+            for (int k = 1; k <= 4; ++k) {
+                const int c = k & 3;
                 if (linf(gload(&t.nodes[nd.first_child + c].box), px, py) <= 0.0)
-                    st.at(sp++) = nd.first_child + c;
+                    st.offer(sp, run, nd.first_child, c);
+            }
+            st.flush(sp, run, nd.first_child);
         }
     }
     return best_id;
@@ -451,8 +528,8 @@ __host__ __device__ inline int q2_point(const DevTree& t, const LaneStack& st, d
 // from p collides, every side below d*-margin does not, and only boxes inside
 // that band (rounding-ambiguous) fall back to an exact Q1.  kLiteral runs a
 // Q1 for every test instead (the reference algorithm as is).
-template <bool kLiteral>
-__host__ __device__ inline DBox biggest_square(const DevTree& t, const LaneStack& st, double px, double py,
+template <bool kLiteral, class Stack>
+__host__ __device__ inline DBox biggest_square(const DevTree& t, const Stack& st, double px, double py,
                                       double dstar, uint32_t* n_q1, Counters* cnt, int* err)
 {
     const double MIN_DIST = (double)0.095f;
@@ -530,7 +607,8 @@ __host__ __device__ inline DBox gauss_surface(const int4& r, double fx, double f
     return DBox{mx + vx, my + vy, mx - vx, my - vy};
 }
 
-__host__ __device__ inline double gauss_factor(const DevTree& t, const LaneStack& st, const int4& r,
+template <class Stack>
+__host__ __device__ inline double gauss_factor(const DevTree& t, const Stack& st, const int4& r,
                                       double factor, bool axis_y)
 {
     bool dividing = true, first = true;

@@ -41,6 +41,7 @@ struct WalkLaunch {
     int n_jobs = 1;
     uint64_t total_walks = 0;
     int max_depth = 1;
+    int max_nodes = 0;  // largest tree's node count (the packed stack needs < kMaxTreeNodes)
     int mode = 0;
     int block = 0;
     int grid = 0;
@@ -154,7 +155,7 @@ int launch_q3(const frw_tree* t, const double* pts, int64_t n, int mode, double*
 int launch_gauss(const frw_tree* t, int4 master, double* d_out, hipStream_t s);
 
 // walk launchers (frw_walk.hip)
-int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds);
+int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds, bool* runs = nullptr);
 int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s);
 int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o,
                 hipStream_t s, unsigned long long* traffic = nullptr, const WalkExtras* extra = nullptr);

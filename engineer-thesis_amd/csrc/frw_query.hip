@@ -109,7 +109,7 @@ int plan_query(const frw_tree* t, int64_t n, QueryLaunch* q)
 
 }  // namespace
 
-int stack_bytes_per_lane(const frw_tree* t) { return (3 * t->depth + 2) * (int)sizeof(int32_t); }
+int stack_bytes_per_lane(const frw_tree* t) { return LaneStack::entries(t->depth) * (int)sizeof(int32_t); }
 
 DevTree dev_view(const frw_tree* t)
 {
@@ -121,7 +121,7 @@ DevTree dev_view(const frw_tree* t)
     v.n_rects = (int32_t)t->n_rects;
     v.n_nodes = (int32_t)t->n_nodes;
     v.depth = t->depth;
-    v.stack_cap = 3 * t->depth + 2;
+    v.stack_cap = LaneStack::entries(t->depth);
     v.margin = t->margin;
     v.dist0 = t->dist0;
     v.start_tab = t->d_start;

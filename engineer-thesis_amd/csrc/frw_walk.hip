@@ -109,9 +109,10 @@ __global__ void __launch_bounds__(64) walk_prologue(WalkJob* jobs, int n_jobs, u
     jb.start_rect = arg >= 0 ? jb.tree.rects[arg] : make_int4(0, 0, 0, 0);
 }
 
-template <int kMode, bool kStats, bool kHarvest>
+template <int kMode, bool kStats, bool kHarvest, bool kRuns>
 // At least three waves per SIMD: the learned instantiation then fits 168
-// VGPRs without spills (the others need fewer).
+// VGPRs without spills (the others need fewer).  kRuns: the traversal stack
+// holds runs of siblings (RunStack) instead of single nodes (PlainStack).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
 {
@@ -122,7 +123,8 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
     for (int i = threadIdx.x; i < min(a.n_jobs, kLdsJobs); i += blockDim.x) s_begin[i] = jobs[i].walk_begin;
     __syncthreads();
     const int lane = threadIdx.x & (kWaveSize - 1);
-    const LaneStack st{stack_mem + threadIdx.x, (int)blockDim.x};
+    using Stack = typename std::conditional<kRuns, RunStack, PlainStack>::type;
+    const Stack st{stack_mem + threadIdx.x, (int)blockDim.x};
     uint16_t* mlp_xs = nullptr;
     uint16_t* mlp_hs = nullptr;
     if (kMode == kModeLearned) {
@@ -477,11 +479,11 @@ struct AutoShape {
     uint64_t total = 0;
 };
 
-template <int kMode, bool kStats, bool kHarvest>
-void launch_kernel(int grid, int block, size_t lds, const AutoShape& as, hipStream_t s, const WalkJob* jobs,
-                   const WalkArgs& a, const MlpNet& m)
+template <int kMode, bool kStats, bool kHarvest, bool kRuns>
+void launch_kernel_as(int grid, int block, size_t lds, const AutoShape& as, hipStream_t s, const WalkJob* jobs,
+                      const WalkArgs& a, const MlpNet& m)
 {
-    auto kern = walk_kernel<kMode, kStats, kHarvest>;
+    auto kern = walk_kernel<kMode, kStats, kHarvest, kRuns>;
     if (as.on) {
         int best = 0;
         for (int b : {256, 192, 128, 64}) {
@@ -502,14 +504,33 @@ void launch_kernel(int grid, int block, size_t lds, const AutoShape& as, hipStre
     kern<<<grid, block, lds, s>>>(jobs, a, m);
 }
 
+template <int kMode, bool kStats, bool kHarvest>
+void launch_kernel(bool runs, int grid, int block, size_t lds, const AutoShape& as, hipStream_t s,
+                   const WalkJob* jobs, const WalkArgs& a, const MlpNet& m)
+{
+    if (runs) launch_kernel_as<kMode, kStats, kHarvest, true>(grid, block, lds, as, s, jobs, a, m);
+    else launch_kernel_as<kMode, kStats, kHarvest, false>(grid, block, lds, as, s, jobs, a, m);
+}
+
 }  // namespace
 
-int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds)
+// Stack layout of a walk launch: runs of siblings (RunStack) when the
+// single-node stack would hold fewer lanes per CU in LDS than the walk
+// kernels' registers allow (16 waves per CU, 12 for the learned and weighted
+// modes), single nodes (PlainStack, fewer instructions per pop) otherwise.
+bool walk_uses_runs(const WalkLaunch& L, size_t lds_cu, size_t per_wave, size_t fixed)
+{
+    if (L.max_nodes >= kMaxTreeNodes) return false;
+    const size_t waves = (L.mode == kModeLearned || L.mode == kModeWeighted) ? 12 : 16;
+    const size_t plain = (size_t)kWaveSize * PlainStack::entries(L.max_depth) * sizeof(int32_t) + per_wave;
+    return waves * plain + (waves / 4) * fixed > lds_cu;
+}
+
+int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds, bool* runs)
 {
     int b = L.block;
     if (b < 0 || b % kWaveSize != 0 || b > 256)
         return set_error(FRW_E_INVALID, "block must be a multiple of 64 and <= 256");
-    const size_t per_lane = (size_t)(3 * L.max_depth + 2) * sizeof(int32_t);
     const size_t per_wave = L.mode == kModeLearned ? (size_t)kMlpLdsPerWave : 0;
     const size_t fixed = kNSample * 4 + kLdsJobs * 8;
     int lds_block = 0, lds_cu = 0, cus = 0;
@@ -519,6 +540,10 @@ int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds)
         return set_error(FRW_E_DEVICE, "device attributes unavailable");
     if (lds_block <= 0) lds_block = 64 * 1024;
     if (lds_cu <= 0) lds_cu = lds_block;
+    const bool use_runs = walk_uses_runs(L, (size_t)lds_cu, per_wave, fixed);
+    if (runs) *runs = use_runs;
+    const size_t per_lane =
+        (size_t)(use_runs ? RunStack::entries(L.max_depth) : PlainStack::entries(L.max_depth)) * sizeof(int32_t);
     auto need = [&](int bb) { return (size_t)bb * per_lane + (size_t)(bb / kWaveSize) * per_wave; };
     if (b == 0) {
         b = 256;
@@ -545,8 +570,10 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
     if (L.total_walks == 0) return FRW_OK;
     int block = 0, grid = 0;
     size_t lds = 0;
-    int rc = plan_walk(L, &block, &grid, &lds);
+    bool runs = false;
+    int rc = plan_walk(L, &block, &grid, &lds, &runs);
     if (rc != FRW_OK) return rc;
+    const int stack_cap = runs ? RunStack::entries(L.max_depth) : PlainStack::entries(L.max_depth);
     AutoShape as;
     if (L.block == 0 && L.grid == 0) {
         int lds_block = 0, cus = 0;
@@ -554,14 +581,14 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, L.device) != hipSuccess)
             return set_error(FRW_E_DEVICE, "device attributes unavailable");
         as.on = true;
-        as.per_lane = (size_t)(3 * L.max_depth + 2) * sizeof(int32_t);
+        as.per_lane = (size_t)stack_cap * sizeof(int32_t);
         as.per_wave = L.mode == kModeLearned ? (size_t)kMlpLdsPerWave : 0;
         as.fixed = kNSample * 4 + kLdsJobs * 8;
         as.lds_block = (size_t)lds_block;
         as.cus = std::max(cus, 1);
         as.total = L.total_walks;
     }
-    const size_t plds = (size_t)64 * (3 * L.max_depth + 2) * sizeof(int32_t);
+    const size_t plds = (size_t)64 * PlainStack::entries(L.max_depth) * sizeof(int32_t);
     walk_prologue<<<(L.n_jobs + 63) / 64, 64, plds, s>>>(d_jobs, L.n_jobs, L.work);
     if ((rc = check_hip(hipGetLastError(), "walk_prologue")) != FRW_OK) return rc;
     WalkArgs a{};
@@ -573,7 +600,7 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
     a.total_walks = L.total_walks;
     a.max_hops = L.max_hops;
     a.n_jobs = L.n_jobs;
-    a.stack_cap = 3 * L.max_depth + 2;
+    a.stack_cap = stack_cap;
     a.h_pts = L.h_pts;
     a.h_prev = L.h_prev;
     a.h_dstar = L.h_dstar;
@@ -588,20 +615,20 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
     const bool st = L.traffic != nullptr;
     if (L.mode == kModeLearned) {
         if (!L.net) return set_error(FRW_E_INVALID, "learned mode needs a network");
-        if (st) launch_kernel<kModeLearned, true, false>(grid, block, lds, as, s, d_jobs, a, m);
-        else launch_kernel<kModeLearned, false, false>(grid, block, lds, as, s, d_jobs, a, m);
+        if (st) launch_kernel<kModeLearned, true, false>(runs, grid, block, lds, as, s, d_jobs, a, m);
+        else launch_kernel<kModeLearned, false, false>(runs, grid, block, lds, as, s, d_jobs, a, m);
     } else if (L.mode == kModeBisect) {
-        if (st) launch_kernel<kModeBisect, true, false>(grid, block, lds, as, s, d_jobs, a, m);
-        else launch_kernel<kModeBisect, false, false>(grid, block, lds, as, s, d_jobs, a, m);
+        if (st) launch_kernel<kModeBisect, true, false>(runs, grid, block, lds, as, s, d_jobs, a, m);
+        else launch_kernel<kModeBisect, false, false>(runs, grid, block, lds, as, s, d_jobs, a, m);
     } else if (L.mode == kModeWeighted) {
         if (!L.wtab) return set_error(FRW_E_INVALID, "weighted mode needs its tables");
-        if (st) launch_kernel<kModeWeighted, true, false>(grid, block, lds, as, s, d_jobs, a, m);
-        else launch_kernel<kModeWeighted, false, false>(grid, block, lds, as, s, d_jobs, a, m);
+        if (st) launch_kernel<kModeWeighted, true, false>(runs, grid, block, lds, as, s, d_jobs, a, m);
+        else launch_kernel<kModeWeighted, false, false>(runs, grid, block, lds, as, s, d_jobs, a, m);
     } else if (L.h_count) {
-        launch_kernel<kModeExact, false, true>(grid, block, lds, as, s, d_jobs, a, m);
+        launch_kernel<kModeExact, false, true>(runs, grid, block, lds, as, s, d_jobs, a, m);
     } else {
-        if (st) launch_kernel<kModeExact, true, false>(grid, block, lds, as, s, d_jobs, a, m);
-        else launch_kernel<kModeExact, false, false>(grid, block, lds, as, s, d_jobs, a, m);
+        if (st) launch_kernel<kModeExact, true, false>(runs, grid, block, lds, as, s, d_jobs, a, m);
+        else launch_kernel<kModeExact, false, false>(runs, grid, block, lds, as, s, d_jobs, a, m);
     }
     return check_hip(hipGetLastError(), "walk_kernel");
 }
@@ -640,6 +667,7 @@ int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outp
     L.n_jobs = 1;
     L.total_walks = (uint64_t)p->n_walks;
     L.max_depth = t->depth;
+    L.max_nodes = (int)t->n_nodes;
     L.mode = p->mode;
     L.block = p->block;
     L.grid = p->grid;

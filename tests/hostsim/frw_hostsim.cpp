@@ -148,7 +148,7 @@ DevTree view(const HostTree& t)
     v.n_rects = (int32_t)t.rects.size();
     v.n_nodes = (int32_t)t.nodes.size();
     v.depth = t.depth;
-    v.stack_cap = 3 * t.depth + 2;
+    v.stack_cap = LaneStack::entries(t.depth);
     v.margin = t.margin;
     v.dist0 = t.dist0;
     v.start_tab = t.start.data();
@@ -158,9 +158,22 @@ DevTree view(const HostTree& t)
     return v;
 }
 
+// Stack layout of the traversals below: 0 single nodes (PlainStack), 1 runs
+// of siblings (RunStack), as the walk kernel chooses per launch.
+int g_runs = 0;
+
+template <class S>
+struct HostStack {
+    std::vector<int32_t> mem;
+    S st;
+    explicit HostStack(int depth) : mem(S::entries(depth)), st{mem.data(), 1, S::entries(depth)} {}
+};
+
 }  // namespace
 
 extern "C" {
+
+void hs_set_stack(int runs) { g_runs = runs; }
 
 void* hs_build(const int32_t* xyxy, int n, const int32_t* bounds, int cap)
 {
@@ -187,14 +200,17 @@ void hs_export(void* h, int32_t* nodes, int32_t* rects, int32_t* ids)
     std::memcpy(ids, t->ids.data(), t->ids.size() * sizeof(int32_t));
 }
 
-// nearest distance, Q2 id, Q3 box (mode 0 replayed, 1 literal), max stack depth used
-void hs_queries(void* h, const double* pts, int n, double* dist, int32_t* q2, double* q3e, double* q3b,
+}  // extern "C"
+
+// nearest distance, Q2 id, Q3 box (replayed and literal), Q1 fallbacks of the replay
+template <class S>
+static void hs_queries_impl(void* h, const double* pts, int n, double* dist, int32_t* q2, double* q3e, double* q3b,
                 int32_t* n_fallback)
 {
     const HostTree* t = static_cast<HostTree*>(h);
     const DevTree v = view(*t);
-    std::vector<int32_t> stack(v.stack_cap + 8);
-    LaneStack st{stack.data(), 1};
+    HostStack<S> hs_(t->depth);
+    const S& st = hs_.st;
     int err = 0;
     for (int i = 0; i < n; ++i) {
         const double px = pts[2 * i], py = pts[2 * i + 1];
@@ -209,14 +225,26 @@ void hs_queries(void* h, const double* pts, int n, double* dist, int32_t* q2, do
     }
 }
 
+extern "C" {
+
+void hs_queries(void* h, const double* pts, int n, double* dist, int32_t* q2, double* q3e, double* q3b,
+                int32_t* n_fallback)
+{
+    if (g_runs) hs_queries_impl<RunStack>(h, pts, n, dist, q2, q3e, q3b, n_fallback);
+    else hs_queries_impl<PlainStack>(h, pts, n, dist, q2, q3e, q3b, n_fallback);
+}
+
+}  // extern "C"
+
 // Traffic of the nearest-rect search and of a Q1 probe of the box p +- f*d*
 // (per point: nodes/rects of each), for sizing a proposal-and-verify step.
-void hs_probe_cost(void* h, const double* pts, int n, double f, int64_t* out)
+template <class S>
+static void hs_probe_cost_impl(void* h, const double* pts, int n, double f, int64_t* out)
 {
     const HostTree* t = static_cast<HostTree*>(h);
     const DevTree v = view(*t);
-    std::vector<int32_t> stack(v.stack_cap + 8);
-    LaneStack st{stack.data(), 1};
+    HostStack<S> hs_(t->depth);
+    const S& st = hs_.st;
     int64_t nn = 0, nr = 0, pn = 0, pr = 0, ok = 0;
     for (int i = 0; i < n; ++i) {
         const double px = pts[2 * i], py = pts[2 * i + 1];
@@ -229,12 +257,23 @@ void hs_probe_cost(void* h, const double* pts, int n, double f, int64_t* out)
     out[0] = nn; out[1] = nr; out[2] = pn; out[3] = pr; out[4] = ok;
 }
 
-void hs_gauss(void* h, int master, double* out)
+extern "C" {
+
+void hs_probe_cost(void* h, const double* pts, int n, double f, int64_t* out)
+{
+    if (g_runs) hs_probe_cost_impl<RunStack>(h, pts, n, f, out);
+    else hs_probe_cost_impl<PlainStack>(h, pts, n, f, out);
+}
+
+}  // extern "C"
+
+template <class S>
+static void hs_gauss_impl(void* h, int master, double* out)
 {
     const HostTree* t = static_cast<HostTree*>(h);
     const DevTree v = view(*t);
-    std::vector<int32_t> stack(v.stack_cap + 8);
-    LaneStack st{stack.data(), 1};
+    HostStack<S> hs_(t->depth);
+    const S& st = hs_.st;
     int4 m = make_int4(0, 0, 0, 0);
     for (size_t i = 0; i < t->ids.size(); ++i)
         if (t->ids[i] == master) m = t->rects[i];
@@ -242,6 +281,14 @@ void hs_gauss(void* h, int master, double* out)
     const double fy = gauss_factor(v, st, m, 1.5, true);
     const DBox s = gauss_surface(m, fx, fy);
     out[0] = s.x0; out[1] = s.y0; out[2] = s.x1; out[3] = s.y1;
+}
+
+extern "C" {
+
+void hs_gauss(void* h, int master, double* out)
+{
+    if (g_runs) hs_gauss_impl<RunStack>(h, master, out);
+    else hs_gauss_impl<PlainStack>(h, master, out);
 }
 
 int64_t hs_walk_hinted(void* h, int master, int mode, uint64_t seed, uint32_t job, uint32_t first_walk,
@@ -257,15 +304,18 @@ int64_t hs_walk(void* h, int master, int mode, uint64_t seed, uint32_t job, uint
                           traffic, 0);
 }
 
+}  // extern "C"
+
 // use_hint: seed each hop's nearest-rect search with the previous hop's nearest rect.
-int64_t hs_walk_hinted(void* h, int master, int mode, uint64_t seed, uint32_t job, uint32_t first_walk,
+template <class S>
+static int64_t hs_walk_hinted_impl(void* h, int master, int mode, uint64_t seed, uint32_t job, uint32_t first_walk,
                        int64_t n_walks, uint32_t max_hops, const uint32_t* T, int32_t* hit_out,
                        int32_t* hops_out, int64_t* traffic, int use_hint)
 {
     const HostTree* t = static_cast<HostTree*>(h);
     const DevTree v = view(*t);
-    std::vector<int32_t> stack(v.stack_cap + 8);
-    LaneStack st{stack.data(), 1};
+    HostStack<S> hs_(t->depth);
+    const S& st = hs_.st;
     double surf[4];
     hs_gauss(h, master, surf);
     // the walk kernel's first-hop hint: a rect at the master's centre
@@ -331,6 +381,16 @@ int64_t hs_walk_hinted(void* h, int master, int mode, uint64_t seed, uint32_t jo
         traffic[2] = cnt.tabs;
     }
     return err ? -1 : total;
+}
+
+extern "C" {
+
+int64_t hs_walk_hinted(void* h, int master, int mode, uint64_t seed, uint32_t job, uint32_t first_walk,
+                       int64_t n_walks, uint32_t max_hops, const uint32_t* T, int32_t* hit_out,
+                       int32_t* hops_out, int64_t* traffic, int use_hint)
+{
+    if (g_runs) return hs_walk_hinted_impl<RunStack>(h, master, mode, seed, job, first_walk, n_walks, max_hops, T, hit_out, hops_out, traffic, use_hint);
+    else return hs_walk_hinted_impl<PlainStack>(h, master, mode, seed, job, first_walk, n_walks, max_hops, T, hit_out, hops_out, traffic, use_hint);
 }
 
 }  // extern "C"

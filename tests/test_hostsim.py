@@ -37,6 +37,7 @@ def hostsim():
     L.hs_walk.restype = C.c_int64
     L.hs_walk.argtypes = [vp, C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64,
                           C.c_uint32, vp, vp, vp, vp]
+    L.hs_set_stack.argtypes = [C.c_int]
     L.hs_walk_hinted.restype = C.c_int64
     L.hs_walk_hinted.argtypes = [vp, C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64,
                                  C.c_uint32, vp, vp, vp, vp, C.c_int]
@@ -44,8 +45,12 @@ def hostsim():
 
 
 class HostTree:
-    def __init__(self, lay, cap=16):
+    """runs: traverse with the packed sibling-run stack (RunStack) instead of
+    the single-node one (PlainStack)."""
+
+    def __init__(self, lay, cap=16, runs=0):
         self.L = hostsim()
+        self.runs = int(runs)
         self.r = np.ascontiguousarray(lay.rects())
         self.b = np.ascontiguousarray(lay.bounds())
         self.h = self.L.hs_build(self.r.ctypes.data, len(self.r), self.b.ctypes.data, cap)
@@ -70,6 +75,7 @@ class HostTree:
         pts = np.ascontiguousarray(pts, dtype=np.float64)
         out = (np.zeros(n), np.zeros(n, np.int32), np.zeros((n, 4)), np.zeros((n, 4)),
                np.zeros(n, np.int32))
+        self.L.hs_set_stack(self.runs)
         self.L.hs_queries(self.h, pts.ctypes.data, n, *[a.ctypes.data for a in out])
         return out
 
@@ -77,6 +83,7 @@ class HostTree:
         T = frw.green_thresholds()
         hit = np.zeros(n, np.int32)
         hp = np.zeros(n, np.int32)
+        self.L.hs_set_stack(self.runs)
         tot = self.L.hs_walk(self.h, master, mode, seed, job, first, n, max_hops, T.ctypes.data,
                              hit.ctypes.data, hp.ctypes.data, None)
         assert tot >= 0, "bisection error flag"
@@ -90,6 +97,7 @@ class HostTree:
         hit = np.zeros(n, np.int32)
         hp = np.zeros(n, np.int32)
         tr = np.zeros(3, np.int64)
+        self.L.hs_set_stack(self.runs)
         tot = self.L.hs_walk_hinted(self.h, master, mode, seed, job, first, n, max_hops, T.ctypes.data,
                                     hit.ctypes.data, hp.ctypes.data, tr.ctypes.data, int(use_hint))
         assert tot >= 0, "bisection error flag"
@@ -98,9 +106,10 @@ class HostTree:
 
 @pytest.mark.parametrize("name,master", FIXTURES)
 @pytest.mark.parametrize("cap", [1, 16, 64])
-def test_hostsim_walks_replay_oracle(oracle_lib, name, master, cap):
+@pytest.mark.parametrize("runs", [0, 1])
+def test_hostsim_walks_replay_oracle(oracle_lib, name, master, cap, runs):
     lay = frw.Layout.load(layout_path(name))
-    ht = HostTree(lay, cap)
+    ht = HostTree(lay, cap, runs)
     ix = O.Index(lay.rects().astype(np.float64), tuple(float(v) for v in lay.bounds()),
                  O.REFTREE_DFS)
     _, tot_o, ohit, ohp = ix.walk(master, 1500, per_walk=True)
@@ -110,11 +119,12 @@ def test_hostsim_walks_replay_oracle(oracle_lib, name, master, cap):
 
 
 @pytest.mark.parametrize("name,master", FIXTURES[:4])
-def test_hostsim_queries_and_gauss(oracle_lib, name, master):
+@pytest.mark.parametrize("runs", [0, 1])
+def test_hostsim_queries_and_gauss(oracle_lib, name, master, runs):
     lay = frw.Layout.load(layout_path(name))
     r = lay.rects()
     b = tuple(float(v) for v in lay.bounds())
-    ht = HostTree(lay, 4)
+    ht = HostTree(lay, 4, runs)
     pts = query_points(r, b, 1500, seed=2)
     dist, q2, q3e, q3b, nfb = ht.queries(pts)
     ex = O.Index(r.astype(np.float64), b, O.EXACT)
@@ -125,6 +135,7 @@ def test_hostsim_queries_and_gauss(oracle_lib, name, master):
         if i % 5 == 0:
             assert tuple(q3e[i]) == ex.q3(px, py)[0]
     g = np.zeros(4)
+    ht.L.hs_set_stack(runs)
     ht.L.hs_gauss(ht.h, master, g.ctypes.data)
     assert tuple(g) == ex.gauss_from(master)
 
@@ -157,3 +168,36 @@ def test_hostsim_tree_invariants():
         owned[s:e] += 1
     assert np.all(owned == 1)
     assert len(nodes) % 4 == 0 and info[1] >= 2
+
+
+@pytest.mark.parametrize("runs", [0, 1])
+def test_hostsim_deep_tree_stack_bound(runs):
+    """Deep trees (dense clusters, one rect per leaf): every traversal of the
+    queries and of exact, hinted and bisection walks stays within the stack
+    entries the kernels allocate for either stack layout (the host build
+    checks each stack access and aborts on overflow), and both layouts visit
+    the same nodes in the same order (equal traffic counts)."""
+    lay = frw.Layout.generate(20_000, 1_000_000, 5, 40, seed=9, cluster_frac=0.9, sigma=3e3,
+                              n_clusters=3)
+    r = lay.rects()
+    b = tuple(float(v) for v in lay.bounds())
+    ht = HostTree(lay, 1, runs)
+    _, _, _, info = ht.export()
+    assert info[1] >= 14, info
+    pts = query_points(r, b, 4000, seed=4)
+    dist, q2, q3e, q3b, nfb = ht.queries(pts)
+    assert np.array_equal(q3e, q3b)
+    ex = O.Index(r.astype(np.float64), b, O.EXACT)
+    for i in range(0, len(pts), 7):
+        assert dist[i] == ex.nearest(*pts[i])[0]
+        assert q2[i] == ex.q2(*pts[i])
+    hinted = ht.walk_counted(0, 400)
+    plain = ht.walk_counted(0, 400, use_hint=False)
+    assert hinted[2] == plain[2]
+    bis = ht.walk_counted(0, 400, mode=frw.MODE_BISECT)
+    hit_b, hp_b, tot_b = ht.walk(0, 400, mode=frw.MODE_BISECT)
+    hit_e, hp_e, tot_e = ht.walk(0, 400, mode=frw.MODE_EXACT)
+    assert np.array_equal(hit_b, hit_e) and tot_b == tot_e == hinted[2]
+    other = HostTree(lay, 1, 1 - runs)
+    assert other.walk_counted(0, 400) == hinted
+    assert other.walk_counted(0, 400, mode=frw.MODE_BISECT) == bis
