@@ -39,7 +39,8 @@ struct WalkArgs {
     uint64_t total_walks;
     uint32_t max_hops;
     int32_t n_jobs;
-    int32_t stack_cap;  // LDS stack entries per lane
+    int32_t wave_ints;  // LDS per wave in 32-bit words: its lanes' traversal stacks, which the
+                        // learned mode's activation tiles alias between traversals
     // harvest: (point, previous half-size, d*) of non-final exact hops
     double* h_pts;
     double* h_prev;
@@ -124,12 +125,15 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
     __syncthreads();
     const int lane = threadIdx.x & (kWaveSize - 1);
     using Stack = typename std::conditional<kRuns, RunStack, PlainStack>::type;
-    const Stack st{stack_mem + threadIdx.x, (int)blockDim.x};
+    // each wave's LDS: slot i of its lanes' stacks at [64 i, 64 i + 64); the
+    // learned mode's two activation tiles reuse it (no traversal is live
+    // while the wave evaluates the network)
+    int32_t* wave_lds = stack_mem + (size_t)(threadIdx.x / kWaveSize) * a.wave_ints;
+    const Stack st{wave_lds + lane, kWaveSize};
     uint16_t* mlp_xs = nullptr;
     uint16_t* mlp_hs = nullptr;
     if (kMode == kModeLearned) {
-        uint16_t* base = (uint16_t*)(stack_mem + (size_t)blockDim.x * a.stack_cap);
-        mlp_xs = base + (size_t)(threadIdx.x / kWaveSize) * (2 * 64 * kMlpStride);
+        mlp_xs = (uint16_t*)wave_lds;
         mlp_hs = mlp_xs + 64 * kMlpStride;
     }
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
@@ -474,7 +478,7 @@ __global__ void set_job(WalkJob* slot, WalkJob jb) { *slot = jb; }
 // registers and its LDS stack, and one such full set of blocks per CU.
 struct AutoShape {
     bool on = false;
-    size_t per_lane = 0, per_wave = 0, fixed = 0, lds_block = 0;
+    size_t wave_bytes = 0, fixed = 0, lds_block = 0;
     int cus = 1;
     uint64_t total = 0;
 };
@@ -487,7 +491,7 @@ void launch_kernel_as(int grid, int block, size_t lds, const AutoShape& as, hipS
     if (as.on) {
         int best = 0;
         for (int b : {256, 192, 128, 64}) {
-            const size_t l = (size_t)b * as.per_lane + (size_t)(b / kWaveSize) * as.per_wave;
+            const size_t l = (size_t)(b / kWaveSize) * as.wave_bytes;
             if (l + as.fixed > as.lds_block) continue;
             int nb = 0;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, b, l) != hipSuccess || nb <= 0) continue;
@@ -514,16 +518,27 @@ void launch_kernel(bool runs, int grid, int block, size_t lds, const AutoShape& 
 
 }  // namespace
 
+// LDS of one wave: its lanes' stacks, or the learned mode's activation
+// tiles when larger (the two alias).
+size_t walk_wave_bytes(const WalkLaunch& L, bool runs)
+{
+    const int entries = runs ? RunStack::entries(L.max_depth) : PlainStack::entries(L.max_depth);
+    const size_t stack = (size_t)kWaveSize * entries * sizeof(int32_t);
+    const size_t mlp = L.mode == kModeLearned ? (size_t)kMlpLdsPerWave : 0;
+    return std::max(stack, mlp);
+}
+
 // Stack layout of a walk launch: runs of siblings (RunStack) when the
 // single-node stack would hold fewer lanes per CU in LDS than the walk
 // kernels' registers allow (16 waves per CU, 12 for the learned and weighted
-// modes), single nodes (PlainStack, fewer instructions per pop) otherwise.
-bool walk_uses_runs(const WalkLaunch& L, size_t lds_cu, size_t per_wave, size_t fixed)
+// modes) and runs take less, single nodes (PlainStack, fewer instructions
+// per pop) otherwise.
+bool walk_uses_runs(const WalkLaunch& L, size_t lds_cu, size_t fixed)
 {
     if (L.max_nodes >= kMaxTreeNodes) return false;
     const size_t waves = (L.mode == kModeLearned || L.mode == kModeWeighted) ? 12 : 16;
-    const size_t plain = (size_t)kWaveSize * PlainStack::entries(L.max_depth) * sizeof(int32_t) + per_wave;
-    return waves * plain + (waves / 4) * fixed > lds_cu;
+    const size_t plain = walk_wave_bytes(L, false);
+    return walk_wave_bytes(L, true) < plain && waves * plain + (waves / 4) * fixed > lds_cu;
 }
 
 int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds, bool* runs)
@@ -531,7 +546,6 @@ int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds, bool* run
     int b = L.block;
     if (b < 0 || b % kWaveSize != 0 || b > 256)
         return set_error(FRW_E_INVALID, "block must be a multiple of 64 and <= 256");
-    const size_t per_wave = L.mode == kModeLearned ? (size_t)kMlpLdsPerWave : 0;
     const size_t fixed = kNSample * 4 + kLdsJobs * 8;
     int lds_block = 0, lds_cu = 0, cus = 0;
     if (hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, L.device) != hipSuccess ||
@@ -540,11 +554,10 @@ int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds, bool* run
         return set_error(FRW_E_DEVICE, "device attributes unavailable");
     if (lds_block <= 0) lds_block = 64 * 1024;
     if (lds_cu <= 0) lds_cu = lds_block;
-    const bool use_runs = walk_uses_runs(L, (size_t)lds_cu, per_wave, fixed);
+    const bool use_runs = walk_uses_runs(L, (size_t)lds_cu, fixed);
     if (runs) *runs = use_runs;
-    const size_t per_lane =
-        (size_t)(use_runs ? RunStack::entries(L.max_depth) : PlainStack::entries(L.max_depth)) * sizeof(int32_t);
-    auto need = [&](int bb) { return (size_t)bb * per_lane + (size_t)(bb / kWaveSize) * per_wave; };
+    const size_t wave_bytes = walk_wave_bytes(L, use_runs);
+    auto need = [&](int bb) { return (size_t)(bb / kWaveSize) * wave_bytes; };
     if (b == 0) {
         b = 256;
         while (b >= 64 && need(b) + fixed > (size_t)lds_block) b -= 64;
@@ -573,7 +586,7 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
     bool runs = false;
     int rc = plan_walk(L, &block, &grid, &lds, &runs);
     if (rc != FRW_OK) return rc;
-    const int stack_cap = runs ? RunStack::entries(L.max_depth) : PlainStack::entries(L.max_depth);
+    const size_t wave_bytes = walk_wave_bytes(L, runs);
     AutoShape as;
     if (L.block == 0 && L.grid == 0) {
         int lds_block = 0, cus = 0;
@@ -581,8 +594,7 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, L.device) != hipSuccess)
             return set_error(FRW_E_DEVICE, "device attributes unavailable");
         as.on = true;
-        as.per_lane = (size_t)stack_cap * sizeof(int32_t);
-        as.per_wave = L.mode == kModeLearned ? (size_t)kMlpLdsPerWave : 0;
+        as.wave_bytes = wave_bytes;
         as.fixed = kNSample * 4 + kLdsJobs * 8;
         as.lds_block = (size_t)lds_block;
         as.cus = std::max(cus, 1);
@@ -600,7 +612,7 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
     a.total_walks = L.total_walks;
     a.max_hops = L.max_hops;
     a.n_jobs = L.n_jobs;
-    a.stack_cap = stack_cap;
+    a.wave_ints = (int32_t)(wave_bytes / sizeof(int32_t));
     a.h_pts = L.h_pts;
     a.h_prev = L.h_prev;
     a.h_dstar = L.h_dstar;
