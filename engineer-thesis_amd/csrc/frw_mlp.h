@@ -38,6 +38,9 @@ constexpr int kMlpIn = 67;   // features (window 64, level, position in cell x a
 constexpr int kMlpInPad = 72;   // K of the first layer, a multiple of 8
 constexpr int kMlpHid = 64;
 constexpr int kMlpStride = 72;  // LDS row stride (bf16 elements) of activations
+// a wave's activation tile: 64 rows of kMlpStride, then 16 bf16 per lane of
+// staging for the layers' first halves
+constexpr int kMlpTileElems = 64 * kMlpStride + 64 * 16;
 constexpr int kMlpMaxLevel = 15;  // feature-window levels
 constexpr int kPyrMaxLevel = 15;  // bitmap / summed-area-table levels
 constexpr int kCertShift = 4;     // verification cells are at most d / 2^4
@@ -207,53 +210,64 @@ __device__ inline void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One hidden layer for the wave's 64 rows: out = bf16(gelu(in * W^T + b)).
-// Computed transposed (W * in^T, weights as the MFMA's A operand): each lane
-// then holds four consecutive output neurons of one row per group, stored
-// as one packed 8-byte write.
+// One hidden layer for the wave's 64 rows, in place: buf = bf16(gelu(buf *
+// W^T + b)).  Computed transposed (W * buf^T, weights as the MFMA's A
+// operand): each lane then holds four consecutive output neurons of one row
+// per group, stored as one packed 8-byte write.  Lane (r, q) of a 32-row
+// block reads exactly the elements 4q + 8k .. 4q + 8k + 3 of row r, and its
+// outputs are the same elements, so it overwrites only what it alone read;
+// both 32-neuron halves read the whole block, so the first half's outputs
+// wait in the lane's staging slots until the second half has been computed.
 template <int K>
-__device__ inline void mlp_layer(const uint16_t* in, uint16_t* out, const uint16_t* W, const float* bias,
-                                 int lane)
+__device__ inline void mlp_layer(uint16_t* buf, const uint16_t* W, const float* bias, int lane)
 {
     const int r32 = lane & 31, q = lane >> 5;
+    bf16x4* held = (bf16x4*)(buf + 64 * kMlpStride) + 4 * lane;
 #pragma unroll 1
-    for (int t = 0; t < 4; ++t) {
-        const int mi = t >> 1, nj = t & 1;
-        const uint16_t* xrow = in + (mi * 32 + r32) * kMlpStride + 4 * q;
-        const uint16_t* wrow = W + (nj * 32 + r32) * K + 4 * q;
-        f32x16 acc = {0};
+    for (int mi = 0; mi < 2; ++mi) {
+        uint16_t* row = buf + (mi * 32 + r32) * kMlpStride;
+#pragma unroll 1
+        for (int nj = 0; nj < 2; ++nj) {
+            const uint16_t* wrow = W + (nj * 32 + r32) * K + 4 * q;
+            f32x16 acc = {0};
 #pragma unroll 3
-        for (int kb = 0; kb < K / 8; ++kb) {
-            const bf16x4 x = *(const bf16x4*)(xrow + kb * 8);
-            const bf16x4 w = *(const bf16x4*)(wrow + kb * 8);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w, x, acc, 0, 0, 0);
-        }
-        uint16_t* orow = out + (mi * 32 + r32) * kMlpStride;
+            for (int kb = 0; kb < K / 8; ++kb) {
+                const bf16x4 x = *(const bf16x4*)(row + 4 * q + kb * 8);
+                const bf16x4 w = *(const bf16x4*)(wrow + kb * 8);
+                acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w, x, acc, 0, 0, 0);
+            }
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const int n0 = nj * 32 + 8 * g + 4 * q;
-            bf16x4 v;
+            for (int g = 0; g < 4; ++g) {
+                const int n0 = nj * 32 + 8 * g + 4 * q;
+                bf16x4 v;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = (short)f2bf(gelu_f(acc[4 * g + u] + bias[n0 + u]));
-            *(bf16x4*)(orow + n0) = v;
+                for (int u = 0; u < 4; ++u) v[u] = (short)f2bf(gelu_f(acc[4 * g + u] + bias[n0 + u]));
+                if (nj == 0)
+                    held[g] = v;
+                else
+                    *(bf16x4*)(row + n0) = v;
+            }
         }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) *(bf16x4*)(row + 8 * g + 4 * q) = held[g];
     }
 }
 
-// Forward pass of the wave's 64 feature rows in xs (LDS, 64 x kMlpStride);
-// hs is a second LDS buffer of the same size.  Returns row `lane`'s output.
+// Forward pass of the wave's 64 feature rows in xs (LDS, kMlpTileElems:
+// 64 x kMlpStride, overwritten by the hidden activations, and the staging
+// slots).  Returns row `lane`'s output.
 // Must be called by all 64 lanes together.
-__device__ inline float mlp_forward_wave(const MlpNet& m, uint16_t* xs, uint16_t* hs, int lane)
+__device__ inline float mlp_forward_wave(const MlpNet& m, uint16_t* xs, int lane)
 {
     wave_sync();
-    mlp_layer<kMlpInPad>(xs, hs, m.W1, m.b1, lane);
+    mlp_layer<kMlpInPad>(xs, m.W1, m.b1, lane);
     wave_sync();
-    mlp_layer<kMlpHid>(hs, xs, m.W2, m.b2, lane);
+    mlp_layer<kMlpHid>(xs, m.W2, m.b2, lane);
     wave_sync();
-    mlp_layer<kMlpHid>(xs, hs, m.W3, m.b3, lane);
+    mlp_layer<kMlpHid>(xs, m.W3, m.b3, lane);
     wave_sync();
     float y = 0.0f;
-    for (int j = 0; j < kMlpHid; ++j) y += bf2f(m.w4[j]) * bf2f(hs[lane * kMlpStride + j]);
+    for (int j = 0; j < kMlpHid; ++j) y += bf2f(m.w4[j]) * bf2f(xs[lane * kMlpStride + j]);
     wave_sync();
     return y + *m.b4;
 }

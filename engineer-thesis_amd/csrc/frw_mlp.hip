@@ -368,13 +368,12 @@ __global__ void gather_rows(const uint16_t* __restrict__ X, const float* __restr
 __global__ void __launch_bounds__(64) forward_waves(MlpNet m, const uint16_t* __restrict__ X, int64_t n,
                                                     float* __restrict__ y)
 {
-    __shared__ uint16_t xs[64 * kMlpStride];
-    __shared__ uint16_t hs[64 * kMlpStride];
+    __shared__ uint16_t xs[kMlpTileElems];
     const int lane = threadIdx.x;
     const int64_t row = (int64_t)blockIdx.x * 64 + lane;
     for (int k = 0; k < kMlpInPad; ++k)
         xs[lane * kMlpStride + k] = row < n ? X[row * kMlpInPad + k] : (uint16_t)0;
-    const float out = mlp_forward_wave(m, xs, hs, lane);
+    const float out = mlp_forward_wave(m, xs, lane);
     if (row < n) y[row] = out;
 }
 

@@ -40,7 +40,7 @@ struct WalkArgs {
     uint32_t max_hops;
     int32_t n_jobs;
     int32_t wave_ints;  // LDS per wave in 32-bit words: its lanes' traversal stacks, which the
-                        // learned mode's activation tiles alias between traversals
+                        // learned mode's activation tile aliases between traversals
     // harvest: (point, previous half-size, d*) of non-final exact hops
     double* h_pts;
     double* h_prev;
@@ -69,7 +69,7 @@ __device__ inline void kahan_add(double& s, double& c, double x)
     c = (t - s) - y;
     s = t;
 }
-constexpr int kMlpLdsPerWave = 2 * 64 * kMlpStride * 2;         // two bf16 activation tiles
+constexpr int kMlpLdsPerWave = kMlpTileElems * 2;               // the bf16 activation tile
 
 __device__ inline int find_job(const WalkJob* jobs, const uint64_t* s_begin, int n_jobs, uint64_t g)
 {
@@ -126,15 +126,13 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
     const int lane = threadIdx.x & (kWaveSize - 1);
     using Stack = typename std::conditional<kRuns, RunStack, PlainStack>::type;
     // each wave's LDS: slot i of its lanes' stacks at [64 i, 64 i + 64); the
-    // learned mode's two activation tiles reuse it (no traversal is live
+    // learned mode's activation tile reuses it (no traversal is live
     // while the wave evaluates the network)
     int32_t* wave_lds = stack_mem + (size_t)(threadIdx.x / kWaveSize) * a.wave_ints;
     const Stack st{wave_lds + lane, kWaveSize};
     uint16_t* mlp_xs = nullptr;
-    uint16_t* mlp_hs = nullptr;
     if (kMode == kModeLearned) {
         mlp_xs = (uint16_t*)wave_lds;
-        mlp_hs = mlp_xs + 64 * kMlpStride;
     }
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
 
@@ -337,7 +335,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
                 bits = mlp_window(pyr, level, cell, px, py, &fx, &fy);
             }
             mlp_feature_row(mlp_xs + lane * kMlpStride, bits, level, fx, fy);
-            y_mlp = mlp_forward_wave(net, mlp_xs, mlp_hs, lane);
+            y_mlp = mlp_forward_wave(net, mlp_xs, lane);
         }
         if (!active) continue;
 
