@@ -42,6 +42,9 @@ DESCR = {
           "step, graph replay, autotuned launch shape",
     "c5": "learned proposal at scale: the 1e6-rect layout, network trained on pairs harvested "
           "from exact walks, master 0, 1e8 walks/step, exact fallback",
+    "c6": "batched at scale: the 1e6-rect layout, 16 masters (rect 0 and 15 drawn uniformly), "
+          "walks per master U[5e5,2e6], one persistent launch per step, graph replay, autotuned "
+          "launch shape",
 }
 
 
@@ -54,7 +57,7 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["c4"], default="c2")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["c4", "c6"], default="c2")
     ap.add_argument("--mode", choices=["exact", "bisect", "learned"], default=None)
     ap.add_argument("--alpha", type=float, default=0.9, help="learned: safety factor")
     ap.add_argument("--pairs", type=int, default=2_000_000, help="learned: training pairs")
@@ -365,7 +368,7 @@ def main():
         torch.cuda.synchronize()
         log("copied 3 x 256 MiB")
         return
-    if args.workload == "c4":
+    if args.workload in ("c4", "c6"):
         return bench_batched(args, frw, torch, np)
     walks = args.walks or WORKLOADS[args.workload][7]
     master = 0
@@ -513,25 +516,39 @@ def main():
 
 
 def bench_batched(args, frw, torch, np):
-    """64 layouts, rect counts log-uniform in [1e2, 1e5] (density of the 1e4 /
-    1e5-box layout, sides U[10,500]), master 0 each, walks per layout
-    U[1e5, 1e6]: one persistent launch per step, captured in a graph, launch
-    shape from the autotuner.  Learned mode: one network trained on pairs
-    harvested from every layout, each job on its own layout's pyramid."""
+    """One persistent launch per step over many jobs, captured in a graph,
+    launch shape from the autotuner.  c4: 64 layouts, rect counts
+    log-uniform in [1e2, 1e5] (density of the 1e4 / 1e5-box layout, sides
+    U[10,500]), master 0 each, walks per layout U[1e5, 1e6].  c6: the
+    1e6-rect layout with 16 masters, walks per master U[5e5, 2e6].  Learned
+    mode: one network trained on pairs harvested from every layout (every
+    master), each job on its own layout's pyramid."""
     rng = np.random.default_rng(1)
-    n_lay = 64
-    sizes = np.exp(rng.uniform(np.log(1e2), np.log(1e5), n_lay)).astype(int)
-    wl = rng.integers(100_000, 1_000_001, n_lay)
     trees, jobs = [], []
     t0 = time.perf_counter()
-    for i, (n, w) in enumerate(zip(sizes, wl)):
-        box = int(1000 * math.sqrt(n))
-        lay = frw.Layout.generate(int(n), box, 10, 500, seed=100 + i)
+    if args.workload == "c4":
+        n_lay = 64
+        sizes = np.exp(rng.uniform(np.log(1e2), np.log(1e5), n_lay)).astype(int)
+        wl = rng.integers(100_000, 1_000_001, n_lay)
+        for i, (n, w) in enumerate(zip(sizes, wl)):
+            box = int(1000 * math.sqrt(n))
+            lay = frw.Layout.generate(int(n), box, 10, 500, seed=100 + i)
+            trees.append(frw.Tree(lay, leaf_cap=args.leaf_cap))
+            jobs.append({"tree": i, "master": 0, "n_walks": int(w), "job": i})
+        harvest = [(i, 0) for i in range(n_lay)]
+    else:
+        lay = make_layout(frw, "c3")
         trees.append(frw.Tree(lay, leaf_cap=args.leaf_cap))
-        jobs.append({"tree": i, "master": 0, "n_walks": int(w), "job": i})
+        sizes = np.array([WORKLOADS["c3"][0]])
+        masters = [0] + [int(m) for m in rng.choice(np.arange(1, WORKLOADS["c3"][0]), 15, replace=False)]
+        wl = rng.integers(500_000, 2_000_001, len(masters))
+        for i, (m, w) in enumerate(zip(masters, wl)):
+            jobs.append({"tree": 0, "master": m, "n_walks": int(w), "job": i})
+        harvest = [(0, m) for m in masters]
+    n_jobs = len(jobs)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
-    log(f"{n_lay} layouts built in {build_s:.2f} s; {int(wl.sum())} walks per step")
+    log(f"{len(trees)} layout(s) built in {build_s:.2f} s; {n_jobs} jobs, {int(wl.sum())} walks per step")
     batch = frw.Batch(trees, jobs)
     mode = args.mode or "exact"
     modes = ["bisect", "exact", "learned"] if args.compare else [mode]
@@ -539,14 +556,14 @@ def bench_batched(args, frw, torch, np):
     if "learned" in modes:
         t0 = time.perf_counter()
         pyrs = [frw.Mlp(t, seed=1, max_level=13) for t in trees]
-        per = max(1024, args.pairs // n_lay)
+        per = max(1024, args.pairs // len(harvest))
         net = frw.Mlp(trees[int(np.argmax(sizes))], seed=1, max_level=13)
         got_total = 0
-        for i, t in enumerate(trees):
+        for i, m in harvest:
             pts = torch.empty((per, 2), dtype=torch.float64, device="cuda")
             hh = torch.empty(per, dtype=torch.float64, device="cuda")
             ds = torch.empty(per, dtype=torch.float64, device="cuda")
-            got = min(per, frw.walk_harvest(t, 0, max(1000, per // 6), pts, hh, ds, seed=1,
+            got = min(per, frw.walk_harvest(trees[i], m, max(1000, per // 6), pts, hh, ds, seed=1,
                                             first_walk=3 << 30))
             net.add_samples(pts[:got].contiguous(), hh[:got].contiguous(), ds[:got].contiguous(),
                             features_of=pyrs[i])
@@ -557,10 +574,10 @@ def bench_batched(args, frw, torch, np):
         batch.set_proposal(pyrs)
         train_info = {"pairs": got_total, "setup_s": t1 - t0, "train_s": t2 - t1, "epochs": args.epochs,
                       "final_loss": loss, "alpha": args.alpha}
-        log(f"proposal: {got_total} pairs from {n_lay} layouts ({t1 - t0:.2f} s incl. pyramids), "
+        log(f"proposal: {got_total} pairs from {len(harvest)} layouts/masters ({t1 - t0:.2f} s incl. pyramids), "
             f"trained in {t2 - t1:.2f} s, loss {loss:.4f}")
     counts = torch.zeros(batch.counts_size(), dtype=torch.int64, device="cuda")
-    hops = torch.zeros(n_lay, dtype=torch.int64, device="cuda")
+    hops = torch.zeros(n_jobs, dtype=torch.int64, device="cuda")
     stats = torch.zeros(2, dtype=torch.int64, device="cuda")
     stream = torch.cuda.Stream()
     sh = stream.cuda_stream
@@ -593,8 +610,8 @@ def bench_batched(args, frw, torch, np):
         n_walks = int(wl.sum()) * args.steps
         c = counts.cpu().numpy()
         worst = 0.0  # steps until every job's fraction is known to 1% (3 sigma)
-        for j in range(n_lay):
-            off = batch.counts_offset(j)
+        for j in range(n_jobs):
+            off = batch.counts_offset(j) + jobs[j]["master"]
             fj = 1.0 - c[off] / (wl[j] * args.steps)
             worst = max(worst, walks_for_1pct(fj) / wl[j])
         st = stats.cpu().numpy()
@@ -620,7 +637,8 @@ def bench_batched(args, frw, torch, np):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": DESCR["c4"], "layouts": n_lay, "walks_per_step": int(wl.sum()),
+        "config": {"workload": DESCR[args.workload], "layouts": len(trees), "jobs": n_jobs,
+                   "walks_per_step": int(wl.sum()),
                    "mode": main_mode, "autotuned": r["autotuned"], "graph": True},
         "walks_per_s": r["walks_per_s"],
         "hops_per_walk": r["hops_per_walk"],

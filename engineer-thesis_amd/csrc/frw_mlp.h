@@ -174,12 +174,23 @@ __device__ inline uint64_t mlp_window(const MlpPyramid& m, int l, double cell, d
 // Writes one feature row (72 bf16) of this lane's point into LDS.
 __device__ inline void mlp_feature_row(uint16_t* row, uint64_t bits, int level, float frac_x, float frac_y)
 {
-    const uint16_t one = 0x3F80u;  // bf16(1.0)
-    for (int k = 0; k < 64; ++k) row[k] = ((bits >> k) & 1u) ? one : (uint16_t)0;
-    row[64] = f2bf((float)level / 16.0f);
-    row[65] = f2bf(frac_x);
-    row[66] = f2bf(frac_y);
-    for (int k = 67; k < kMlpInPad; ++k) row[k] = 0;
+    const short one = 0x3F80;  // bf16(1.0)
+    // four elements per 8-byte store (rows are 8-byte aligned)
+#pragma unroll
+    for (int k = 0; k < 64; k += 4) {
+        bf16x4 v;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = ((bits >> (k + u)) & 1u) ? one : (short)0;
+        *(bf16x4*)(row + k) = v;
+    }
+    bf16x4 a;
+    a[0] = (short)f2bf((float)level / 16.0f);
+    a[1] = (short)f2bf(frac_x);
+    a[2] = (short)f2bf(frac_y);
+    a[3] = 0;
+    *(bf16x4*)(row + 64) = a;
+    static_assert(kMlpInPad == 72, "feature row layout");
+    *(bf16x4*)(row + 68) = bf16x4{0, 0, 0, 0};
 }
 
 // Verification probe: true when the closed square (x0, y0, x1, y1) of
@@ -253,6 +264,46 @@ __device__ inline void mlp_layer(uint16_t* buf, const uint16_t* W, const float* 
     }
 }
 
+// Third hidden layer and output neuron for the wave's 64 rows: y = w4 .
+// bf16(gelu(buf * W3^T + b3)) + b4, returned in lane `row`.  Each lane sums
+// the 32 neurons it holds of rows r and 32 + r (lane = r + 32 q); the other
+// 32 of each row are in lane r + 32 (1 - q), which sends its sum for the
+// row this lane returns.  The activations are not stored.
+__device__ inline float mlp_last_layer(const uint16_t* buf, const MlpNet& m, int lane)
+{
+    const int r32 = lane & 31, q = lane >> 5;
+    float p0 = 0.0f, p1 = 0.0f;
+#pragma unroll 1
+    for (int mi = 0; mi < 2; ++mi) {
+        const uint16_t* row = buf + (mi * 32 + r32) * kMlpStride;
+        float part = 0.0f;
+#pragma unroll 1
+        for (int nj = 0; nj < 2; ++nj) {
+            const uint16_t* wrow = m.W3 + (nj * 32 + r32) * kMlpHid + 4 * q;
+            f32x16 acc = {0};
+#pragma unroll 4
+            for (int kb = 0; kb < kMlpHid / 8; ++kb) {
+                const bf16x4 x = *(const bf16x4*)(row + 4 * q + kb * 8);
+                const bf16x4 w = *(const bf16x4*)(wrow + kb * 8);
+                acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w, x, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int n0 = nj * 32 + 8 * g + 4 * q;
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    part += bf2f(m.w4[n0 + u]) * bf2f(f2bf(gelu_f(acc[4 * g + u] + m.b3[n0 + u])));
+            }
+        }
+        if (mi == 0)
+            p0 = part;
+        else
+            p1 = part;
+    }
+    const float other = __shfl_xor(q ? p0 : p1, 32);
+    return (q ? p1 : p0) + other + *m.b4;
+}
+
 // Forward pass of the wave's 64 feature rows in xs (LDS, kMlpTileElems:
 // 64 x kMlpStride, overwritten by the hidden activations, and the staging
 // slots).  Returns row `lane`'s output.
@@ -264,12 +315,9 @@ __device__ inline float mlp_forward_wave(const MlpNet& m, uint16_t* xs, int lane
     wave_sync();
     mlp_layer<kMlpHid>(xs, m.W2, m.b2, lane);
     wave_sync();
-    mlp_layer<kMlpHid>(xs, m.W3, m.b3, lane);
+    const float y = mlp_last_layer(xs, m, lane);
     wave_sync();
-    float y = 0.0f;
-    for (int j = 0; j < kMlpHid; ++j) y += bf2f(m.w4[j]) * bf2f(xs[lane * kMlpStride + j]);
-    wave_sync();
-    return y + *m.b4;
+    return y;
 }
 
 }  // namespace frw
