@@ -164,20 +164,60 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
     Counters* cp = kStats ? &cnt : nullptr;
     unsigned long long my_nodes = 0, my_rects = 0, my_tabs = 0, my_acc = 0, my_fb = 0;
 
-    auto flush = [&]() {
-        if (my_master) atomicAdd(&jobs[j].counts[mid], (unsigned long long)my_master);
-        if (my_escaped) atomicAdd(&jobs[j].counts[t.n_rects], (unsigned long long)my_escaped);
-        my_master = my_escaped = 0;
-        if (kMode == kModeWeighted) {
-            if (wm_s != 0.0 || wm_q != 0.0) {
-                atomicAdd(&jobs[j].wsum[mid], wm_s - wm_c);
-                atomicAdd(&jobs[j].wsq[mid], wm_q);
+    // Adds the per-lane tallies (master, escape, hops; weighted sums) of the
+    // lanes with `part` set to their jobs and clears them: a wave-wide sum and
+    // one atomic per bin for each distinct job among those lanes.  Per-lane
+    // atomics serialised here: every lane of the grid leaves a job at about
+    // the same time, and all of them add to the same two bins.  Called by
+    // all 64 lanes together.
+    auto wave_flush = [&](bool part) {
+        unsigned long long pend = __ballot(part);
+        while (pend) {
+            const int leader = __ffsll((long long)pend) - 1;
+            const int lj = __shfl(j, leader);
+            const bool mine = part && j == lj;
+            unsigned long long m = mine ? my_master : 0u, e = mine ? my_escaped : 0u, h = mine ? my_hops : 0ull;
+            double s0 = 0.0, q0 = 0.0, s1 = 0.0, q1 = 0.0;
+            if (kMode == kModeWeighted && mine) {
+                s0 = wm_s - wm_c;
+                q0 = wm_q;
+                s1 = we_s - we_c;
+                q1 = we_q;
             }
-            if (we_s != 0.0 || we_q != 0.0) {
-                atomicAdd(&jobs[j].wsum[t.n_rects], we_s - we_c);
-                atomicAdd(&jobs[j].wsq[t.n_rects], we_q);
+            for (int off = 32; off >= 1; off >>= 1) {
+                m += __shfl_xor(m, off);
+                e += __shfl_xor(e, off);
+                h += __shfl_xor(h, off);
+                if (kMode == kModeWeighted) {
+                    s0 += __shfl_xor(s0, off);
+                    q0 += __shfl_xor(q0, off);
+                    s1 += __shfl_xor(s1, off);
+                    q1 += __shfl_xor(q1, off);
+                }
             }
-            wm_s = wm_c = wm_q = we_s = we_c = we_q = 0.0;
+            if (lane == leader) {
+                const int lm = jobs[lj].master_id, le = jobs[lj].tree.n_rects;
+                if (m) atomicAdd(&jobs[lj].counts[lm], m);
+                if (e) atomicAdd(&jobs[lj].counts[le], e);
+                if (h) atomicAdd(jobs[lj].hops, h);
+                if (kMode == kModeWeighted) {
+                    if (s0 != 0.0 || q0 != 0.0) {
+                        atomicAdd(&jobs[lj].wsum[lm], s0);
+                        atomicAdd(&jobs[lj].wsq[lm], q0);
+                    }
+                    if (s1 != 0.0 || q1 != 0.0) {
+                        atomicAdd(&jobs[lj].wsum[le], s1);
+                        atomicAdd(&jobs[lj].wsq[le], q1);
+                    }
+                }
+            }
+            if (mine) {
+                my_master = my_escaped = 0;
+                my_hops = 0;
+                if (kMode == kModeWeighted) wm_s = wm_c = wm_q = we_s = we_c = we_q = 0.0;
+                part = false;
+            }
+            pend = __ballot(part);
         }
     };
 
@@ -218,14 +258,15 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
                 wnext += n_need;
             }
         }
+        int nj = j;
+        if (got && a.n_jobs > 1) nj = find_job(jobs, s_begin, a.n_jobs, g);
+        // lanes moving to another job hand in the old job's tallies first
+        const bool change = got && nj != j;
+        if (__ballot(change)) wave_flush(change);
         if (got) {
             active = true;
             hop = 0;
-            const int nj = a.n_jobs == 1 ? 0 : find_job(jobs, s_begin, a.n_jobs, g);
-            if (nj != j) {
-                if (my_hops) atomicAdd(jobs[j].hops, my_hops);
-                my_hops = 0;
-                flush();
+            if (change) {
                 j = nj;
                 t = jobs[j].tree;
                 mid = jobs[j].master_id;
@@ -424,26 +465,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
             active = false;
         }
     }
-    if (a.n_jobs == 1) {
-        unsigned long long m = my_master, e = my_escaped;
-        for (int off = 32; off >= 1; off >>= 1) {
-            my_hops += __shfl_xor(my_hops, off);
-            m += __shfl_xor(m, off);
-            e += __shfl_xor(e, off);
-        }
-        if (lane == 0) {
-            if (my_hops) atomicAdd(jobs[0].hops, my_hops);
-            if (m) atomicAdd(&jobs[0].counts[mid], m);
-            if (e) atomicAdd(&jobs[0].counts[t.n_rects], e);
-        }
-        if (kMode == kModeWeighted) {
-            my_master = my_escaped = 0;  // counted above
-            flush();
-        }
-    } else {
-        if (my_hops) atomicAdd(jobs[j].hops, my_hops);
-        flush();
-    }
+    wave_flush(true);
     if (kStats) {
         for (int off = 32; off >= 1; off >>= 1) {
             my_nodes += __shfl_xor(my_nodes, off);
