@@ -59,6 +59,7 @@ struct WalkArgs {
 
 constexpr int kLdsJobs = 128;                                   // job walk offsets cached in LDS
 constexpr int kWaveChunk = 256;                                 // walk indices a wave claims per queue visit
+constexpr int kTailChunk = 64;                                  // ... once the queue nears its end
 constexpr uint32_t kGausDomain = 0x47415553u;                   // 'GAUS': Gaussian-surface start points
 
 // Compensated (Kahan) accumulation; -ffp-contract=off keeps it exact.
@@ -154,6 +155,8 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
     // bins take most walks; one atomic per walk on them would serialise)
     uint32_t my_master = 0, my_escaped = 0;
     uint64_t wnext = 0, wend = 0;  // the wave's unclaimed walk indices [wnext, wend), equal in every lane
+    uint64_t chunk = kWaveChunk;
+    const uint64_t tail_walks = (uint64_t)gridDim.x * (blockDim.x / kWaveSize) * 4 * kWaveChunk;
     // weighted estimator: the walk's weight, its preset first point, and
     // per-lane compensated sums (and sums of squares) for the master and
     // escape bins
@@ -228,7 +231,10 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
         // lanes without a walk take the next indices of the wave's range in
         // lane order; when the range runs short one lane claims kWaveChunk
         // more from the queue (one atomic on the shared counter per 256
-        // walks: a claim per few walks serialised the kernel on that address)
+        // walks: a claim per few walks serialised the kernel on that
+        // address), kTailChunk once fewer than 4 chunks per wave of the grid
+        // are left (a late large claim keeps its wave walking after the
+        // others have finished)
         const unsigned long long mask = __ballot(need);
         if (mask) {
             const uint64_t n_need = (uint64_t)__popcll(mask);
@@ -237,7 +243,9 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
             uint64_t base = 0;
             if (refill) {
                 const int leader = __ffsll((long long)mask) - 1;
-                if (lane == leader) base = atomicAdd(a.work, (unsigned long long)kWaveChunk);
+                const uint64_t left = a.total_walks > wend ? a.total_walks - wend : 0;
+                chunk = left > tail_walks ? kWaveChunk : kTailChunk;
+                if (lane == leader) base = atomicAdd(a.work, (unsigned long long)chunk);
                 base = __shfl(base, leader);
             }
             if (need) {
@@ -253,7 +261,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
             }
             if (refill) {
                 wnext = base + (n_need - avail);
-                wend = base + kWaveChunk;
+                wend = base + chunk;
             } else {
                 wnext += n_need;
             }
