@@ -427,7 +427,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
                                 a.h_dstar[slot] = ds;
                             }
                         }
-                        sq = biggest_square<false>(t, st, px, py, ds, nullptr, cp, a.err);
+                        sq = biggest_square<false>(t, st, px, py, ds, nullptr, cp, a.err, &hint_r);
                     }
                 } else {
                     sq = biggest_square<true>(t, st, px, py, 0.0, nullptr, cp, a.err);

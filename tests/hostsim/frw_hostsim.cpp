@@ -362,7 +362,8 @@ static int64_t hs_walk_hinted_impl(void* h, int master, int mode, uint64_t seed,
                 if (ds <= 0.0)
                     result = v.ids[arg];
                 else
-                    sq = biggest_square<false>(v, st, px, py, ds, nullptr, traffic ? &cnt : nullptr, &err);
+                    sq = biggest_square<false>(v, st, px, py, ds, nullptr, traffic ? &cnt : nullptr, &err,
+                                              use_hint ? &hint_r : nullptr);
             } else {
                 sq = biggest_square<true>(v, st, px, py, 0.0, nullptr, traffic ? &cnt : nullptr, &err);
                 const int hh = q2_point(v, st, px, py, traffic ? &cnt : nullptr);
