@@ -79,13 +79,16 @@ class HostTree:
         self.L.hs_queries(self.h, pts.ctypes.data, n, *[a.ctypes.data for a in out])
         return out
 
-    def walk(self, master, n, mode=0, seed=1, job=0, first=0, max_hops=10 ** 7):
+    def walk(self, master, n, mode=0, seed=1, job=0, first=0, max_hops=10 ** 7, use_hint=False):
+        """Per-walk hits and hops; use_hint: the exact mode seeds each search
+        with the previous hop's nearest rect and hands that rect to the
+        bisection replay, as the walk kernel does."""
         T = frw.green_thresholds()
         hit = np.zeros(n, np.int32)
         hp = np.zeros(n, np.int32)
         self.L.hs_set_stack(self.runs)
-        tot = self.L.hs_walk(self.h, master, mode, seed, job, first, n, max_hops, T.ctypes.data,
-                             hit.ctypes.data, hp.ctypes.data, None)
+        tot = self.L.hs_walk_hinted(self.h, master, mode, seed, job, first, n, max_hops, T.ctypes.data,
+                                    hit.ctypes.data, hp.ctypes.data, None, int(use_hint))
         assert tot >= 0, "bisection error flag"
         return hit, hp, tot
 
@@ -113,8 +116,8 @@ def test_hostsim_walks_replay_oracle(oracle_lib, name, master, cap, runs):
     ix = O.Index(lay.rects().astype(np.float64), tuple(float(v) for v in lay.bounds()),
                  O.REFTREE_DFS)
     _, tot_o, ohit, ohp = ix.walk(master, 1500, per_walk=True)
-    for mode in (frw.MODE_EXACT, frw.MODE_BISECT):
-        hit, hp, tot = ht.walk(master, 1500, mode=mode)
+    for mode, hint in ((frw.MODE_EXACT, False), (frw.MODE_EXACT, True), (frw.MODE_BISECT, False)):
+        hit, hp, tot = ht.walk(master, 1500, mode=mode, use_hint=hint)
         assert np.array_equal(hit, ohit) and np.array_equal(hp, ohp) and tot == tot_o
 
 
