@@ -14,7 +14,10 @@
 //   p outside the strict bounds            -> walk escaped (hop not counted)
 //   exact mode:   d* = nearest-rect distance (branch and bound seeded with
 //                 the rect nearest to the previous point); d* <= 0 -> hit
-//                 (Q2 for the id), else square = biggest square replayed from d*
+//                 (the search's nearest rect is the point query's answer),
+//                 else square = biggest square replayed from d*
+//   weighted:     the exact step, from a Gaussian-surface start with a
+//                 Green-gradient weight
 //   bisect mode:  square = literal bisection (one Q1 per step), hit = Q2(p)
 //   learned mode: the wave's 64 points go through the proposal network
 //                 (bf16 MFMA); an occupancy-pyramid probe that certifies the
