@@ -118,14 +118,22 @@ template <int kMode, bool kStats, bool kHarvest, bool kRuns>
 // At least three waves per SIMD: the learned instantiation then fits 168
 // VGPRs without spills (the others need fewer).  kRuns: the traversal stack
 // holds runs of siblings (RunStack) instead of single nodes (PlainStack).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMode == kModeLearned || kMode == kModeWeighted ? 3 : 4)))
 walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
 {
     extern __shared__ int32_t stack_mem[];
     __shared__ uint32_t sT[kNSample];
     __shared__ uint64_t s_begin[kLdsJobs];
+    // the block's tallies of the first kLdsJobs jobs (master, escaped, hops;
+    // weighted: master and escape sums and sums of squares), added to the
+    // jobs' bins once when the block ends
+    __shared__ unsigned long long s_tally[3 * kLdsJobs];
+    __shared__ double s_wtally[kMode == kModeWeighted ? 4 * kLdsJobs : 1];
     for (int i = threadIdx.x; i < kNSample; i += blockDim.x) sT[i] = a.thresholds[i];
     for (int i = threadIdx.x; i < min(a.n_jobs, kLdsJobs); i += blockDim.x) s_begin[i] = jobs[i].walk_begin;
+    for (int i = threadIdx.x; i < 3 * kLdsJobs; i += blockDim.x) s_tally[i] = 0ull;
+    if (kMode == kModeWeighted)
+        for (int i = threadIdx.x; i < 4 * kLdsJobs; i += blockDim.x) s_wtally[i] = 0.0;
     __syncthreads();
     const int lane = threadIdx.x & (kWaveSize - 1);
     using Stack = typename std::conditional<kRuns, RunStack, PlainStack>::type;
@@ -170,61 +178,46 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
     Counters* cp = kStats ? &cnt : nullptr;
     unsigned long long my_nodes = 0, my_rects = 0, my_tabs = 0, my_acc = 0, my_fb = 0;
 
-    // Adds the per-lane tallies (master, escape, hops; weighted sums) of the
-    // lanes with `part` set to their jobs and clears them: a wave-wide sum and
-    // one atomic per bin for each distinct job among those lanes.  Per-lane
-    // atomics serialised here: every lane of the grid leaves a job at about
-    // the same time, and all of them add to the same two bins.  Called by
-    // all 64 lanes together.
-    auto wave_flush = [&](bool part) {
-        unsigned long long pend = __ballot(part);
-        while (pend) {
-            const int leader = __ffsll((long long)pend) - 1;
-            const int lj = __shfl(j, leader);
-            const bool mine = part && j == lj;
-            unsigned long long m = mine ? my_master : 0u, e = mine ? my_escaped : 0u, h = mine ? my_hops : 0ull;
-            double s0 = 0.0, q0 = 0.0, s1 = 0.0, q1 = 0.0;
-            if (kMode == kModeWeighted && mine) {
-                s0 = wm_s - wm_c;
-                q0 = wm_q;
-                s1 = we_s - we_c;
-                q1 = we_q;
-            }
-            for (int off = 32; off >= 1; off >>= 1) {
-                m += __shfl_xor(m, off);
-                e += __shfl_xor(e, off);
-                h += __shfl_xor(h, off);
-                if (kMode == kModeWeighted) {
-                    s0 += __shfl_xor(s0, off);
-                    q0 += __shfl_xor(q0, off);
-                    s1 += __shfl_xor(s1, off);
-                    q1 += __shfl_xor(q1, off);
+    // A lane hands in its tallies (master, escape, hops; weighted sums) for
+    // its current job and clears them when it moves to another job and when
+    // the launch ends: into the block's LDS table for the first kLdsJobs
+    // jobs, added to the jobs' bins once per block at the end (per-lane
+    // atomics on a job's bins serialised: every lane of the grid leaves a
+    // job at about the same time), straight to the bins beyond.
+    auto hand_in = [&]() {
+        const double ws0 = wm_s - wm_c, ws1 = we_s - we_c;
+        if (j < kLdsJobs) {
+            if (my_master) atomicAdd(&s_tally[3 * j], (unsigned long long)my_master);
+            if (my_escaped) atomicAdd(&s_tally[3 * j + 1], (unsigned long long)my_escaped);
+            if (my_hops) atomicAdd(&s_tally[3 * j + 2], my_hops);
+            if (kMode == kModeWeighted) {
+                if (ws0 != 0.0 || wm_q != 0.0) {
+                    atomicAdd(&s_wtally[4 * j], ws0);
+                    atomicAdd(&s_wtally[4 * j + 1], wm_q);
+                }
+                if (ws1 != 0.0 || we_q != 0.0) {
+                    atomicAdd(&s_wtally[4 * j + 2], ws1);
+                    atomicAdd(&s_wtally[4 * j + 3], we_q);
                 }
             }
-            if (lane == leader) {
-                const int lm = jobs[lj].master_id, le = jobs[lj].tree.n_rects;
-                if (m) atomicAdd(&jobs[lj].counts[lm], m);
-                if (e) atomicAdd(&jobs[lj].counts[le], e);
-                if (h) atomicAdd(jobs[lj].hops, h);
-                if (kMode == kModeWeighted) {
-                    if (s0 != 0.0 || q0 != 0.0) {
-                        atomicAdd(&jobs[lj].wsum[lm], s0);
-                        atomicAdd(&jobs[lj].wsq[lm], q0);
-                    }
-                    if (s1 != 0.0 || q1 != 0.0) {
-                        atomicAdd(&jobs[lj].wsum[le], s1);
-                        atomicAdd(&jobs[lj].wsq[le], q1);
-                    }
+        } else {
+            if (my_master) atomicAdd(&jobs[j].counts[mid], (unsigned long long)my_master);
+            if (my_escaped) atomicAdd(&jobs[j].counts[t.n_rects], (unsigned long long)my_escaped);
+            if (my_hops) atomicAdd(jobs[j].hops, my_hops);
+            if (kMode == kModeWeighted) {
+                if (ws0 != 0.0 || wm_q != 0.0) {
+                    atomicAdd(&jobs[j].wsum[mid], ws0);
+                    atomicAdd(&jobs[j].wsq[mid], wm_q);
+                }
+                if (ws1 != 0.0 || we_q != 0.0) {
+                    atomicAdd(&jobs[j].wsum[t.n_rects], ws1);
+                    atomicAdd(&jobs[j].wsq[t.n_rects], we_q);
                 }
             }
-            if (mine) {
-                my_master = my_escaped = 0;
-                my_hops = 0;
-                if (kMode == kModeWeighted) wm_s = wm_c = wm_q = we_s = we_c = we_q = 0.0;
-                part = false;
-            }
-            pend = __ballot(part);
         }
+        my_master = my_escaped = 0;
+        my_hops = 0;
+        if (kMode == kModeWeighted) wm_s = wm_c = wm_q = we_s = we_c = we_q = 0.0;
     };
 
     for (;;) {
@@ -273,7 +266,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
         if (got && a.n_jobs > 1) nj = find_job(jobs, s_begin, a.n_jobs, g);
         // lanes moving to another job hand in the old job's tallies first
         const bool change = got && nj != j;
-        if (__ballot(change)) wave_flush(change);
+        if (change) hand_in();
         if (got) {
             active = true;
             hop = 0;
@@ -476,7 +469,26 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
             active = false;
         }
     }
-    wave_flush(true);
+    hand_in();
+    // the block's table to the jobs' bins, once all its waves are done
+    __syncthreads();
+    for (int i = threadIdx.x; i < min(a.n_jobs, kLdsJobs); i += blockDim.x) {
+        const unsigned long long m = s_tally[3 * i], e = s_tally[3 * i + 1], h = s_tally[3 * i + 2];
+        const int lm = jobs[i].master_id, le = jobs[i].tree.n_rects;
+        if (m) atomicAdd(&jobs[i].counts[lm], m);
+        if (e) atomicAdd(&jobs[i].counts[le], e);
+        if (h) atomicAdd(jobs[i].hops, h);
+        if (kMode == kModeWeighted) {
+            if (s_wtally[4 * i] != 0.0 || s_wtally[4 * i + 1] != 0.0) {
+                atomicAdd(&jobs[i].wsum[lm], s_wtally[4 * i]);
+                atomicAdd(&jobs[i].wsq[lm], s_wtally[4 * i + 1]);
+            }
+            if (s_wtally[4 * i + 2] != 0.0 || s_wtally[4 * i + 3] != 0.0) {
+                atomicAdd(&jobs[i].wsum[le], s_wtally[4 * i + 2]);
+                atomicAdd(&jobs[i].wsq[le], s_wtally[4 * i + 3]);
+            }
+        }
+    }
     if (kStats) {
         for (int off = 32; off >= 1; off >>= 1) {
             my_nodes += __shfl_xor(my_nodes, off);
@@ -549,6 +561,12 @@ void launch_kernel(bool runs, int grid, int block, size_t lds, const AutoShape& 
 
 }  // namespace
 
+// Static LDS of a walk block: Green thresholds, job offsets, job tallies.
+size_t walk_static_lds(int mode)
+{
+    return kNSample * 4 + kLdsJobs * 8 + kLdsJobs * 24 + (mode == kModeWeighted ? kLdsJobs * 32 : 8);
+}
+
 // LDS of one wave: its lanes' stacks, or the learned mode's activation
 // tiles when larger (the two alias).
 size_t walk_wave_bytes(const WalkLaunch& L, bool runs)
@@ -577,7 +595,7 @@ int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds, bool* run
     int b = L.block;
     if (b < 0 || b % kWaveSize != 0 || b > 256)
         return set_error(FRW_E_INVALID, "block must be a multiple of 64 and <= 256");
-    const size_t fixed = kNSample * 4 + kLdsJobs * 8;
+    const size_t fixed = walk_static_lds(L.mode);
     int lds_block = 0, lds_cu = 0, cus = 0;
     if (hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, L.device) != hipSuccess ||
         hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, L.device) != hipSuccess ||
@@ -626,7 +644,7 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
             return set_error(FRW_E_DEVICE, "device attributes unavailable");
         as.on = true;
         as.wave_bytes = wave_bytes;
-        as.fixed = kNSample * 4 + kLdsJobs * 8;
+        as.fixed = walk_static_lds(L.mode);
         as.lds_block = (size_t)lds_block;
         as.cus = std::max(cus, 1);
         as.total = L.total_walks;
