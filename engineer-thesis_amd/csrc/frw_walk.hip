@@ -114,11 +114,15 @@ __global__ void __launch_bounds__(64) walk_prologue(WalkJob* jobs, int n_jobs, u
     jb.start_rect = arg >= 0 ? jb.tree.rects[arg] : make_int4(0, 0, 0, 0);
 }
 
-template <int kMode, bool kStats, bool kHarvest, bool kRuns>
-// At least three waves per SIMD: the learned instantiation then fits 168
-// VGPRs without spills (the others need fewer).  kRuns: the traversal stack
-// holds runs of siblings (RunStack) instead of single nodes (PlainStack).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMode == kModeLearned || kMode == kModeWeighted ? 3 : 4)))
+template <int kMode, bool kStats, bool kHarvest, bool kRuns, bool kSingle>
+// Waves per SIMD the register budget allows: 3 for the learned mode (168
+// VGPRs without spills) and for the weighted one, 4 for the others; one more
+// for single-job launches (kSingle: the job's tree view, master and pyramid
+// are wave-uniform and live in scalar registers), measured faster at 10^6
+// rects despite a few spilled registers.  kRuns: the traversal stack holds
+// runs of siblings (RunStack) instead of single nodes (PlainStack).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    kMode == kModeLearned ? 3 : kMode == kModeWeighted ? (kSingle ? 4 : 3) : (kSingle ? 5 : 4))))
 walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
 {
     extern __shared__ int32_t stack_mem[];
@@ -263,9 +267,9 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
             }
         }
         int nj = j;
-        if (got && a.n_jobs > 1) nj = find_job(jobs, s_begin, a.n_jobs, g);
+        if (!kSingle && got) nj = find_job(jobs, s_begin, a.n_jobs, g);
         // lanes moving to another job hand in the old job's tallies first
-        const bool change = got && nj != j;
+        const bool change = !kSingle && got && nj != j;
         if (change) hand_in();
         if (got) {
             active = true;
@@ -526,11 +530,11 @@ struct AutoShape {
     uint64_t total = 0;
 };
 
-template <int kMode, bool kStats, bool kHarvest, bool kRuns>
+template <int kMode, bool kStats, bool kHarvest, bool kRuns, bool kSingle>
 void launch_kernel_as(int grid, int block, size_t lds, const AutoShape& as, hipStream_t s, const WalkJob* jobs,
                       const WalkArgs& a, const MlpNet& m)
 {
-    auto kern = walk_kernel<kMode, kStats, kHarvest, kRuns>;
+    auto kern = walk_kernel<kMode, kStats, kHarvest, kRuns, kSingle>;
     if (as.on) {
         int best = 0;
         for (int b : {256, 192, 128, 64}) {
@@ -555,8 +559,13 @@ template <int kMode, bool kStats, bool kHarvest>
 void launch_kernel(bool runs, int grid, int block, size_t lds, const AutoShape& as, hipStream_t s,
                    const WalkJob* jobs, const WalkArgs& a, const MlpNet& m)
 {
-    if (runs) launch_kernel_as<kMode, kStats, kHarvest, true>(grid, block, lds, as, s, jobs, a, m);
-    else launch_kernel_as<kMode, kStats, kHarvest, false>(grid, block, lds, as, s, jobs, a, m);
+    if (a.n_jobs == 1) {
+        if (runs) launch_kernel_as<kMode, kStats, kHarvest, true, true>(grid, block, lds, as, s, jobs, a, m);
+        else launch_kernel_as<kMode, kStats, kHarvest, false, true>(grid, block, lds, as, s, jobs, a, m);
+    } else {
+        if (runs) launch_kernel_as<kMode, kStats, kHarvest, true, false>(grid, block, lds, as, s, jobs, a, m);
+        else launch_kernel_as<kMode, kStats, kHarvest, false, false>(grid, block, lds, as, s, jobs, a, m);
+    }
 }
 
 }  // namespace
@@ -579,13 +588,13 @@ size_t walk_wave_bytes(const WalkLaunch& L, bool runs)
 
 // Stack layout of a walk launch: runs of siblings (RunStack) when the
 // single-node stack would hold fewer lanes per CU in LDS than the walk
-// kernels' registers allow (16 waves per CU, 12 for the learned and weighted
-// modes) and runs take less, single nodes (PlainStack, fewer instructions
-// per pop) otherwise.
+// kernels' registers allow (4 SIMDs x the kernel's waves per SIMD) and runs
+// take less, single nodes (PlainStack, fewer instructions per pop) otherwise.
 bool walk_uses_runs(const WalkLaunch& L, size_t lds_cu, size_t fixed)
 {
     if (L.max_nodes >= kMaxTreeNodes) return false;
-    const size_t waves = (L.mode == kModeLearned || L.mode == kModeWeighted) ? 12 : 16;
+    const bool single = L.n_jobs == 1;
+    const size_t waves = L.mode == kModeLearned ? 12 : L.mode == kModeWeighted ? (single ? 16 : 12) : (single ? 20 : 16);
     const size_t plain = walk_wave_bytes(L, false);
     return walk_wave_bytes(L, true) < plain && waves * plain + (waves / 4) * fixed > lds_cu;
 }
