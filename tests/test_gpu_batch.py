@@ -173,3 +173,31 @@ def test_batch_learned_needs_pyramids():
         batch.walk_learned(net, counts, hops)
     with pytest.raises(frw.FrwError):  # a network built for another tree
         batch.set_proposal([net] * len(trees))
+
+
+@pytest.mark.parametrize("mode", [frw.MODE_EXACT, frw.MODE_BISECT])
+def test_batch_beyond_lds_job_table(mode):
+    """200 jobs: the walk kernel keeps the walk offsets and the tallies of
+    the first 128 jobs in LDS and reads / adds those of the others in global
+    memory.  Every job, on both sides of that line, equals its single
+    launch."""
+    trees = []
+    for name, master in FIXTURES[:4]:
+        lay, rects, _ = load_fixture(name)
+        trees.append(frw.Tree(lay))
+    rng = np.random.default_rng(7)
+    jobs = []
+    for k in range(200):
+        ti = k % len(trees)
+        nr = trees[ti].info().n_rects
+        jobs.append({"tree": ti, "master": int(rng.integers(0, nr)), "n_walks": int(rng.integers(200, 1500)),
+                     "first_walk": int(rng.integers(0, 1 << 20)), "job": k})
+    batch = frw.Batch(trees, jobs)
+    counts, hops = _run(batch, mode)
+    for j, q in enumerate(jobs):
+        tree = trees[q["tree"]]
+        nr = tree.info().n_rects
+        off = batch.counts_offset(j)
+        c1, h1 = _single(tree, q["master"], q["n_walks"], q["job"], q["first_walk"], mode)
+        assert np.array_equal(counts[off:off + nr + 2], c1), j
+        assert hops[j] == h1, j
