@@ -36,6 +36,34 @@ int check_hip(hipError_t e, const char* what)
                      std::string(what) + ": " + hipGetErrorString(e));
 }
 
+int order_before(LaunchOrder& o, hipStream_t s, bool* capturing)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    int rc = check_hip(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
+    if (rc != FRW_OK) return rc;
+    *capturing = cs != hipStreamCaptureStatusNone;
+    if (*capturing) return FRW_OK;
+    if (!o.ev && (rc = check_hip(hipEventCreateWithFlags(&o.ev, hipEventDisableTiming), "hipEventCreate")) != FRW_OK)
+        return rc;
+    if (o.recorded) return check_hip(hipStreamWaitEvent(s, o.ev, 0), "hipStreamWaitEvent");
+    return FRW_OK;
+}
+
+int order_after(LaunchOrder& o, hipStream_t s, bool capturing)
+{
+    if (capturing || !o.ev) return FRW_OK;
+    const int rc = check_hip(hipEventRecord(o.ev, s), "hipEventRecord");
+    if (rc == FRW_OK) o.recorded = true;
+    return rc;
+}
+
+void order_free(LaunchOrder& o)
+{
+    if (o.ev) (void)hipEventDestroy(o.ev);
+    o.ev = nullptr;
+    o.recorded = false;
+}
+
 }  // namespace frw
 
 using namespace frw;
@@ -169,6 +197,7 @@ void frw_tree_free(frw_tree* t)
     if (t->d_err) (void)hipFree(t->d_err);
     if (t->d_surface) (void)hipFree(t->d_surface);
     if (t->d_job) (void)hipFree(t->d_job);
+    order_free(t->order);
     delete t;
 }
 
@@ -270,7 +299,6 @@ static int check_err_flag(const frw_tree* t, hipStream_t s)
     return FRW_OK;
 }
 
-static int check_walk_params(const frw_tree* t, const frw_walk_params* p);
 
 int frw_query_refcompat(const frw_tree* t, const double* pts, const double* boxes, int64_t n,
                         const int32_t* ignore, uint8_t* q1, int32_t* q2, double* q3, void* stream)
@@ -346,14 +374,39 @@ int frw_gauss_surface(const frw_tree* t, int32_t master, double out[4])
 {
     if (!t || !out) return set_error(FRW_E_INVALID, "null argument");
     if (master < 0 || master >= t->n_rects) return set_error(FRW_E_RANGE, "master out of range");
-    int rc = launch_gauss(t, t->host_rects[master], t->d_surface, nullptr);
+    bool capturing = false;
+    int rc = order_before(t->order, nullptr, &capturing);  // d_surface is the tree's
+    if (rc == FRW_OK) rc = launch_gauss(t, t->host_rects[master], t->d_surface, nullptr);
     if (rc == FRW_OK) rc = check_hip(hipMemcpy(out, t->d_surface, 4 * sizeof(double), hipMemcpyDeviceToHost), "copy surface");
-    return rc;
+    const int rc2 = order_after(t->order, nullptr, capturing);
+    return rc != FRW_OK ? rc : rc2;
 }
 
 // ---------------------------------------------------------------- walks
 
-static int check_walk_params(const frw_tree* t, const frw_walk_params* p)
+}  // extern "C"
+
+void frw::fill_summary(const frw_tree* t, const frw_walk_params* p, const int64_t* counts, int64_t hops, double ms,
+                       frw_walk_summary* summary)
+{
+    summary->n_walks = p->n_walks;
+    summary->hops = hops;
+    summary->n_master = counts[p->master];
+    summary->n_escaped = counts[t->n_rects];
+    summary->n_stuck = counts[t->n_rects + 1];
+    // getAvgPathLen (randomWalk/CPU/getAvgPathLen.cpp:96,113): hits not
+    // equal (by value) to the master, escapes and stuck walks included.
+    int64_t same = 0;
+    const int4 m = t->host_rects[p->master];
+    for (int64_t i = 0; i < t->n_rects; ++i) {
+        const int4 r = t->host_rects[i];
+        if (r.x == m.x && r.y == m.y && r.z == m.z && r.w == m.w) same += counts[i];
+    }
+    summary->fraction_other = (double)(p->n_walks - same) / (double)p->n_walks;
+    summary->walk_ms = ms;
+}
+
+int frw::check_walk_params(const frw_tree* t, const frw_walk_params* p)
 {
     if (!t || !p) return set_error(FRW_E_INVALID, "null argument");
     if (p->master < 0 || p->master >= t->n_rects) return set_error(FRW_E_RANGE, "master out of range");
@@ -367,6 +420,8 @@ static int check_walk_params(const frw_tree* t, const frw_walk_params* p)
     if (p->max_hops < 0 || p->max_hops > 0xFFFFFFFFll) return set_error(FRW_E_RANGE, "bad max_hops");
     return FRW_OK;
 }
+
+extern "C" {
 
 int frw_walk_async(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o, void* stream)
 {
@@ -579,23 +634,7 @@ int frw_walk(const frw_tree* t, const frw_walk_params* p, int64_t* counts_host, 
     if (d_hops) (void)hipFree(d_hops);
     if (rc != FRW_OK) return rc;
     if (counts_host) std::memcpy(counts_host, counts.data(), sizeof(int64_t) * nb);
-    if (summary) {
-        summary->n_walks = p->n_walks;
-        summary->hops = hops;
-        summary->n_master = counts[p->master];
-        summary->n_escaped = counts[t->n_rects];
-        summary->n_stuck = counts[t->n_rects + 1];
-        // getAvgPathLen (randomWalk/CPU/getAvgPathLen.cpp:96,113): hits not
-        // equal (by value) to the master, escapes and stuck walks included.
-        int64_t same = 0;
-        const int4 m = t->host_rects[p->master];
-        for (int64_t i = 0; i < t->n_rects; ++i) {
-            const int4 r = t->host_rects[i];
-            if (r.x == m.x && r.y == m.y && r.z == m.z && r.w == m.w) same += counts[i];
-        }
-        summary->fraction_other = (double)(p->n_walks - same) / (double)p->n_walks;
-        summary->walk_ms = ms;
-    }
+    if (summary) fill_summary(t, p, counts.data(), hops, ms, summary);
     return FRW_OK;
     FRW_GUARD_END
 }

@@ -34,6 +34,7 @@ struct frw_batch {
     uint64_t* graph_stats = nullptr;
     hipStream_t capture = nullptr;  // private stream the graph is captured on
     std::vector<MlpPyramid> pyrs;   // per tree; occ == nullptr until a proposal is set
+    LaunchOrder order;              // launches of this batch run one after another
 };
 
 namespace {
@@ -178,8 +179,8 @@ namespace {
 
 // One batched launch (or graph replay) in any mode; net/alpha/stats only for
 // the learned mode.
-int batch_walk(frw_batch* b, const frw_batch_params* p, const frw_mlp* net, float alpha, uint64_t* stats,
-               int64_t* counts, int64_t* hops, hipStream_t s)
+int batch_walk_on(frw_batch* b, const frw_batch_params* p, const frw_mlp* net, float alpha, uint64_t* stats,
+                  int64_t* counts, int64_t* hops, hipStream_t s)
 {
     int rc = upload_jobs(b, counts, hops, s);
     if (rc != FRW_OK) return rc;
@@ -225,6 +226,20 @@ int batch_walk(frw_batch* b, const frw_batch_params* p, const frw_mlp* net, floa
     return check_hip(hipGraphLaunch(b->exec, s), "graph launch");
 }
 
+// Launches of one batch share its job table, work counter and error flag:
+// each is ordered after the batch's previous launch, whatever its stream.
+int batch_walk(frw_batch* b, const frw_batch_params* p, const frw_mlp* net, float alpha, uint64_t* stats,
+               int64_t* counts, int64_t* hops, hipStream_t s)
+{
+    if (p->max_hops < 0 || p->max_hops > 0xFFFFFFFFll) return set_error(FRW_E_RANGE, "bad max_hops");
+    bool capturing = false;
+    int rc = order_before(b->order, s, &capturing);
+    if (rc != FRW_OK) return rc;
+    rc = batch_walk_on(b, p, net, alpha, stats, counts, hops, s);
+    const int rc2 = order_after(b->order, s, capturing);
+    return rc != FRW_OK ? rc : rc2;
+}
+
 int check_learned_batch(const frw_batch* b, const frw_mlp* net, float alpha)
 {
     if (!net) return set_error(FRW_E_INVALID, "learned mode needs a network");
@@ -246,6 +261,7 @@ int autotune(frw_batch* b, const frw_batch_params* base, frw_batch_params* best,
     if (rc == FRW_OK) rc = check_hip(hipEventCreate(&e0), "hipEventCreate");
     if (rc == FRW_OK) rc = check_hip(hipEventCreate(&e1), "hipEventCreate");
     double best_rate = -1.0;
+    if (base->max_hops < 0 || base->max_hops > 0xFFFFFFFFll) rc = set_error(FRW_E_RANGE, "bad max_hops");
     frw_batch_params cand = *base;
     cand.use_graph = 0;
     int cus = 0;
@@ -369,6 +385,7 @@ void frw_batch_free(frw_batch* b)
 {
     if (!b) return;
     drop_graph(b);
+    order_free(b->order);
     if (b->capture) (void)hipStreamDestroy(b->capture);
     if (b->d_jobs) (void)hipFree(b->d_jobs);
     if (b->d_work) (void)hipFree(b->d_work);

@@ -83,6 +83,19 @@ struct WalkExtras {
     uint32_t wflags = 0;  // FRW_WEIGHT_* options
 };
 
+// Orders the launches that share one handle's device scratch (job slot, work
+// counter, error flag, start square): each launch first waits for the
+// handle's previous launch, on whatever stream that ran, and then becomes the
+// one the next launch waits for.  While the launch stream is being captured
+// into a graph neither step is taken (the graph orders its own nodes).
+struct LaunchOrder {
+    hipEvent_t ev = nullptr;
+    bool recorded = false;
+};
+int order_before(LaunchOrder& o, hipStream_t s, bool* capturing);
+int order_after(LaunchOrder& o, hipStream_t s, bool capturing);
+void order_free(LaunchOrder& o);
+
 }  // namespace frw
 
 struct frw_layout {
@@ -121,6 +134,7 @@ struct frw_tree {
     int* d_err = nullptr;                  // kernel error bits
     double* d_surface = nullptr;           // Gaussian surface of the master
     frw::WalkJob* d_job = nullptr;         // job slot of a plain walk
+    mutable frw::LaunchOrder order;        // launches on this tree run one after another
 };
 
 namespace frw {
@@ -159,6 +173,21 @@ int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds, bool* run
 int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s);
 int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o,
                 hipStream_t s, unsigned long long* traffic = nullptr, const WalkExtras* extra = nullptr);
+// Device scratch of one single-layout launch.
+struct WalkScratch {
+    WalkJob* job;
+    unsigned long long* work;
+    int* err;
+};
+// launch_walk without the tree's launch ordering, on the given scratch
+// (the reference-compatible mode always uses the tree's own).
+int launch_walk_in(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o, hipStream_t s,
+                   const WalkScratch& ws, unsigned long long* traffic = nullptr, const WalkExtras* extra = nullptr);
+// Argument checks of the single-layout walk entry points.
+int check_walk_params(const frw_tree* t, const frw_walk_params* p);
+// The summary of frw_walk from host counts (n_rects + 2) and the hop total.
+void fill_summary(const frw_tree* t, const frw_walk_params* p, const int64_t* counts, int64_t hops, double ms,
+                  frw_walk_summary* summary);
 
 // reference-compatible index (frw_refcompat.hip)
 int build_refcompat(frw_tree* t);

@@ -859,6 +859,7 @@ int check_learned(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p,
         p->first_walk + p->n_walks > ((int64_t)1 << 32))
         return set_error(FRW_E_RANGE, "bad master or walk range");
     if (!(alpha > 0.0f)) return set_error(FRW_E_INVALID, "alpha must be positive");
+    if (p->max_hops < 0 || p->max_hops > 0xFFFFFFFFll) return set_error(FRW_E_RANGE, "bad max_hops");
     return FRW_OK;
 }
 

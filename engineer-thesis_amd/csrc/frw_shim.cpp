@@ -3,7 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <ctime>
@@ -155,23 +157,130 @@ floatingPoint getAvgPathLenCUDA(char* path, int ITER_NUM, int RECT_ID, bool meas
     return run_engine(path, ITER_NUM, RECT_ID, measure, layerID, FRW_MODE_EXACT, "[GPU]");
 }
 
+namespace {
+
+// Structural check of a built tree (the role of checkQuadTree,
+// randomWalk/Cuda/createquadtree.cu:373-425, on this library's flat layout,
+// where rects live only in leaves): the sorted rects are a permutation of the
+// layout's; an inner node owns no rects and its subtree's rects are its
+// children's, in quadrant order; the leaves' rect ranges partition [0, n) in
+// order, each within the leaf capacity unless the leaf is at the last level;
+// every node box is the tight bounding box of its subtree's rects, and a
+// subtree without rects has an inverted box.
+struct TreeCheck {
+    const std::vector<int32_t>& nodes;
+    const std::vector<int32_t>& rects;
+    int64_t n_nodes, next_leaf = 0;
+    int leaf_cap, depth;
+    const char* error = nullptr;
+    int64_t error_node = -1;
+
+    bool fail(const char* what, int64_t i)
+    {
+        error = what;
+        error_node = i;
+        return false;
+    }
+
+    // Checks the subtree of node i; *empty / box = its rects' bounding box.
+    bool subtree(int64_t i, int level, bool* empty, int64_t box[4])
+    {
+        if (i < 0 || i >= n_nodes) return fail("node index out of range", i);
+        if (level > depth) return fail("deeper than the tree's depth", i);
+        const int32_t* q = &nodes[(size_t)i * 8];
+        const int32_t fc = q[4], rb = q[5], re = q[6];
+        box[0] = box[1] = INT64_MAX;
+        box[2] = box[3] = INT64_MIN;
+        *empty = true;
+        if (fc < 0) {
+            if (rb < 0 || re < rb || (size_t)re * 4 > rects.size()) return fail("rect range out of bounds", i);
+            if (re - rb > leaf_cap && level + 1 < depth) return fail("leaf over capacity", i);
+            if (rb != re && rb != next_leaf) return fail("leaf ranges do not partition the rects in order", i);
+            if (rb != re) next_leaf = re;
+            for (int32_t r = rb; r < re; ++r) {
+                box[0] = std::min<int64_t>(box[0], rects[4 * (size_t)r]);
+                box[1] = std::min<int64_t>(box[1], rects[4 * (size_t)r + 1]);
+                box[2] = std::max<int64_t>(box[2], rects[4 * (size_t)r + 2]);
+                box[3] = std::max<int64_t>(box[3], rects[4 * (size_t)r + 3]);
+                *empty = false;
+            }
+        } else {
+            if (rb != re) return fail("inner node owns rects", i);
+            if (fc + 4 > n_nodes) return fail("child group out of range", i);
+            for (int c = 0; c < 4; ++c) {
+                bool ce = true;
+                int64_t cb[4];
+                if (!subtree(fc + c, level + 1, &ce, cb)) return false;
+                if (ce) continue;
+                *empty = false;
+                box[0] = std::min(box[0], cb[0]);
+                box[1] = std::min(box[1], cb[1]);
+                box[2] = std::max(box[2], cb[2]);
+                box[3] = std::max(box[3], cb[3]);
+            }
+        }
+        if (*empty) return (q[0] > q[2] || q[1] > q[3]) ? true : fail("empty subtree with a box", i);
+        if (q[0] != box[0] || q[1] != box[1] || q[2] != box[2] || q[3] != box[3])
+            return fail("node box is not the bounding box of its subtree", i);
+        return true;
+    }
+};
+
+bool check_tree(frw_tree* t, const std::vector<int32_t>& layout_rects)
+{
+    frw_tree_info info{};
+    if (frw_tree_info_get(t, &info) != FRW_OK) return false;
+    const int64_t n = info.n_rects;
+    std::vector<int32_t> nodes((size_t)info.n_nodes * 8), rects((size_t)n * 4), ids((size_t)n);
+    if (frw_tree_export(t, nodes.data(), rects.data(), ids.data()) != FRW_OK) return false;
+    std::vector<char> seen((size_t)n, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t id = ids[i];
+        bool ok = id >= 0 && id < n && !seen[id];
+        if (ok) seen[id] = 1;
+        for (int k = 0; ok && k < 4; ++k) ok = rects[4 * i + k] == layout_rects[4 * (size_t)id + k];
+        if (!ok) {
+            std::fprintf(stderr, "CreateTree: sorted rect %lld is not a distinct layout rect\n", (long long)i);
+            return false;
+        }
+    }
+    TreeCheck c{nodes, rects, info.n_nodes, 0, info.leaf_cap, info.depth};
+    bool empty = true;
+    int64_t box[4];
+    bool ok = c.subtree(0, 0, &empty, box);
+    if (ok && c.next_leaf != n) ok = c.fail("leaves do not cover every rect", -1);
+    if (!ok) std::fprintf(stderr, "CreateTree: error in node %lld: %s\n", (long long)c.error_node, c.error);
+    return ok;
+}
+
+}  // namespace
+
 QuadTreeManager* createQuadTree(const std::vector<RectHost>& layer, RectHost const& spaceSize,
                                 int RECT_ID, bool doCheck)
 {
-    (void)spaceSize;  // the bounds are recomputed from the rects (Parser::getLayerSize)
-    (void)doCheck;    // the build is checked by its own invariants in the test-suite
     std::vector<int32_t> r;
-    if (!to_ints(layer, &r)) {
+    int32_t b[4];
+    const std::vector<RectHost> space{spaceSize};
+    std::vector<int32_t> bi;
+    if (!to_ints(layer, &r) || !to_ints(space, &bi)) {
         std::fprintf(stderr, "createQuadTree: coordinates must be integers\n");
         return nullptr;
     }
+    for (int k = 0; k < 4; ++k) b[k] = bi[k];
     QuadTreeManager* q = new QuadTreeManager;
     q->master = RECT_ID;
+    // the root bounds are the caller's spaceSize (createquadtree.cu:32)
     if (frw_layout_from_rects(r.data(), (int64_t)layer.size(), &q->layout) != FRW_OK ||
+        frw_layout_set_bounds(q->layout, b) != FRW_OK ||
         frw_tree_build(q->layout, nullptr, nullptr, &q->tree) != FRW_OK) {
         std::fprintf(stderr, "createQuadTree: %s\n", frw_last_error_string());
         freeQuadTreeManager(q);
         return nullptr;
+    }
+    if (doCheck) {
+        // createquadtree.cu:61-74: the tree is checked and the outcome logged
+        std::fprintf(stderr, "Sprawdzanie drzewa.\n");
+        std::fprintf(stderr, check_tree(q->tree, r) ? "Stworzono pomyslnie\n" : "Blad tworzenia drzewa\n");
     }
     return q;
 }

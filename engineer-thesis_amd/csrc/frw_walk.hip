@@ -703,10 +703,11 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
     return check_hip(hipGetLastError(), "walk_kernel");
 }
 
-int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o,
-                hipStream_t s, unsigned long long* traffic, const WalkExtras* extra)
+// One single-layout launch with its job record, work counter and error flag
+// at the given device addresses (the tree's own, or a caller's scratch).
+int launch_walk_in(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o, hipStream_t s,
+                   const WalkScratch& ws, unsigned long long* traffic, const WalkExtras* extra)
 {
-    if (p->n_walks == 0) return FRW_OK;
     if (p->mode == kModeRefCompat) {
         if (traffic) return set_error(FRW_E_INVALID, "no traffic counters in the reference-compatible mode");
         return launch_ref_walk(t, p, o, s);
@@ -729,7 +730,7 @@ int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outp
         jb.w_out = extra->w_out;
     }
     if (extra && extra->pyr) jb.pyr = *extra->pyr;
-    set_job<<<1, 1, 0, s>>>(t->d_job, jb);
+    set_job<<<1, 1, 0, s>>>(ws.job, jb);
     int rc = check_hip(hipGetLastError(), "set_job");
     if (rc != FRW_OK) return rc;
     WalkLaunch L;
@@ -744,8 +745,8 @@ int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outp
     L.seed = p->seed;
     L.max_hops = (uint32_t)(p->max_hops > 0 ? p->max_hops : 10000000);
     L.thresholds = t->d_thresholds;
-    L.work = t->d_work;
-    L.err = t->d_err;
+    L.work = ws.work;
+    L.err = ws.err;
     L.traffic = traffic;
     if (extra) {
         L.h_pts = extra->h_pts;
@@ -759,7 +760,21 @@ int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outp
         L.wtab = extra->wsum ? t->d_wtab : nullptr;
         L.l_stats = extra->l_stats;
     }
-    return launch_walk_jobs(t->d_job, L, s);
+    return launch_walk_jobs(ws.job, L, s);
+}
+
+// Walks on one tree share its job slot, work counter and error flag, so each
+// launch is ordered after the tree's previous one (whatever its stream).
+int launch_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_outputs* o,
+                hipStream_t s, unsigned long long* traffic, const WalkExtras* extra)
+{
+    if (p->n_walks == 0) return FRW_OK;
+    bool capturing = false;
+    int rc = order_before(t->order, s, &capturing);
+    if (rc != FRW_OK) return rc;
+    rc = launch_walk_in(t, p, o, s, WalkScratch{t->d_job, t->d_work, t->d_err}, traffic, extra);
+    const int rc2 = order_after(t->order, s, capturing);
+    return rc != FRW_OK ? rc : rc2;
 }
 
 }  // namespace frw

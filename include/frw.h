@@ -17,7 +17,10 @@
  *     taking a `stream` (a hipStream_t, NULL = default stream) are
  *     asynchronous on it.  Host pointers are read or written before return.
  *   - Handles are owned by the caller and may be used from one host thread
- *     at a time.
+ *     at a time.  Launches on one tree (or one batch) share that handle's
+ *     device scratch, so a launch on any stream first waits for the handle's
+ *     previous launch (an event recorded after each one); launches under
+ *     graph capture are not ordered this way.
  */
 #ifndef FRW_H
 #define FRW_H
