@@ -96,7 +96,7 @@ EXPORTS = [
     "frw_mlp_levels", "frw_mlp_certify", "frw_mlp_add_samples", "frw_batch_set_proposal",
     "frw_batch_walk_learned", "frw_batch_autotune_learned", "frw_walk_weighted_async", "frw_walk_weighted", "frw_walk_adaptive", "frw_weight_tables", "frw_probe_read_bandwidth",
     "frw_query_refcompat", "frw_refcompat_gauss_surface", "frw_refcompat_info", "frw_refcompat_walk_from",
-    "frw_layout_set_bounds",
+    "frw_layout_set_bounds", "frw_walk_noalloc",
 ]
 
 _lib = None
@@ -149,6 +149,7 @@ def lib():
     L.frw_gauss_surface.argtypes = [vp, i32, vp]
     L.frw_walk_async.argtypes = [vp, C.POINTER(WalkParams), C.POINTER(WalkOutputs), vp]
     L.frw_walk.argtypes = [vp, C.POINTER(WalkParams), vp, C.POINTER(WalkSummary)]
+    L.frw_walk_noalloc.argtypes = [vp, C.POINTER(WalkParams), vp, vp, C.POINTER(WalkSummary)]
     L.frw_walk_traffic.argtypes = [vp, C.POINTER(WalkParams), vp]
     L.frw_walk_weighted_async.argtypes = [vp, C.POINTER(WalkParams), C.POINTER(WalkOutputs), C.c_double,
                                           C.c_uint32, vp, vp, vp, vp]
@@ -381,6 +382,18 @@ class Tree:
         counts = np.zeros(self.info().n_rects + 2, dtype=np.int64)
         s = WalkSummary()
         _check(lib().frw_walk(self.h, C.byref(p), counts.ctypes.data, C.byref(s)))
+        return counts, s
+
+    def walk_noalloc(self, master, n_walks, seed=1, mode=MODE_EXACT, job=0, first_walk=0, max_hops=0,
+                     block=0, grid=0, stream=None):
+        """frw_walk_noalloc: the walks of walk() with the library's fixed
+        scratch array instead of per-call device allocations; returns
+        (counts[n+2], summary)."""
+        p = WalkParams(master=master, mode=mode, n_walks=n_walks, first_walk=first_walk, seed=seed,
+                       job=job, block=block, grid=grid, reserved=0, max_hops=max_hops)
+        counts = np.zeros(self.info().n_rects + 2, dtype=np.int64)
+        s = WalkSummary()
+        _check(lib().frw_walk_noalloc(self.h, C.byref(p), stream, counts.ctypes.data, C.byref(s)))
         return counts, s
 
     def walk_traffic(self, master, n_walks, seed=1, mode=MODE_EXACT, job=0, first_walk=0,

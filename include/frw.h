@@ -203,6 +203,22 @@ typedef struct {
 int frw_walk(const frw_tree* t, const frw_walk_params* p, int64_t* counts_host,
              frw_walk_summary* summary);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the walks of frw_walk (FRW_MODE_EXACT or
+ * FRW_MODE_BISECT), synchronous on `stream`, with every scratch buffer (job
+ * record, work counter, error flag, hop counter and the n_rects + 2 count
+ * bins) in one fixed device array of the library: 8 MiB + 2 KiB per GPU,
+ * held until the process ends, bins for at most FRW_NOALLOC_MAX_RECTS rects
+ * (FRW_E_RANGE beyond).  No hipMalloc/hipFree and no allocating library call.
+ * counts_host (n_rects + 2, required when summary is given) and the summary
+ * as for frw_walk; summary->walk_ms is the host wall time of the call.  The
+ * tree's device must be current.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other threads, other streams) race. */
+enum { FRW_NOALLOC_MAX_RECTS = 1 << 20 };
+int frw_walk_noalloc(const frw_tree* t, const frw_walk_params* p, void* stream, int64_t* counts_host,
+                     frw_walk_summary* summary);
+
 /* ---------------------------------------------------------------- batches
  * Many layouts and masters, each with its own walk count, in one persistent
  * kernel launch.  No reference counterpart: the reference runs one layout and
