@@ -55,7 +55,7 @@ struct WalkLaunch {
     double* h_pts = nullptr;
     double* h_prev = nullptr;
     double* h_dstar = nullptr;
-    unsigned long long* h_count = nullptr;
+    const unsigned long long* h_off = nullptr;  // per-walk offsets of the records (walk order)
     uint64_t h_cap = 0;
     // learned mode
     const MlpNet* net = nullptr;
@@ -71,7 +71,7 @@ struct WalkExtras {
     double* h_pts = nullptr;
     double* h_prev = nullptr;
     double* h_dstar = nullptr;
-    unsigned long long* h_count = nullptr;
+    const unsigned long long* h_off = nullptr;  // per-walk offsets of the records (walk order)
     uint64_t h_cap = 0;
     const MlpNet* net = nullptr;
     const MlpPyramid* pyr = nullptr;

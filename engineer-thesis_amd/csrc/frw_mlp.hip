@@ -816,36 +816,67 @@ int frw_walk_harvest(const frw_tree* t, const frw_walk_params* p, double* pts, d
                      int64_t capacity, int64_t* n_out)
 {
     if (!t || !p || !pts || !h || !dstar || capacity <= 0) return set_error(FRW_E_INVALID, "bad arguments");
-    if (p->master < 0 || p->master >= t->n_rects || p->n_walks <= 0)
-        return set_error(FRW_E_RANGE, "bad master or walk count");
     frw_walk_params q = *p;
     q.mode = FRW_MODE_EXACT;
-    const int64_t nb = t->n_rects + 2;
+    int rc = check_walk_params(t, &q);
+    if (rc != FRW_OK) return rc;
+    // Two passes over the same walks: the first gives each walk's hop count
+    // and end, hence its number of records (every hop that did not end the
+    // walk: hops - 1 when it ended on a rect, hops otherwise); the second
+    // writes walk w's records at the exclusive prefix sum of those counts.
+    // The records thus come in walk and hop order whatever the scheduling,
+    // and a capacity below the total keeps the first walks' records.
+    const int64_t nb = t->n_rects + 2, nw = q.n_walks;
     int64_t *d_counts = nullptr, *d_hops = nullptr;
-    unsigned long long* d_n = nullptr;
-    int rc = check_hip(hipMalloc(&d_counts, nb * 8), "hipMalloc");
+    int32_t *d_hit = nullptr, *d_hp = nullptr;
+    unsigned long long* d_off = nullptr;
+    rc = check_hip(hipMalloc(&d_counts, nb * 8), "hipMalloc");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hops, 8), "hipMalloc");
-    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_n, 8), "hipMalloc");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hit, nw * 4), "hipMalloc");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hp, nw * 4), "hipMalloc");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_off, nw * 8), "hipMalloc");
     if (rc == FRW_OK) rc = check_hip(hipMemset(d_counts, 0, nb * 8), "memset");
     if (rc == FRW_OK) rc = check_hip(hipMemset(d_hops, 0, 8), "memset");
-    if (rc == FRW_OK) rc = check_hip(hipMemset(d_n, 0, 8), "memset");
     if (rc == FRW_OK) {
-        frw_walk_outputs o{d_counts, d_hops, nullptr, nullptr};
+        const frw_walk_outputs o{d_counts, d_hops, d_hit, d_hp};
+        rc = launch_walk(t, &q, &o, nullptr);
+    }
+    if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "harvest pass 1");
+    std::vector<int32_t> hit, hp;
+    std::vector<unsigned long long> off;
+    int64_t used = 0;  // walks of the second pass
+    uint64_t total = 0;
+    if (rc == FRW_OK) {
+        hit.resize((size_t)nw);
+        hp.resize((size_t)nw);
+        off.resize((size_t)nw);
+        rc = check_hip(hipMemcpy(hit.data(), d_hit, nw * 4, hipMemcpyDeviceToHost), "copy hits");
+        if (rc == FRW_OK) rc = check_hip(hipMemcpy(hp.data(), d_hp, nw * 4, hipMemcpyDeviceToHost), "copy hops");
+    }
+    if (rc == FRW_OK) {
+        for (int64_t w = 0; w < nw && total < (uint64_t)capacity; ++w) {
+            off[(size_t)w] = total;
+            total += (uint64_t)hp[(size_t)w] - (hit[(size_t)w] >= 0 ? 1u : 0u);
+            used = w + 1;
+        }
+        rc = check_hip(hipMemcpy(d_off, off.data(), (size_t)used * 8, hipMemcpyHostToDevice), "copy offsets");
+    }
+    if (rc == FRW_OK) rc = check_hip(hipMemset(d_counts, 0, nb * 8), "memset");
+    if (rc == FRW_OK && used > 0) {
+        frw_walk_params q2 = q;
+        q2.n_walks = used;
+        const frw_walk_outputs o{d_counts, d_hops, nullptr, nullptr};
         WalkExtras ex;
         ex.h_pts = pts;
         ex.h_prev = h;
         ex.h_dstar = dstar;
-        ex.h_count = d_n;
+        ex.h_off = d_off;
         ex.h_cap = (uint64_t)capacity;
-        rc = launch_walk(t, &q, &o, nullptr, nullptr, &ex);
+        rc = launch_walk(t, &q2, &o, nullptr, nullptr, &ex);
     }
-    if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "harvest walk");
-    unsigned long long got = 0;
-    if (rc == FRW_OK) rc = check_hip(hipMemcpy(&got, d_n, 8, hipMemcpyDeviceToHost), "copy count");
-    if (rc == FRW_OK && n_out) *n_out = (int64_t)std::min<unsigned long long>(got, (unsigned long long)capacity);
-    (void)hipFree(d_counts);
-    (void)hipFree(d_hops);
-    (void)hipFree(d_n);
+    if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "harvest pass 2");
+    if (rc == FRW_OK && n_out) *n_out = (int64_t)std::min<uint64_t>(total, (uint64_t)capacity);
+    for (void* x : {(void*)d_counts, (void*)d_hops, (void*)d_hit, (void*)d_hp, (void*)d_off}) (void)hipFree(x);
     return rc;
 }
 

@@ -44,11 +44,13 @@ struct WalkArgs {
     int32_t n_jobs;
     int32_t wave_ints;  // LDS per wave in 32-bit words: its lanes' traversal stacks, which the
                         // learned mode's activation tile aliases between traversals
-    // harvest: (point, previous half-size, d*) of non-final exact hops
+    // harvest: (point, previous half-size, d*) of non-final exact hops, the
+    // k-th record of walk w at slot h_off[w] + k (offsets from a first pass
+    // of the same walks, so the records come in walk and hop order)
     double* h_pts;
     double* h_prev;
     double* h_dstar;
-    unsigned long long* h_count;
+    const unsigned long long* h_off;
     uint64_t h_cap;
     // learned proposal
     unsigned long long* l_stats;  // accepted, fallback
@@ -181,6 +183,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
     Counters cnt{0u, 0u, 0u};
     Counters* cp = kStats ? &cnt : nullptr;
     unsigned long long my_nodes = 0, my_rects = 0, my_tabs = 0, my_acc = 0, my_fb = 0;
+    uint64_t my_rec = 0;  // harvest: records of the lane's current walk
 
     // A lane hands in its tallies (master, escape, hops; weighted sums) for
     // its current job and clears them when it moves to another job and when
@@ -274,6 +277,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
         if (got) {
             active = true;
             hop = 0;
+            if (kHarvest) my_rec = 0;
             if (change) {
                 j = nj;
                 t = jobs[j].tree;
@@ -419,7 +423,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
                         result = gload(t.ids + arg);
                     } else {
                         if (kHarvest) {
-                            const unsigned long long slot = atomicAdd(a.h_count, 1ull);
+                            const uint64_t slot = a.h_off[w] + my_rec++;
                             if (slot < a.h_cap) {
                                 a.h_pts[2 * slot] = px;
                                 a.h_pts[2 * slot + 1] = py;
@@ -674,7 +678,7 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
     a.h_pts = L.h_pts;
     a.h_prev = L.h_prev;
     a.h_dstar = L.h_dstar;
-    a.h_count = L.h_count;
+    a.h_off = L.h_off;
     a.h_cap = L.h_cap;
     a.l_stats = L.l_stats;
     a.wtab = L.wtab;
@@ -694,7 +698,7 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
         if (!L.wtab) return set_error(FRW_E_INVALID, "weighted mode needs its tables");
         if (st) launch_kernel<kModeWeighted, true, false>(runs, grid, block, lds, as, s, d_jobs, a, m);
         else launch_kernel<kModeWeighted, false, false>(runs, grid, block, lds, as, s, d_jobs, a, m);
-    } else if (L.h_count) {
+    } else if (L.h_off) {
         launch_kernel<kModeExact, false, true>(runs, grid, block, lds, as, s, d_jobs, a, m);
     } else {
         if (st) launch_kernel<kModeExact, true, false>(runs, grid, block, lds, as, s, d_jobs, a, m);
@@ -752,7 +756,7 @@ int launch_walk_in(const frw_tree* t, const frw_walk_params* p, const frw_walk_o
         L.h_pts = extra->h_pts;
         L.h_prev = extra->h_prev;
         L.h_dstar = extra->h_dstar;
-        L.h_count = extra->h_count;
+        L.h_off = extra->h_off;
         L.h_cap = extra->h_cap;
         L.net = extra->net;
         L.eps = extra->eps;

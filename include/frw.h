@@ -314,12 +314,15 @@ int frw_mlp_add_samples(frw_mlp* m, const frw_mlp* features_of, const double* pt
 int frw_mlp_train(frw_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* last_loss);
 /* Runs exact walks and records, for up to `capacity` hops that did not end
  * the walk, the point, the previous half-size and d* (device arrays pts
- * capacity x 2, h capacity, dstar capacity); *n_out = records written. */
+ * capacity x 2, h capacity, dstar capacity); *n_out = records written.  The
+ * records are in walk-id and hop order (two passes over the same walks), so
+ * the same arguments give the same records in every run. */
 int frw_walk_harvest(const frw_tree* t, const frw_walk_params* p, double* pts, double* h, double* dstar,
                      int64_t capacity, int64_t* n_out);
 /* Walks with the learned proposal (FRW_MODE_LEARNED semantics): per hop the
- * network proposes alpha * s * 2^y; a collision-free Q1 probe accepts it,
- * otherwise the exact search decides the square.  stats (host, may be NULL)
+ * network proposes alpha * s * 2^y; a probe of the occupancy pyramid that
+ * certifies the square free accepts it, otherwise the exact search decides
+ * the square.  stats (host, may be NULL)
  * = {accepted proposals, fallbacks}.  Synchronous. */
 int frw_walk_learned(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p,
                      const frw_walk_outputs* o, float alpha, int64_t stats[2]);

@@ -8,8 +8,9 @@
 * training on harvested pairs lowers the loss;
 * learned walks: every accepted square is collision-free by construction, so
   the hit distribution must equal the exact method's (chi-square homogeneity
-  over independent walk ranges), and the trained network must cut the tree
-  records read per walk (the gain it buys)."""
+  over independent walk ranges);
+* what the network buys: walks/s against the reference's bisection search,
+  and the measured per-hop bound that keeps it below the exact search."""
 import numpy as np
 import pytest
 
@@ -195,6 +196,32 @@ def harvested(setup):
     return pts[:n].contiguous(), h[:n].contiguous(), ds[:n].contiguous()
 
 
+def test_harvest_is_deterministic_and_ordered(setup, harvested):
+    """The records come in walk and hop order: the same walks give the same
+    arrays, and a smaller capacity gives a prefix of them."""
+    torch = torch_dev()
+    lay, rects, bnds, tree, mlp = setup
+    pts, h, ds = harvested
+    n = pts.shape[0]
+    cap = 400_000
+    p2 = torch.zeros((cap, 2), dtype=torch.float64, device="cuda")
+    h2 = torch.zeros(cap, dtype=torch.float64, device="cuda")
+    d2 = torch.zeros(cap, dtype=torch.float64, device="cuda")
+    assert frw.walk_harvest(tree, 0, 40_000, p2, h2, d2, seed=11) == n
+    assert torch.equal(p2[:n], pts) and torch.equal(h2[:n], h) and torch.equal(d2[:n], ds)
+    small = 12_345
+    p3 = torch.zeros((small, 2), dtype=torch.float64, device="cuda")
+    h3 = torch.zeros(small, dtype=torch.float64, device="cuda")
+    d3 = torch.zeros(small, dtype=torch.float64, device="cuda")
+    assert frw.walk_harvest(tree, 0, 40_000, p3, h3, d3, seed=11) == small
+    assert torch.equal(p3, pts[:small]) and torch.equal(d3, ds[:small])
+    # the first walk's records are its hops in order: the first point lies on
+    # the master's Gaussian surface
+    g = tree.gauss_surface(0)
+    x, y = pts[0].tolist()
+    assert g[0] <= x <= g[2] and g[1] <= y <= g[3] and (x in (g[0], g[2]) or y in (g[1], g[3]))
+
+
 def test_harvest_records_exact_distances(setup, harvested):
     torch = torch_dev()
     lay, rects, bnds, tree, mlp = setup
@@ -270,7 +297,7 @@ def _homogeneous(c1, c2, min_expected=20):
     return chi2_contingency(table)[1]
 
 
-def test_learned_walk_distribution_and_gain(setup, trained):
+def test_learned_walk_distribution(setup, trained):
     lay, rects, bnds, tree, mlp = setup
     mlp2 = trained[0]
     n = 200_000
@@ -284,19 +311,111 @@ def test_learned_walk_distribution_and_gain(setup, trained):
     # determinism: the same walks again give the same tallies
     c_l2, hops_l2, acc2, fb2 = _learned(tree, mlp2, 0, n, seed=21)
     assert np.array_equal(c_l, c_l2) and hops_l2 == hops_l and acc2 == acc
-    # gain: tree bytes read per hop, learned vs exact (same walk count).  Per walk the
-    # learned walks' extra hops can cost more than the searches they save: on this
-    # layout with 8-rect leaves the two are within a few per cent, so only the
-    # per-hop figure is asserted and both are printed.
-    m = 20_000
-    tl = frw.walk_learned_traffic(tree, mlp2, 0, m, seed=23)
-    te = tree.walk_traffic(0, m, seed=23)
-    # tree records of the fallback searches + one 16-byte table probe per hop
-    bytes_l = 32 * tl[0] + 16 * tl[1] + 16 * (tl[3] + tl[4]) + 4 * tl[5]
-    bytes_e = 32 * te[0] + 16 * te[1] + 4 * te[3]
-    print(f"\nlearned: hops/walk {tl[2] / m:.2f} accept {tl[3] / max(1, tl[2]):.3f} "
-          f"bytes/walk {bytes_l / m:.0f}; exact: hops/walk {te[2] / m:.2f} bytes/walk {bytes_e / m:.0f}")
-    assert bytes_l / tl[2] < bytes_e / te[2]
+
+
+def test_training_is_deterministic_and_its_seed_spread(setup, harvested):
+    """Training on the same (ordered) harvest with the same seed gives the
+    same parameters bit for bit; over five seeds the acceptance rate of the
+    learned walks is reported and stays within a factor of two."""
+    torch = torch_dev()
+    lay, rects, bnds, tree, mlp = setup
+    pts, h, ds = harvested
+    acc_rates, params = [], []
+    for seed in (1, 2, 3, 4, 5, 1):
+        m = frw.Mlp(tree, seed=seed)
+        m.set_samples(pts, h, ds)
+        m.train(epochs=3, batch=1024, lr=2e-3, seed=seed)
+        params.append(m.params())
+        if len(acc_rates) < 5:
+            c, hp, acc, fb = _learned(tree, m, 0, 50_000, seed=61)
+            acc_rates.append(acc / (acc + fb))
+    print("acceptance over seeds 1-5:", " ".join(f"{a:.3f}" for a in acc_rates))
+    assert np.array_equal(params[0], params[5])
+    assert not np.array_equal(params[0], params[1])
+    assert min(acc_rates) > 0.5 * max(acc_rates), acc_rates
+
+
+def _timed_walks(launch, reps=3):
+    """Walks per second of launch() (HIP events on the current stream, best
+    of `reps` after a warm-up)."""
+    torch = torch_dev()
+    launch()
+    torch.cuda.synchronize()
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        launch()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    return best
+
+
+def _walk_rates(tree, mlp2, n, alpha=0.8):
+    """(walks/s, hops/walk) of bisection, exact and learned walks over the
+    same walk ids, and of learned walks whose every proposal is rejected
+    (alpha so small that no proposal reaches the 2-unit minimum: the
+    network and its features run every hop, then the exact step)."""
+    torch = torch_dev()
+    nr = tree.info().n_rects
+    counts = torch.zeros(nr + 2, dtype=torch.int64, device="cuda")
+    hops = torch.zeros(1, dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {}
+
+    def run(name, launch):
+        hops.zero_()
+        ms = _timed_walks(launch)
+        per = int(hops.item()) / 4 / n  # warm-up + 3 timed launches
+        out[name] = (n / (ms * 1e-3), per)
+
+    run("bisect", lambda: tree.walk_async(0, n, counts, hops, seed=41, mode=frw.MODE_BISECT, stream=stream))
+    run("exact", lambda: tree.walk_async(0, n, counts, hops, seed=41, stream=stream))
+    run("learned", lambda: frw.walk_learned_async(tree, mlp2, 0, n, counts, hops, seed=41, alpha=alpha,
+                                                  stream=stream))
+    run("network_only", lambda: frw.walk_learned_async(tree, mlp2, 0, n, counts, hops, seed=41, alpha=1e-9,
+                                                       stream=stream))
+    for k, (w, hp) in out.items():
+        print(f"{k}: {w:.3e} walks/s, {hp:.2f} hops/walk")
+    return out
+
+
+def test_learned_gain_over_the_plain_bisection(setup, trained):
+    """What the trained network gains over the plain FRW it was built for:
+    the reference's bisection square search (20-45 collision queries per
+    hop, randomWalk/CPU/quadTree.cpp:264-312) on the same walk ids."""
+    lay, rects, bnds, tree, mlp = setup
+    r = _walk_rates(tree, trained[0], 200_000)
+    assert r["learned"][0] > 1.5 * r["bisect"][0], r
+
+
+def test_network_cost_bounds_learned_below_exact(setup, trained):
+    """Why the learned walk cannot beat the exact search: a learned walk costs
+    at least its hops times the network's own per-hop cost, and its hops are
+    no fewer than the exact walk's (an accepted square is certified free, so
+    it is no larger than the exact one).  Measured here: the network and its
+    features alone (every proposal rejected) cost more per hop than a whole
+    exact hop, so no acceptance rate can make the learned walk faster."""
+    lay, rects, bnds, tree, mlp = setup
+    r = _walk_rates(tree, trained[0], 200_000)
+    t_exact = 1.0 / (r["exact"][0] * r["exact"][1])           # s per exact hop
+    t_net = 1.0 / (r["network_only"][0] * r["network_only"][1]) - t_exact  # network + features per hop
+    print(f"per hop: exact {t_exact * 1e12:.1f} ps, network and features {t_net * 1e12:.1f} ps")
+    assert r["learned"][1] >= 0.98 * r["exact"][1]
+    assert t_net > t_exact
+    # hence learned walks/s <= 1 / (hops_learned * t_net) < exact walks/s
+    assert r["learned"][0] <= 1.05 / (r["learned"][1] * t_net)
+    assert r["learned"][0] < r["exact"][0]
+
+
+@pytest.mark.xfail(strict=True, reason="measured: the learned walk runs below the exact search's walks/s "
+                   "(the network's per-hop cost alone exceeds an exact hop; see the bound test above)")
+def test_learned_walks_per_second_reach_the_exact_search(setup, trained):
+    lay, rects, bnds, tree, mlp = setup
+    r = _walk_rates(tree, trained[0], 200_000)
+    assert r["learned"][0] >= r["exact"][0], r
 
 
 def test_learned_alpha_sweep(setup, trained):
