@@ -197,6 +197,7 @@ void frw_tree_free(frw_tree* t)
     if (t->d_err) (void)hipFree(t->d_err);
     if (t->d_surface) (void)hipFree(t->d_surface);
     if (t->d_job) (void)hipFree(t->d_job);
+    if (t->d_wacc) (void)hipFree(t->d_wacc);
     order_free(t->order);
     delete t;
 }
@@ -451,10 +452,23 @@ int frw_walk_weighted_async(const frw_tree* t, const frw_walk_params* p, const f
     if (!(eps > 0.0) || !std::isfinite(eps)) return set_error(FRW_E_INVALID, "eps must be positive");
     if (flags & ~(uint32_t)FRW_WEIGHT_IMPORTANCE) return set_error(FRW_E_INVALID, "unknown weighted-walk flags");
     FRW_GUARD_BEGIN
+    if (!t->d_wacc) {  // the tree's fixed-point accumulators, zero between launches
+        frw_tree* tt = const_cast<frw_tree*>(t);
+        const size_t bytes = sizeof(unsigned long long) * 4 * (size_t)(t->n_rects + 2);
+        rc = check_hip(hipMalloc(&tt->d_wacc, bytes), "hipMalloc weighted accumulators");
+        if (rc == FRW_OK) rc = check_hip(hipMemset(tt->d_wacc, 0, bytes), "memset weighted accumulators");
+        if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "weighted accumulators");
+        if (rc != FRW_OK) {
+            (void)hipFree(tt->d_wacc);
+            tt->d_wacc = nullptr;
+            return rc;
+        }
+    }
     q.mode = FRW_MODE_WEIGHTED;
     WalkExtras ex;
     ex.wsum = wsum;
     ex.wsq = wsq;
+    ex.wacc = t->d_wacc;
     ex.w_out = w_out;
     ex.eps = eps;
     ex.wflags = flags;

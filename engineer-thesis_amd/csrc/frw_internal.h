@@ -30,8 +30,8 @@ struct WalkJob {
     int32_t start_hint;            // sorted index of a rect at the master's centre (prologue): the
                                    // first hop's nearest-rect search starts from its distance
     int4 start_rect;               // that rect
-    double* wsum;                  // weighted mode: per-bin sums of weights and of squared weights
-    double* wsq;
+    unsigned long long* wacc;      // weighted mode: per-bin fixed-point sums of weights and of squared
+                                   // weights (4 words per bin), converted at the end of the launch
     double* w_out;                 // weighted mode: optional per-walk weights
     MlpPyramid pyr;                // learned mode: the layout's occupancy pyramid
 };
@@ -78,6 +78,7 @@ struct WalkExtras {
     unsigned long long* l_stats = nullptr;
     double* wsum = nullptr;  // weighted mode outputs
     double* wsq = nullptr;
+    unsigned long long* wacc = nullptr;  // and the launch's fixed-point accumulators (4 words per bin, zero)
     double* w_out = nullptr;
     double eps = 1.0;
     uint32_t wflags = 0;  // FRW_WEIGHT_* options
@@ -134,6 +135,8 @@ struct frw_tree {
     int* d_err = nullptr;                  // kernel error bits
     double* d_surface = nullptr;           // Gaussian surface of the master
     frw::WalkJob* d_job = nullptr;         // job slot of a plain walk
+    unsigned long long* d_wacc = nullptr;  // weighted walks' fixed-point accumulators (4 words per bin, kept zero
+                                           // between launches), allocated by the first weighted walk
     mutable frw::LaunchOrder order;        // launches on this tree run one after another
 };
 

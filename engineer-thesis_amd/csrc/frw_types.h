@@ -14,6 +14,7 @@ constexpr int kStartLevels = 11;     // Morton levels 0..10 in the nearest-searc
 // Error bits reported by kernels.
 constexpr int kErrBisect = 1;        // biggest-square search exceeded kMaxBisect
 constexpr int kErrStack = 2;         // traversal stack would overflow
+constexpr int kErrWeight = 4;        // a weighted walk's weight beyond the fixed-point range (|w| >= 2^31)
 
 // Flattened quadtree node, 32 bytes.  Children are stored as a group of four
 // consecutive nodes (quadrants in Morton order; an empty quadrant is a dummy

@@ -67,13 +67,72 @@ constexpr int kWaveChunk = 256;                                 // walk indices 
 constexpr int kTailChunk = 64;                                  // ... once the queue nears its end
 constexpr uint32_t kGausDomain = 0x47415553u;                   // 'GAUS': Gaussian-surface start points
 
-// Compensated (Kahan) accumulation; -ffp-contract=off keeps it exact.
-__device__ inline void kahan_add(double& s, double& c, double x)
+// Exact fixed-point sums of the weighted estimator.  Each walk's weight w is
+// rounded once, to q = w 2^32 as an int64 (|w| < 2^31, else an error flag),
+// and its square to (q^2) >> 32; both are added as 128-bit two's-complement
+// integers (lo, hi).  Integer addition is associative, so the sums, and the
+// doubles they convert to at the end of the launch, do not depend on the
+// order in which walks end: any launch shape gives the same bits.
+struct Fx {
+    unsigned long long lo, hi;
+};
+
+__device__ inline void fx_add(Fx& a, const Fx& b)
 {
-    const double y = x - c;
-    const double t = s + y;
-    c = (t - s) - y;
-    s = t;
+    const unsigned long long l = a.lo + b.lo;
+    a.hi += b.hi + (l < a.lo ? 1ull : 0ull);
+    a.lo = l;
+}
+
+// p[0] += lo, p[1] += hi + carry: two 64-bit atomics form one exact 128-bit add
+__device__ inline void fx_atomic(unsigned long long* p, const Fx& b)
+{
+    if (b.lo == 0ull && b.hi == 0ull) return;
+    const unsigned long long old = atomicAdd(p, b.lo);
+    atomicAdd(p + 1, b.hi + (old + b.lo < old ? 1ull : 0ull));
+}
+
+__device__ inline void fx_weight(double w, Fx& sum_term, Fx& sq_term, int* err)
+{
+    double a = w * 4294967296.0;
+    if (!(fabs(a) < 9223372036854775808.0)) {
+        raise_error(err, kErrWeight);
+        a = 0.0;
+    }
+    const long long v = __double2ll_rn(a);
+    sum_term.lo = (unsigned long long)v;
+    sum_term.hi = v < 0 ? ~0ull : 0ull;
+    const unsigned long long u = v < 0 ? 0ull - (unsigned long long)v : (unsigned long long)v;
+    const unsigned long long plo = u * u, phi = __umul64hi(u, u);
+    sq_term.lo = (plo >> 32) | (phi << 32);
+    sq_term.hi = phi >> 32;
+}
+
+// The fixed-point sums of a launch (4 words per bin: sum lo, hi, squares lo,
+// hi) added to the caller's double bins and cleared for the next launch.  A
+// sum converts as sign and magnitude, magnitude = hi 2^64 + lo, scaled by
+// 2^-32.
+__global__ void weighted_finish(unsigned long long* __restrict__ acc, int64_t n_bins, double* __restrict__ wsum,
+                                double* __restrict__ wsq)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_bins) return;
+    unsigned long long* p = acc + 4 * i;
+    double v[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        unsigned long long lo = p[2 * k], hi = p[2 * k + 1];
+        const bool neg = (long long)hi < 0;
+        if (neg) {  // 128-bit negation
+            lo = ~lo + 1ull;
+            hi = ~hi + (lo == 0ull ? 1ull : 0ull);
+        }
+        const double mag = (double)hi * 18446744073709551616.0 + (double)lo;
+        v[k] = (neg ? -mag : mag) * (1.0 / 4294967296.0);
+    }
+    wsum[i] += v[0];
+    wsq[i] += v[1];
+    p[0] = p[1] = p[2] = p[3] = 0ull;
 }
 constexpr int kMlpLdsPerWave = kMlpTileElems * 2;               // the bf16 activation tile
 
@@ -134,12 +193,14 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
     // weighted: master and escape sums and sums of squares), added to the
     // jobs' bins once when the block ends
     __shared__ unsigned long long s_tally[3 * kLdsJobs];
-    __shared__ double s_wtally[kMode == kModeWeighted ? 4 * kLdsJobs : 1];
+    // weighted (single-job launches): the block's fixed-point master and
+    // escape sums and sums of squares
+    __shared__ unsigned long long s_wtally[kMode == kModeWeighted ? 8 : 1];
     for (int i = threadIdx.x; i < kNSample; i += blockDim.x) sT[i] = a.thresholds[i];
     for (int i = threadIdx.x; i < min(a.n_jobs, kLdsJobs); i += blockDim.x) s_begin[i] = jobs[i].walk_begin;
     for (int i = threadIdx.x; i < 3 * kLdsJobs; i += blockDim.x) s_tally[i] = 0ull;
     if (kMode == kModeWeighted)
-        for (int i = threadIdx.x; i < 4 * kLdsJobs; i += blockDim.x) s_wtally[i] = 0.0;
+        for (int i = threadIdx.x; i < 8; i += blockDim.x) s_wtally[i] = 0ull;
     __syncthreads();
     const int lane = threadIdx.x & (kWaveSize - 1);
     using Stack = typename std::conditional<kRuns, RunStack, PlainStack>::type;
@@ -175,11 +236,11 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
     uint64_t chunk = kWaveChunk;
     const uint64_t tail_walks = (uint64_t)gridDim.x * (blockDim.x / kWaveSize) * 4 * kWaveChunk;
     // weighted estimator: the walk's weight, its preset first point, and
-    // per-lane compensated sums (and sums of squares) for the master and
+    // per-lane fixed-point sums (and sums of squares) for the master and
     // escape bins
     double wgt = 0.0, fpx = 0.0, fpy = 0.0;
     bool first_pt = false, start_stuck = false;
-    double wm_s = 0.0, wm_c = 0.0, wm_q = 0.0, we_s = 0.0, we_c = 0.0, we_q = 0.0;
+    Fx wm_s{0, 0}, wm_q{0, 0}, we_s{0, 0}, we_q{0, 0};
     Counters cnt{0u, 0u, 0u};
     Counters* cp = kStats ? &cnt : nullptr;
     unsigned long long my_nodes = 0, my_rects = 0, my_tabs = 0, my_acc = 0, my_fb = 0;
@@ -192,39 +253,24 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
     // atomics on a job's bins serialised: every lane of the grid leaves a
     // job at about the same time), straight to the bins beyond.
     auto hand_in = [&]() {
-        const double ws0 = wm_s - wm_c, ws1 = we_s - we_c;
         if (j < kLdsJobs) {
             if (my_master) atomicAdd(&s_tally[3 * j], (unsigned long long)my_master);
             if (my_escaped) atomicAdd(&s_tally[3 * j + 1], (unsigned long long)my_escaped);
             if (my_hops) atomicAdd(&s_tally[3 * j + 2], my_hops);
-            if (kMode == kModeWeighted) {
-                if (ws0 != 0.0 || wm_q != 0.0) {
-                    atomicAdd(&s_wtally[4 * j], ws0);
-                    atomicAdd(&s_wtally[4 * j + 1], wm_q);
-                }
-                if (ws1 != 0.0 || we_q != 0.0) {
-                    atomicAdd(&s_wtally[4 * j + 2], ws1);
-                    atomicAdd(&s_wtally[4 * j + 3], we_q);
-                }
-            }
         } else {
             if (my_master) atomicAdd(&jobs[j].counts[mid], (unsigned long long)my_master);
             if (my_escaped) atomicAdd(&jobs[j].counts[t.n_rects], (unsigned long long)my_escaped);
             if (my_hops) atomicAdd(jobs[j].hops, my_hops);
-            if (kMode == kModeWeighted) {
-                if (ws0 != 0.0 || wm_q != 0.0) {
-                    atomicAdd(&jobs[j].wsum[mid], ws0);
-                    atomicAdd(&jobs[j].wsq[mid], wm_q);
-                }
-                if (ws1 != 0.0 || we_q != 0.0) {
-                    atomicAdd(&jobs[j].wsum[t.n_rects], ws1);
-                    atomicAdd(&jobs[j].wsq[t.n_rects], we_q);
-                }
-            }
+        }
+        if (kMode == kModeWeighted) {  // single-job launches: j == 0
+            fx_atomic(&s_wtally[0], wm_s);
+            fx_atomic(&s_wtally[2], wm_q);
+            fx_atomic(&s_wtally[4], we_s);
+            fx_atomic(&s_wtally[6], we_q);
+            wm_s = wm_q = we_s = we_q = Fx{0, 0};
         }
         my_master = my_escaped = 0;
         my_hops = 0;
-        if (kMode == kModeWeighted) wm_s = wm_c = wm_q = we_s = we_c = we_q = 0.0;
     };
 
     for (;;) {
@@ -459,16 +505,18 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
             }
             if (kMode == kModeWeighted) {
                 if (jobs[j].w_out) jobs[j].w_out[w] = wgt;
+                Fx s1, q1;
+                fx_weight(wgt, s1, q1, a.err);
                 if (result == mid) {
-                    kahan_add(wm_s, wm_c, wgt);
-                    wm_q += wgt * wgt;
+                    fx_add(wm_s, s1);
+                    fx_add(wm_q, q1);
                 } else if (result == -1) {
-                    kahan_add(we_s, we_c, wgt);
-                    we_q += wgt * wgt;
+                    fx_add(we_s, s1);
+                    fx_add(we_q, q1);
                 } else {
                     const int bin = result >= 0 ? result : t.n_rects + 1;
-                    atomicAdd(&jobs[j].wsum[bin], wgt);
-                    atomicAdd(&jobs[j].wsq[bin], wgt * wgt);
+                    fx_atomic(jobs[j].wacc + 4 * bin, s1);
+                    fx_atomic(jobs[j].wacc + 4 * bin + 2, q1);
                 }
             }
             my_hops += hop;
@@ -486,15 +534,11 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
         if (m) atomicAdd(&jobs[i].counts[lm], m);
         if (e) atomicAdd(&jobs[i].counts[le], e);
         if (h) atomicAdd(jobs[i].hops, h);
-        if (kMode == kModeWeighted) {
-            if (s_wtally[4 * i] != 0.0 || s_wtally[4 * i + 1] != 0.0) {
-                atomicAdd(&jobs[i].wsum[lm], s_wtally[4 * i]);
-                atomicAdd(&jobs[i].wsq[lm], s_wtally[4 * i + 1]);
-            }
-            if (s_wtally[4 * i + 2] != 0.0 || s_wtally[4 * i + 3] != 0.0) {
-                atomicAdd(&jobs[i].wsum[le], s_wtally[4 * i + 2]);
-                atomicAdd(&jobs[i].wsq[le], s_wtally[4 * i + 3]);
-            }
+        if (kMode == kModeWeighted && i == 0) {
+            fx_atomic(jobs[0].wacc + 4 * lm, Fx{s_wtally[0], s_wtally[1]});
+            fx_atomic(jobs[0].wacc + 4 * lm + 2, Fx{s_wtally[2], s_wtally[3]});
+            fx_atomic(jobs[0].wacc + 4 * le, Fx{s_wtally[4], s_wtally[5]});
+            fx_atomic(jobs[0].wacc + 4 * le + 2, Fx{s_wtally[6], s_wtally[7]});
         }
     }
     if (kStats) {
@@ -577,7 +621,7 @@ void launch_kernel(bool runs, int grid, int block, size_t lds, const AutoShape& 
 // Static LDS of a walk block: Green thresholds, job offsets, job tallies.
 size_t walk_static_lds(int mode)
 {
-    return kNSample * 4 + kLdsJobs * 8 + kLdsJobs * 24 + (mode == kModeWeighted ? kLdsJobs * 32 : 8);
+    return kNSample * 4 + kLdsJobs * 8 + kLdsJobs * 24 + (mode == kModeWeighted ? 64 : 8);
 }
 
 // LDS of one wave: its lanes' stacks, or the learned mode's activation
@@ -696,6 +740,7 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
         else launch_kernel<kModeBisect, false, false>(runs, grid, block, lds, as, s, d_jobs, a, m);
     } else if (L.mode == kModeWeighted) {
         if (!L.wtab) return set_error(FRW_E_INVALID, "weighted mode needs its tables");
+        if (L.n_jobs != 1) return set_error(FRW_E_INVALID, "weighted walks run one job per launch");
         if (st) launch_kernel<kModeWeighted, true, false>(runs, grid, block, lds, as, s, d_jobs, a, m);
         else launch_kernel<kModeWeighted, false, false>(runs, grid, block, lds, as, s, d_jobs, a, m);
     } else if (L.h_off) {
@@ -729,8 +774,7 @@ int launch_walk_in(const frw_tree* t, const frw_walk_params* p, const frw_walk_o
     jb.master = t->host_rects[p->master];
     jb.master_id = p->master;
     if (extra) {
-        jb.wsum = extra->wsum;
-        jb.wsq = extra->wsq;
+        jb.wacc = extra->wacc;
         jb.w_out = extra->w_out;
     }
     if (extra && extra->pyr) jb.pyr = *extra->pyr;
@@ -764,7 +808,13 @@ int launch_walk_in(const frw_tree* t, const frw_walk_params* p, const frw_walk_o
         L.wtab = extra->wsum ? t->d_wtab : nullptr;
         L.l_stats = extra->l_stats;
     }
-    return launch_walk_jobs(ws.job, L, s);
+    int rc2 = launch_walk_jobs(ws.job, L, s);
+    if (rc2 != FRW_OK || p->mode != kModeWeighted) return rc2;
+    if (!extra || !extra->wacc || !extra->wsum || !extra->wsq)
+        return set_error(FRW_E_INVALID, "weighted walks need their accumulators");
+    const int64_t nb = t->n_rects + 2;
+    weighted_finish<<<(unsigned)((nb + 255) / 256), 256, 0, s>>>(extra->wacc, nb, extra->wsum, extra->wsq);
+    return check_hip(hipGetLastError(), "weighted_finish");
 }
 
 // Walks on one tree share its job slot, work counter and error flag, so each

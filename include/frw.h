@@ -360,8 +360,11 @@ int frw_batch_autotune_learned(frw_batch* b, const frw_batch_params* base, const
  * escape bin is the grounded boundary), wsq[j] the second moment.
  * counts/hops as for frw_walk_async; wsum, wsq: device double[n_rects + 2],
  * accumulated (caller zeroes); w_out: optional device double[n_walks].
- * Sums are compensated per lane and added across lanes with FP64 atomics
- * (their order, hence the last bits, varies run to run).
+ * The sums are exact: each weight is rounded once to a multiple of 2^-32
+ * (q = rint(w 2^32), |w| < 2^31, else FRW_E_KERNEL), the q and q^2 >> 32 of
+ * the walks are added as 128-bit integers in the tree's accumulators, and
+ * each launch adds their conversion to wsum and wsq, so the results are the
+ * same bits for any launch shape or run.
  * flags: FRW_WEIGHT_IMPORTANCE draws the first panel with probability q_k
  * proportional to |n . g_k| instead of p_k (Yu et al. 2013, importance
  * sampling of the first hop); the weight divides by q_k, so every weight of

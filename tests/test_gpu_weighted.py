@@ -2,8 +2,10 @@
 
 * walk for walk: the same start point, first square, panel and weight, then
   the same exact walk (hit rect and hop count) as the oracle's restatement;
-* the per-conductor weight sums equal the oracle's within FP64 summation
-  order;
+* the per-conductor weight sums are exact fixed-point sums of the per-walk
+  weights (each rounded once to a multiple of 2^-32): bit-identical for any
+  launch shape, equal to the same sums taken on the host, and equal to the
+  oracle's sums within the oracle's own FP64 rounding;
 * physics: a parallel-plate pair couples with about -length/gap (plus
   fringe), the self term is positive, and all weights (escapes included)
   average to zero within the Monte Carlo error, at 10^6 rects too."""
@@ -101,6 +103,60 @@ def test_weighted_seed_job_offset_and_max_hops():
         np.testing.assert_allclose(w, ow, rtol=1e-13, atol=0)
 
 
+def fixed_point_sums(w, bins, nb):
+    """The sums the kernel forms: q = rint(w 2^32) per walk, sum of q and of
+    q^2 >> 32 per bin as exact integers, converted as sign and magnitude
+    (hi 2^64 + lo) and scaled by 2^-32."""
+    q = np.rint(np.asarray(w) * 2.0 ** 32).astype(np.int64).tolist()
+    S, Q = [0] * nb, [0] * nb
+    for b, v in zip(np.asarray(bins).tolist(), q):
+        S[b] += v
+        Q[b] += (v * v) >> 32
+
+    def conv(v):
+        m = -v if v < 0 else v
+        d = float(m >> 64) * 18446744073709551616.0 + float(m & ((1 << 64) - 1))
+        return (-d if v < 0 else d) * (1.0 / 4294967296.0)
+
+    return np.array([conv(v) for v in S]), np.array([conv(v) for v in Q])
+
+
+@pytest.mark.parametrize("flags", [0, frw.WEIGHT_IMPORTANCE])
+def test_weighted_sums_exact_for_any_launch_shape(flags):
+    torch = torch_dev()
+    lay, rects, bounds = load_fixture("test1000.txt")
+    tree = frw.Tree(lay)
+    nr = len(rects)
+    n = 200_000
+    outs = []
+    for block, grid in ((0, 0), (64, 37), (256, 1000)):
+        counts = torch.zeros(nr + 2, dtype=torch.int64, device="cuda")
+        hops = torch.zeros(1, dtype=torch.int64, device="cuda")
+        wsum = torch.zeros(nr + 2, dtype=torch.float64, device="cuda")
+        wsq = torch.zeros(nr + 2, dtype=torch.float64, device="cuda")
+        w_out = torch.zeros(n, dtype=torch.float64, device="cuda")
+        hit = torch.zeros(n, dtype=torch.int32, device="cuda")
+        tree.walk_weighted_async(0, n, counts, hops, wsum, wsq, w_out=w_out, hit_out=hit, seed=3, block=block,
+                                 grid=grid, flags=flags)
+        torch.cuda.synchronize()
+        outs.append([t.cpu().numpy() for t in (counts, wsum, wsq, w_out, hit)])
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)  # bit for bit, whatever the shape
+    counts, wsum, wsq, w, hit = outs[0]
+    bins = np.where(hit >= 0, hit, np.where(hit == -1, nr, nr + 1))
+    S, Q = fixed_point_sums(w, bins, nr + 2)
+    assert np.array_equal(wsum, S) and np.array_equal(wsq, Q)
+    # a second launch into the same bins adds its own converted sums
+    wsum2 = torch.from_numpy(wsum.copy()).cuda()
+    wsq2 = torch.from_numpy(wsq.copy()).cuda()
+    c2 = torch.zeros(nr + 2, dtype=torch.int64, device="cuda")
+    h2 = torch.zeros(1, dtype=torch.int64, device="cuda")
+    tree.walk_weighted_async(0, n, c2, h2, wsum2, wsq2, seed=3, flags=flags)
+    torch.cuda.synchronize()
+    assert np.array_equal(wsum2.cpu().numpy(), wsum + S) and np.array_equal(wsq2.cpu().numpy(), wsq + Q)
+
+
 def _plates(gap=100, length=2000, thick=10):
     return np.array([[0, 0, length, thick], [0, thick + gap, length, 2 * thick + gap]], dtype=np.int32)
 
@@ -118,10 +174,10 @@ def test_parallel_plates():
     tot = wsum.sum() / n
     sig_tot = np.sqrt(max(wsq.sum() / n - tot * tot, 0) / n)
     assert abs(tot) < 4 * sig_tot, (tot, sig_tot)
-    # the same walks again: identical tallies, sums equal to the last bits
+    # the same walks again: identical tallies and sums, bit for bit
     counts2, wsum2, wsq2, hops2 = tree.walk_weighted(0, n, seed=11)
     assert np.array_equal(counts, counts2) and hops == hops2
-    np.testing.assert_allclose(wsum2, wsum, rtol=1e-12, atol=1e-9 * np.abs(wsum).max())
+    assert np.array_equal(wsum2, wsum) and np.array_equal(wsq2, wsq)
 
 
 def test_weighted_at_scale():
