@@ -95,6 +95,36 @@ __host__ __device__ inline uint16_t f2bf(float x)
 
 __host__ __device__ inline float bf2f(uint16_t b) { return __builtin_bit_cast(float, (uint32_t)b << 16); }
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bfx2 __attribute__((ext_vector_type(2)));
+// Four floats to bf16 with the hardware's packed round-to-nearest-even
+// conversion (v_cvt_pk_bf16_f32, two values per instruction); equal to f2bf
+// for every finite value (a host build of this header uses f2bf itself).
+__host__ __device__ inline bf16x4 to_bf16x4(float a, float b, float c, float d)
+{
+    bf16x4 v;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const bfx2 lo = __builtin_convertvector((f32x2){a, b}, bfx2);
+    const bfx2 hi = __builtin_convertvector((f32x2){c, d}, bfx2);
+    const uint32_t l = __builtin_bit_cast(uint32_t, lo), h = __builtin_bit_cast(uint32_t, hi);
+    v[0] = (short)(l & 0xFFFFu);
+    v[1] = (short)(l >> 16);
+    v[2] = (short)(h & 0xFFFFu);
+    v[3] = (short)(h >> 16);
+#else
+    v[0] = (short)f2bf(a);
+    v[1] = (short)f2bf(b);
+    v[2] = (short)f2bf(c);
+    v[3] = (short)f2bf(d);
+#endif
+    return v;
+}
+
+// The network's small operands staged in LDS by every block that evaluates
+// it: b1 | b2 | b3 | w4 (as FP32) | b4.
+constexpr int kMlpSmallB1 = 0, kMlpSmallB2 = 64, kMlpSmallB3 = 128, kMlpSmallW4 = 192, kMlpSmallB4 = 256;
+constexpr int kMlpSmallFloats = 260;
+
 // erf by Abramowitz & Stegun 7.1.26 (Hastings' rational approximation,
 // |error| <= 1.5e-7 in exact arithmetic, well inside a bf16 activation's
 // rounding): erf(x) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2)
@@ -112,7 +142,27 @@ __device__ inline float erf_as(float x)
     return copysignf(e, x);
 }
 
-__device__ inline float gelu_f(float z) { return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f)); }
+// GELU(z) = z Phi(z) = max(z, 0) - t Phi(-t), t = |z|, with Phi(-t) replaced by
+// a degree-10 polynomial in x = 1 - t / 4.5 that vanishes at t = 4.5 (where
+// t Phi(-t) < 1.6e-5), t clamped there: FMA and min/max only, no
+// transcendental; within 1.7e-5 of the exact GELU, far inside a bf16
+// activation's rounding.  Coefficients from fit_gelu.py (a minimax fit).
+__device__ inline float gelu_f(float z)
+{
+    const float t = fminf(fabsf(z), 4.5f);
+    const float x = __builtin_fmaf(t, -1.0f / 4.5f, 1.0f);
+    float p = -7.503951923e-03f;
+    p = __builtin_fmaf(p, x, 1.011371091e+01f);
+    p = __builtin_fmaf(p, x, -3.778662722e+01f);
+    p = __builtin_fmaf(p, x, 5.380414907e+01f);
+    p = __builtin_fmaf(p, x, -3.819883533e+01f);
+    p = __builtin_fmaf(p, x, 1.585222026e+01f);
+    p = __builtin_fmaf(p, x, -3.753199773e+00f);
+    p = __builtin_fmaf(p, x, 5.077477714e-01f);
+    p = __builtin_fmaf(p, x, -3.255581238e-02f);
+    p = __builtin_fmaf(p, x, 9.095615919e-04f);
+    return fmaxf(z, 0.0f) - t * (p * x);
+}
 
 __device__ inline float gelu_grad_f(float z)
 {
@@ -121,6 +171,8 @@ __device__ inline float gelu_grad_f(float z)
     return cdf + z * pdf;
 }
 
+// The feature level of a point whose previous square had half-size h, and
+// its cell side s = 2^(K0 - l).
 __device__ inline int mlp_level(const MlpPyramid& m, double h, double* cell)
 {
     const int e = h >= 1.0 ? ilogb(h) : 0;
@@ -137,8 +189,11 @@ __device__ inline uint64_t mlp_window(const MlpPyramid& m, int l, double cell, d
                                       float* frac_y)
 {
     const int64_t n = (int64_t)1 << l;
-    const double qx = (px - (double)m.bounds.x) / cell;
-    const double qy = (py - (double)m.bounds.y) / cell;
+    // cell is 2^(K0 - l): multiplying by its exact inverse equals the division
+    (void)cell;
+    const double inv = ldexp(1.0, l - m.K0);
+    const double qx = (px - (double)m.bounds.x) * inv;
+    const double qy = (py - (double)m.bounds.y) * inv;
     const double fx = floor(qx);
     const double fy = floor(qy);
     *frac_x = 0.0f;
@@ -201,10 +256,10 @@ __device__ inline bool mlp_certify(const MlpPyramid& m, double x0, double y0, do
     if (!in_bounds_box(m.bounds, x0, y0, x1, y1)) return false;
     int v = m.K0 - ilogb(d) + kCertShift;
     v = v < 0 ? 0 : (v > m.Vmax ? m.Vmax : v);
-    const double s = ldexp(1.0, m.K0 - v);
+    const double inv = ldexp(1.0, v - m.K0);  // the exact inverse of the cell side 2^(K0 - v)
     const int64_t n = (int64_t)1 << v;
-    const double a0 = floor((x0 - (double)m.bounds.x) / s), a1 = floor((x1 - (double)m.bounds.x) / s);
-    const double b0 = floor((y0 - (double)m.bounds.y) / s), b1 = floor((y1 - (double)m.bounds.y) / s);
+    const double a0 = floor((x0 - (double)m.bounds.x) * inv), a1 = floor((x1 - (double)m.bounds.x) * inv);
+    const double b0 = floor((y0 - (double)m.bounds.y) * inv), b1 = floor((y1 - (double)m.bounds.y) * inv);
     if (!(a0 >= 0.0 && b0 >= 0.0 && a1 < (double)n && b1 < (double)n)) return false;
     const int64_t i0 = (int64_t)a0, i1 = (int64_t)a1 + 1, j0 = (int64_t)b0, j1 = (int64_t)b1 + 1;
     const uint32_t* S = m.sat + m.sat_off[v];
@@ -230,7 +285,7 @@ __device__ inline void wave_sync()
 // both 32-neuron halves read the whole block, so the first half's outputs
 // wait in the lane's staging slots until the second half has been computed.
 template <int K>
-__device__ inline void mlp_layer(uint16_t* buf, const uint16_t* W, const float* bias, int lane)
+__device__ inline void mlp_layer(uint16_t* buf, const uint16_t* W, const float* __restrict__ bias, int lane)
 {
     const int r32 = lane & 31, q = lane >> 5;
     bf16x4* held = (bf16x4*)(buf + 64 * kMlpStride) + 4 * lane;
@@ -250,9 +305,9 @@ __device__ inline void mlp_layer(uint16_t* buf, const uint16_t* W, const float* 
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int n0 = nj * 32 + 8 * g + 4 * q;
-                bf16x4 v;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) v[u] = (short)f2bf(gelu_f(acc[4 * g + u] + bias[n0 + u]));
+                const float4 b = *(const float4*)(bias + n0);  // one LDS read for four biases
+                const bf16x4 v = to_bf16x4(gelu_f(acc[4 * g] + b.x), gelu_f(acc[4 * g + 1] + b.y),
+                                           gelu_f(acc[4 * g + 2] + b.z), gelu_f(acc[4 * g + 3] + b.w));
                 if (nj == 0)
                     held[g] = v;
                 else
@@ -269,7 +324,7 @@ __device__ inline void mlp_layer(uint16_t* buf, const uint16_t* W, const float* 
 // the 32 neurons it holds of rows r and 32 + r (lane = r + 32 q); the other
 // 32 of each row are in lane r + 32 (1 - q), which sends its sum for the
 // row this lane returns.  The activations are not stored.
-__device__ inline float mlp_last_layer(const uint16_t* buf, const MlpNet& m, int lane)
+__device__ inline float mlp_last_layer(const uint16_t* buf, const MlpNet& m, const float* __restrict__ small, int lane)
 {
     const int r32 = lane & 31, q = lane >> 5;
     float p0 = 0.0f, p1 = 0.0f;
@@ -290,9 +345,14 @@ __device__ inline float mlp_last_layer(const uint16_t* buf, const MlpNet& m, int
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int n0 = nj * 32 + 8 * g + 4 * q;
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    part += bf2f(m.w4[n0 + u]) * bf2f(f2bf(gelu_f(acc[4 * g + u] + m.b3[n0 + u])));
+                const float4 b = *(const float4*)(small + kMlpSmallB3 + n0);
+                const float4 w = *(const float4*)(small + kMlpSmallW4 + n0);
+                const bf16x4 a = to_bf16x4(gelu_f(acc[4 * g] + b.x), gelu_f(acc[4 * g + 1] + b.y),
+                                           gelu_f(acc[4 * g + 2] + b.z), gelu_f(acc[4 * g + 3] + b.w));
+                part += w.x * bf2f((uint16_t)a[0]);
+                part += w.y * bf2f((uint16_t)a[1]);
+                part += w.z * bf2f((uint16_t)a[2]);
+                part += w.w * bf2f((uint16_t)a[3]);
             }
         }
         if (mi == 0)
@@ -301,21 +361,36 @@ __device__ inline float mlp_last_layer(const uint16_t* buf, const MlpNet& m, int
             p1 = part;
     }
     const float other = __shfl_xor(q ? p0 : p1, 32);
-    return (q ? p1 : p0) + other + *m.b4;
+    return (q ? p1 : p0) + other + small[kMlpSmallB4];
+}
+
+// Stages the network's biases, output weights and output bias in LDS
+// (kMlpSmallFloats floats); the caller synchronises the block afterwards.
+__device__ inline void mlp_stage_small(const MlpNet& m, float* small, int tid, int n_threads)
+{
+    for (int i = tid; i < kMlpSmallFloats; i += n_threads) {
+        float v = 0.0f;
+        if (i < kMlpSmallB2) v = m.b1[i];
+        else if (i < kMlpSmallB3) v = m.b2[i - kMlpSmallB2];
+        else if (i < kMlpSmallW4) v = m.b3[i - kMlpSmallB3];
+        else if (i < kMlpSmallB4) v = bf2f(m.w4[i - kMlpSmallW4]);
+        else if (i == kMlpSmallB4) v = *m.b4;
+        small[i] = v;
+    }
 }
 
 // Forward pass of the wave's 64 feature rows in xs (LDS, kMlpTileElems:
 // 64 x kMlpStride, overwritten by the hidden activations, and the staging
-// slots).  Returns row `lane`'s output.
-// Must be called by all 64 lanes together.
-__device__ inline float mlp_forward_wave(const MlpNet& m, uint16_t* xs, int lane)
+// slots), with the small operands staged by mlp_stage_small.  Returns row
+// `lane`'s output.  Must be called by all 64 lanes together.
+__device__ inline float mlp_forward_wave(const MlpNet& m, uint16_t* xs, const float* small, int lane)
 {
     wave_sync();
-    mlp_layer<kMlpInPad>(xs, m.W1, m.b1, lane);
+    mlp_layer<kMlpInPad>(xs, m.W1, small + kMlpSmallB1, lane);
     wave_sync();
-    mlp_layer<kMlpHid>(xs, m.W2, m.b2, lane);
+    mlp_layer<kMlpHid>(xs, m.W2, small + kMlpSmallB2, lane);
     wave_sync();
-    const float y = mlp_last_layer(xs, m, lane);
+    const float y = mlp_last_layer(xs, m, small, lane);
     wave_sync();
     return y;
 }

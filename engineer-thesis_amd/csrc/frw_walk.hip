@@ -196,11 +196,14 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
     // weighted (single-job launches): the block's fixed-point master and
     // escape sums and sums of squares
     __shared__ unsigned long long s_wtally[kMode == kModeWeighted ? 8 : 1];
+    // learned mode: the network's biases and output weights
+    __shared__ __align__(16) float s_small[kMode == kModeLearned ? kMlpSmallFloats : 4];
     for (int i = threadIdx.x; i < kNSample; i += blockDim.x) sT[i] = a.thresholds[i];
     for (int i = threadIdx.x; i < min(a.n_jobs, kLdsJobs); i += blockDim.x) s_begin[i] = jobs[i].walk_begin;
     for (int i = threadIdx.x; i < 3 * kLdsJobs; i += blockDim.x) s_tally[i] = 0ull;
     if (kMode == kModeWeighted)
         for (int i = threadIdx.x; i < 8; i += blockDim.x) s_wtally[i] = 0ull;
+    if (kMode == kModeLearned) mlp_stage_small(net, s_small, threadIdx.x, blockDim.x);
     __syncthreads();
     const int lane = threadIdx.x & (kWaveSize - 1);
     using Stack = typename std::conditional<kRuns, RunStack, PlainStack>::type;
@@ -434,7 +437,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
                 bits = mlp_window(pyr, level, cell, px, py, &fx, &fy);
             }
             mlp_feature_row(mlp_xs + lane * kMlpStride, bits, level, fx, fy);
-            y_mlp = mlp_forward_wave(net, mlp_xs, lane);
+            y_mlp = mlp_forward_wave(net, mlp_xs, s_small, lane);
         }
         if (!active) continue;
 
@@ -621,7 +624,8 @@ void launch_kernel(bool runs, int grid, int block, size_t lds, const AutoShape& 
 // Static LDS of a walk block: Green thresholds, job offsets, job tallies.
 size_t walk_static_lds(int mode)
 {
-    return kNSample * 4 + kLdsJobs * 8 + kLdsJobs * 24 + (mode == kModeWeighted ? 64 : 8);
+    return kNSample * 4 + kLdsJobs * 8 + kLdsJobs * 24 + (mode == kModeWeighted ? 64 : 8) +
+           (mode == kModeLearned ? kMlpSmallFloats * 4 : 16);
 }
 
 // LDS of one wave: its lanes' stacks, or the learned mode's activation
