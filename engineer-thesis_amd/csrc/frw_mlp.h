@@ -75,8 +75,8 @@ struct MlpPyramid {
 // Network weights (bf16 copies of the FP32 parameters) and the safety factor.
 struct MlpNet {
     const uint16_t* W1;   // bf16 [64][72]
-    const uint16_t* W2;   // bf16 [64][64]
-    const uint16_t* W3;   // bf16 [64][64]
+    const uint16_t* W2;   // bf16 [64][64], input columns in mlp_col order (the walk copy)
+    const uint16_t* W3;   // bf16 [64][64], input columns in mlp_col order (the walk copy)
     const uint16_t* w4;   // bf16 [64]
     const float* b1;
     const float* b2;
@@ -276,14 +276,49 @@ __device__ inline void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16v8 __attribute__((ext_vector_type(8)));
+
+// Column of the activation tile that holds hidden neuron n (0..63) of a
+// layer's output: n = 32 nj + 8 g + 4 q + u is stored at 16 (2 nj + g/2) +
+// 8 q + 4 (g mod 2) + u (bits 2 and 3 of n swapped; an involution).  Lane
+// (r, q) of a 32-row block then writes exactly the columns it reads as the
+// next v_mfma_f32_32x32x16_bf16's B operand, 16 k + 8 q .. 16 k + 8 q + 7.
+// The next layer's weights are stored with their input columns in the same
+// order (the walk copy of W2 and W3).
+__host__ __device__ constexpr int mlp_col(int n) { return (n & ~12) | ((n & 4) << 1) | ((n & 8) >> 1); }
+
+// 32x32 outputs (neurons nj*32 + 8 g + 4 q + u of tile rows r) of W * buf^T
+// for one 32-row block: four v_mfma_f32_32x32x16_bf16 steps over columns
+// 0..63 (lane (r, q) holds columns 16 k + 8 q .. + 7) and, for K = 72, one
+// v_mfma_f32_32x32x8_bf16 step over columns 64..71 (lane holds 64 + 4 q .. + 3).
+template <int K>
+__device__ inline f32x16 mlp_block(const uint16_t* row, const uint16_t* wrow, int q)
+{
+    f32x16 acc = {0};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+        const bf16x8 x = *(const bf16x8*)(row + 16 * kb + 8 * q);
+        const bf16x8 w = *(const bf16x8*)(wrow + 16 * kb + 8 * q);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16v8, w), __builtin_bit_cast(bf16v8, x),
+                                                      acc, 0, 0, 0);
+    }
+    if (K == 72) {
+        const bf16x4 x = *(const bf16x4*)(row + 64 + 4 * q);
+        const bf16x4 w = *(const bf16x4*)(wrow + 64 + 4 * q);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w, x, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
 // One hidden layer for the wave's 64 rows, in place: buf = bf16(gelu(buf *
-// W^T + b)).  Computed transposed (W * buf^T, weights as the MFMA's A
-// operand): each lane then holds four consecutive output neurons of one row
-// per group, stored as one packed 8-byte write.  Lane (r, q) of a 32-row
-// block reads exactly the elements 4q + 8k .. 4q + 8k + 3 of row r, and its
-// outputs are the same elements, so it overwrites only what it alone read;
-// both 32-neuron halves read the whole block, so the first half's outputs
-// wait in the lane's staging slots until the second half has been computed.
+// W^T + b)) with the outputs in mlp_col order.  Computed transposed (W *
+// buf^T, weights as the MFMA's A operand): each lane then holds four
+// consecutive output neurons of one row per group, stored as one packed
+// 8-byte write.  Lane (r, q) of a 32-row block overwrites only columns that
+// it alone read (mlp_col); both 32-neuron halves read the whole block, so
+// the first half's outputs wait in the lane's staging slots until the second
+// half has been computed.
 template <int K>
 __device__ inline void mlp_layer(uint16_t* buf, const uint16_t* W, const float* __restrict__ bias, int lane)
 {
@@ -294,14 +329,7 @@ __device__ inline void mlp_layer(uint16_t* buf, const uint16_t* W, const float* 
         uint16_t* row = buf + (mi * 32 + r32) * kMlpStride;
 #pragma unroll 1
         for (int nj = 0; nj < 2; ++nj) {
-            const uint16_t* wrow = W + (nj * 32 + r32) * K + 4 * q;
-            f32x16 acc = {0};
-#pragma unroll 3
-            for (int kb = 0; kb < K / 8; ++kb) {
-                const bf16x4 x = *(const bf16x4*)(row + 4 * q + kb * 8);
-                const bf16x4 w = *(const bf16x4*)(wrow + kb * 8);
-                acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w, x, acc, 0, 0, 0);
-            }
+            const f32x16 acc = mlp_block<K>(row, W + (nj * 32 + r32) * K, q);
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int n0 = nj * 32 + 8 * g + 4 * q;
@@ -311,11 +339,11 @@ __device__ inline void mlp_layer(uint16_t* buf, const uint16_t* W, const float* 
                 if (nj == 0)
                     held[g] = v;
                 else
-                    *(bf16x4*)(row + n0) = v;
+                    *(bf16x4*)(row + mlp_col(n0)) = v;
             }
         }
 #pragma unroll
-        for (int g = 0; g < 4; ++g) *(bf16x4*)(row + 8 * g + 4 * q) = held[g];
+        for (int g = 0; g < 4; ++g) *(bf16x4*)(row + mlp_col(8 * g + 4 * q)) = held[g];
     }
 }
 
@@ -334,14 +362,7 @@ __device__ inline float mlp_last_layer(const uint16_t* buf, const MlpNet& m, con
         float part = 0.0f;
 #pragma unroll 1
         for (int nj = 0; nj < 2; ++nj) {
-            const uint16_t* wrow = m.W3 + (nj * 32 + r32) * kMlpHid + 4 * q;
-            f32x16 acc = {0};
-#pragma unroll 4
-            for (int kb = 0; kb < kMlpHid / 8; ++kb) {
-                const bf16x4 x = *(const bf16x4*)(row + 4 * q + kb * 8);
-                const bf16x4 w = *(const bf16x4*)(wrow + kb * 8);
-                acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w, x, acc, 0, 0, 0);
-            }
+            const f32x16 acc = mlp_block<kMlpHid>(row, m.W3 + (nj * 32 + r32) * kMlpHid, q);
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int n0 = nj * 32 + 8 * g + 4 * q;

@@ -22,7 +22,8 @@ struct frw_mlp {
     int64_t* d_occ_off = nullptr;  // device copies of the offsets
     int64_t* d_sat_off = nullptr;
     float* d_params = nullptr;     // FP32 master parameters
-    uint16_t* d_bf = nullptr;      // bf16 copy of W1 | W2 | W3 | w4
+    uint16_t* d_bf = nullptr;      // bf16 copy of W1 | W2 | W3 | w4 (training)
+    uint16_t* d_bfw = nullptr;     // bf16 W2 | W3 with input columns in mlp_col order (the walk's network)
     float* d_adam_m = nullptr;
     float* d_adam_v = nullptr;
     float* d_grad = nullptr;
@@ -337,8 +338,10 @@ __global__ void narrow_bf16(const float* __restrict__ F, uint16_t* __restrict__ 
     if (i < total) X[i] = f2bf(F[i]);
 }
 
-// bf16 copies of the weights; the padding columns of W1 stay zero.
-__global__ void to_bf16(const float* __restrict__ p, uint16_t* __restrict__ bf)
+// bf16 copies of the weights; the padding columns of W1 stay zero.  The walk
+// copy bfw holds W2 and W3 with input column n at mlp_col(n), the column of
+// the activation tile where the walk kernel's previous layer stores neuron n.
+__global__ void to_bf16(const float* __restrict__ p, uint16_t* __restrict__ bf, uint16_t* __restrict__ bfw)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < kMlpHid * kMlpInPad) {
@@ -346,8 +349,12 @@ __global__ void to_bf16(const float* __restrict__ p, uint16_t* __restrict__ bf)
         bf[kBfW1 + i] = pad ? (uint16_t)0 : f2bf(p[kOffW1 + i]);
     }
     if (i < kMlpHid * kMlpHid) {
-        bf[kBfW2 + i] = f2bf(p[kOffW2 + i]);
-        bf[kBfW3 + i] = f2bf(p[kOffW3 + i]);
+        const uint16_t w2 = f2bf(p[kOffW2 + i]), w3 = f2bf(p[kOffW3 + i]);
+        bf[kBfW2 + i] = w2;
+        bf[kBfW3 + i] = w3;
+        const int o = i / kMlpHid, n = i % kMlpHid;
+        bfw[o * kMlpHid + mlp_col(n)] = w2;
+        bfw[kMlpHid * kMlpHid + o * kMlpHid + mlp_col(n)] = w3;
     }
     if (i < kMlpHid) bf[kBfW4 + i] = f2bf(p[kOffW4 + i]);
 }
@@ -403,8 +410,8 @@ MlpNet mlp_net(const frw_mlp* m, float alpha)
 {
     MlpNet v;
     v.W1 = m->d_bf + kBfW1;
-    v.W2 = m->d_bf + kBfW2;
-    v.W3 = m->d_bf + kBfW3;
+    v.W2 = m->d_bfw;
+    v.W3 = m->d_bfw + kMlpHid * kMlpHid;
     v.w4 = m->d_bf + kBfW4;
     v.b1 = m->d_params + kOffB1;
     v.b2 = m->d_params + kOffB2;
@@ -594,12 +601,13 @@ int frw_mlp_create(const frw_tree* t, uint64_t seed, int32_t max_level, frw_mlp*
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_adam_m, kMlpParams * 4), "hipMalloc adam");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_adam_v, kMlpParams * 4), "hipMalloc adam");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_bf, kBfCount * 2), "hipMalloc bf16");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_bfw, 2 * kMlpHid * kMlpHid * 2), "hipMalloc bf16 walk copy");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_loss, 4), "hipMalloc loss");
     if (rc == FRW_OK) rc = check_hip(hipMemset(m->d_adam_m, 0, kMlpParams * 4), "memset");
     if (rc == FRW_OK) rc = check_hip(hipMemset(m->d_adam_v, 0, kMlpParams * 4), "memset");
     if (rc == FRW_OK) rc = check_hip(hipMemcpy(m->d_params, P.data(), kMlpParams * 4, hipMemcpyHostToDevice), "copy params");
     if (rc == FRW_OK) {
-        to_bf16<<<blocks(kMlpHid * kMlpInPad, 256), 256>>>(m->d_params, m->d_bf);
+        to_bf16<<<blocks(kMlpHid * kMlpInPad, 256), 256>>>(m->d_params, m->d_bf, m->d_bfw);
         rc = check_hip(hipDeviceSynchronize(), "to_bf16");
     }
     if (rc != FRW_OK) {
@@ -613,7 +621,7 @@ int frw_mlp_create(const frw_tree* t, uint64_t seed, int32_t max_level, frw_mlp*
 void frw_mlp_free(frw_mlp* m)
 {
     if (!m) return;
-    for (void* p : {(void*)m->d_occ, (void*)m->d_sat, (void*)m->d_occ_off, (void*)m->d_sat_off, (void*)m->d_params, (void*)m->d_bf, (void*)m->d_adam_m,
+    for (void* p : {(void*)m->d_occ, (void*)m->d_sat, (void*)m->d_occ_off, (void*)m->d_sat_off, (void*)m->d_params, (void*)m->d_bf, (void*)m->d_bfw, (void*)m->d_adam_m,
                     (void*)m->d_adam_v, (void*)m->d_grad, (void*)m->d_gs, (void*)m->d_X, (void*)m->d_t, (void*)m->d_Z1,
                     (void*)m->d_Z2, (void*)m->d_Z3, (void*)m->d_F, (void*)m->d_A1, (void*)m->d_A2,
                     (void*)m->d_A3, (void*)m->d_D, (void*)m->d_Xb, (void*)m->d_tb, (void*)m->d_loss,
@@ -635,7 +643,7 @@ int frw_mlp_set_params(frw_mlp* m, const float* in, int32_t n)
     if (!m || !in || n != kMlpParams) return set_error(FRW_E_INVALID, "bad parameter buffer");
     int rc = check_hip(hipMemcpy(m->d_params, in, kMlpParams * 4, hipMemcpyHostToDevice), "copy params");
     if (rc != FRW_OK) return rc;
-    to_bf16<<<blocks(kMlpHid * kMlpInPad, 256), 256>>>(m->d_params, m->d_bf);
+    to_bf16<<<blocks(kMlpHid * kMlpInPad, 256), 256>>>(m->d_params, m->d_bf, m->d_bfw);
     return check_hip(hipDeviceSynchronize(), "to_bf16");
 }
 
@@ -796,7 +804,7 @@ int frw_mlp_train(frw_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t 
             const float b2c = 1.0f - std::pow(0.999f, (float)m->adam_step);
             adam_step<<<blocks(kMlpParams, 256), 256>>>(m->d_params, m->d_grad, m->d_adam_m, m->d_adam_v,
                                                         kMlpParams, lr, b1c, b2c);
-            to_bf16<<<blocks(kMlpHid * kMlpInPad, 256), 256>>>(m->d_params, m->d_bf);
+            to_bf16<<<blocks(kMlpHid * kMlpInPad, 256), 256>>>(m->d_params, m->d_bf, m->d_bfw);
             if (((b0 / batch) & 63) == 63 || b0 + 2 * batch > n) {
                 float l = 0.f;
                 rc = check_hip(hipMemcpy(&l, m->d_loss, 4, hipMemcpyDeviceToHost), "read loss");
