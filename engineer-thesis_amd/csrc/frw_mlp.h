@@ -38,9 +38,8 @@ constexpr int kMlpIn = 67;   // features (window 64, level, position in cell x a
 constexpr int kMlpInPad = 72;   // K of the first layer, a multiple of 8
 constexpr int kMlpHid = 64;
 constexpr int kMlpStride = 72;  // LDS row stride (bf16 elements) of activations
-// a wave's activation tile: 64 rows of kMlpStride, then 16 bf16 per lane of
-// staging for the layers' first halves
-constexpr int kMlpTileElems = 64 * kMlpStride + 64 * 16;
+// a wave's activation tile: 64 rows of kMlpStride, then the 64 FP32 outputs
+constexpr int kMlpTileElems = 64 * kMlpStride + 64 * 2;
 constexpr int kMlpMaxLevel = 15;  // feature-window levels
 constexpr int kPyrMaxLevel = 15;  // bitmap / summed-area-table levels
 constexpr int kCertShift = 4;     // verification cells are at most d / 2^4
@@ -278,111 +277,95 @@ __device__ inline void wave_sync()
 
 typedef short bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16v8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// Column of the activation tile that holds hidden neuron n (0..63) of a
-// layer's output: n = 32 nj + 8 g + 4 q + u is stored at 16 (2 nj + g/2) +
-// 8 q + 4 (g mod 2) + u (bits 2 and 3 of n swapped; an involution).  Lane
-// (r, q) of a 32-row block then writes exactly the columns it reads as the
-// next v_mfma_f32_32x32x16_bf16's B operand, 16 k + 8 q .. 16 k + 8 q + 7.
-// The next layer's weights are stored with their input columns in the same
-// order (the walk copy of W2 and W3).
-__host__ __device__ constexpr int mlp_col(int n) { return (n & ~12) | ((n & 4) << 1) | ((n & 8) >> 1); }
+// The network runs on blocks of 16 rows of the wave's activation tile with
+// v_mfma_f32_16x16x32_bf16, computed transposed (W * rows^T, weights as the A
+// operand): lane l = j + 16 h (j = row in the block, h = 0..3) supplies
+// columns 32 k + 8 h .. + 7 of row j as the B operand and receives neurons
+// 16 t + 4 h + r (t = 0..3 tiles of 16 neurons, r = 0..3) of row j.
+//
+// Column of the tile that holds hidden neuron n (0..63) of a layer's output:
+// n = 16 t + 4 h + r goes to 32 (t / 2) + 8 h + 4 (t mod 2) + r, so that lane
+// (j, h) writes exactly the columns it reads as the next layer's B operand
+// (no other lane touches them, hence no barrier between layers).  The next
+// layer's weights are stored with their input columns in the same order (the
+// walk copy of W2 and W3).
+__host__ __device__ constexpr int mlp_col(int n) { return (n & 0x23) | ((n & 0x0C) << 1) | ((n & 0x10) >> 2); }
 
-// 32x32 outputs (neurons nj*32 + 8 g + 4 q + u of tile rows r) of W * buf^T
-// for one 32-row block: four v_mfma_f32_32x32x16_bf16 steps over columns
-// 0..63 (lane (r, q) holds columns 16 k + 8 q .. + 7) and, for K = 72, one
-// v_mfma_f32_32x32x8_bf16 step over columns 64..71 (lane holds 64 + 4 q .. + 3).
+// The four 16x16 output tiles (neurons 16 t + 4 h + r of row j) of W * rows^T
+// for one block: two 16x16x32 steps over columns 0..63 and, for K = 72, one
+// v_mfma_f32_16x16x16_bf16 step over columns 64..79 of which the lanes with
+// h >= 2 (columns 72..79) supply zeros.
 template <int K>
-__device__ inline f32x16 mlp_block(const uint16_t* row, const uint16_t* wrow, int q)
+__device__ inline void mlp_tiles(const uint16_t* row, const uint16_t* W, int j, int h, f32x4 acc[4])
 {
-    f32x16 acc = {0};
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb) {
-        const bf16x8 x = *(const bf16x8*)(row + 16 * kb + 8 * q);
-        const bf16x8 w = *(const bf16x8*)(wrow + 16 * kb + 8 * q);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16v8, w), __builtin_bit_cast(bf16v8, x),
-                                                      acc, 0, 0, 0);
+    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        const bf16x8 x = *(const bf16x8*)(row + 32 * kb + 8 * h);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const bf16x8 w = *(const bf16x8*)(W + (16 * t + j) * K + 32 * kb + 8 * h);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8, w),
+                                                             __builtin_bit_cast(bf16v8, x), acc[t], 0, 0, 0);
+        }
     }
     if (K == 72) {
-        const bf16x4 x = *(const bf16x4*)(row + 64 + 4 * q);
-        const bf16x4 w = *(const bf16x4*)(wrow + 64 + 4 * q);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w, x, acc, 0, 0, 0);
+        const bool in = h < 2;
+        const bf16x4 zero = {0, 0, 0, 0};
+        const bf16x4 x = in ? *(const bf16x4*)(row + 64 + 4 * h) : zero;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const bf16x4 w = in ? *(const bf16x4*)(W + (16 * t + j) * K + 64 + 4 * h) : zero;
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w, x, acc[t], 0, 0, 0);
+        }
     }
-    return acc;
 }
 
-// One hidden layer for the wave's 64 rows, in place: buf = bf16(gelu(buf *
-// W^T + b)) with the outputs in mlp_col order.  Computed transposed (W *
-// buf^T, weights as the MFMA's A operand): each lane then holds four
-// consecutive output neurons of one row per group, stored as one packed
-// 8-byte write.  Lane (r, q) of a 32-row block overwrites only columns that
-// it alone read (mlp_col); both 32-neuron halves read the whole block, so
-// the first half's outputs wait in the lane's staging slots until the second
-// half has been computed.
+// One hidden layer for a 16-row block, in place: row = bf16(gelu(row * W^T +
+// b)) with the outputs in mlp_col order.
 template <int K>
-__device__ inline void mlp_layer(uint16_t* buf, const uint16_t* W, const float* __restrict__ bias, int lane)
+__device__ inline void mlp_layer(uint16_t* block, const uint16_t* W, const float* __restrict__ bias, int lane)
 {
-    const int r32 = lane & 31, q = lane >> 5;
-    bf16x4* held = (bf16x4*)(buf + 64 * kMlpStride) + 4 * lane;
-#pragma unroll 1
-    for (int mi = 0; mi < 2; ++mi) {
-        uint16_t* row = buf + (mi * 32 + r32) * kMlpStride;
-#pragma unroll 1
-        for (int nj = 0; nj < 2; ++nj) {
-            const f32x16 acc = mlp_block<K>(row, W + (nj * 32 + r32) * K, q);
+    const int j = lane & 15, h = lane >> 4;
+    uint16_t* row = block + j * kMlpStride;
+    f32x4 acc[4];
+    mlp_tiles<K>(row, W, j, h, acc);
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const int n0 = nj * 32 + 8 * g + 4 * q;
-                const float4 b = *(const float4*)(bias + n0);  // one LDS read for four biases
-                const bf16x4 v = to_bf16x4(gelu_f(acc[4 * g] + b.x), gelu_f(acc[4 * g + 1] + b.y),
-                                           gelu_f(acc[4 * g + 2] + b.z), gelu_f(acc[4 * g + 3] + b.w));
-                if (nj == 0)
-                    held[g] = v;
-                else
-                    *(bf16x4*)(row + mlp_col(n0)) = v;
-            }
-        }
-#pragma unroll
-        for (int g = 0; g < 4; ++g) *(bf16x4*)(row + mlp_col(8 * g + 4 * q)) = held[g];
+    for (int t = 0; t < 4; ++t) {
+        const int n0 = 16 * t + 4 * h;
+        const float4 b = *(const float4*)(bias + n0);  // one LDS read for four biases
+        *(bf16x4*)(row + mlp_col(n0)) = to_bf16x4(gelu_f(acc[t][0] + b.x), gelu_f(acc[t][1] + b.y),
+                                                  gelu_f(acc[t][2] + b.z), gelu_f(acc[t][3] + b.w));
     }
 }
 
-// Third hidden layer and output neuron for the wave's 64 rows: y = w4 .
-// bf16(gelu(buf * W3^T + b3)) + b4, returned in lane `row`.  Each lane sums
-// the 32 neurons it holds of rows r and 32 + r (lane = r + 32 q); the other
-// 32 of each row are in lane r + 32 (1 - q), which sends its sum for the
-// row this lane returns.  The activations are not stored.
-__device__ inline float mlp_last_layer(const uint16_t* buf, const MlpNet& m, const float* __restrict__ small, int lane)
+// Third hidden layer and output neuron for a 16-row block: y = w4 .
+// bf16(gelu(row * W3^T + b3)) + b4 of row j, in every lane j + 16 h.  Each
+// lane sums its 16 neurons; the four lanes of a row add their sums.
+__device__ inline float mlp_last_layer(const uint16_t* block, const uint16_t* W3, const float* __restrict__ small, int lane)
 {
-    const int r32 = lane & 31, q = lane >> 5;
-    float p0 = 0.0f, p1 = 0.0f;
-#pragma unroll 1
-    for (int mi = 0; mi < 2; ++mi) {
-        const uint16_t* row = buf + (mi * 32 + r32) * kMlpStride;
-        float part = 0.0f;
-#pragma unroll 1
-        for (int nj = 0; nj < 2; ++nj) {
-            const f32x16 acc = mlp_block<kMlpHid>(row, m.W3 + (nj * 32 + r32) * kMlpHid, q);
+    const int j = lane & 15, h = lane >> 4;
+    f32x4 acc[4];
+    mlp_tiles<kMlpHid>(block + j * kMlpStride, W3, j, h, acc);
+    float part = 0.0f;
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const int n0 = nj * 32 + 8 * g + 4 * q;
-                const float4 b = *(const float4*)(small + kMlpSmallB3 + n0);
-                const float4 w = *(const float4*)(small + kMlpSmallW4 + n0);
-                const bf16x4 a = to_bf16x4(gelu_f(acc[4 * g] + b.x), gelu_f(acc[4 * g + 1] + b.y),
-                                           gelu_f(acc[4 * g + 2] + b.z), gelu_f(acc[4 * g + 3] + b.w));
-                part += w.x * bf2f((uint16_t)a[0]);
-                part += w.y * bf2f((uint16_t)a[1]);
-                part += w.z * bf2f((uint16_t)a[2]);
-                part += w.w * bf2f((uint16_t)a[3]);
-            }
-        }
-        if (mi == 0)
-            p0 = part;
-        else
-            p1 = part;
+    for (int t = 0; t < 4; ++t) {
+        const int n0 = 16 * t + 4 * h;
+        const float4 b = *(const float4*)(small + kMlpSmallB3 + n0);
+        const float4 w = *(const float4*)(small + kMlpSmallW4 + n0);
+        const bf16x4 a = to_bf16x4(gelu_f(acc[t][0] + b.x), gelu_f(acc[t][1] + b.y), gelu_f(acc[t][2] + b.z),
+                                   gelu_f(acc[t][3] + b.w));
+        part += w.x * bf2f((uint16_t)a[0]);
+        part += w.y * bf2f((uint16_t)a[1]);
+        part += w.z * bf2f((uint16_t)a[2]);
+        part += w.w * bf2f((uint16_t)a[3]);
     }
-    const float other = __shfl_xor(q ? p0 : p1, 32);
-    return (q ? p1 : p0) + other + small[kMlpSmallB4];
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    return part + small[kMlpSmallB4];
 }
 
 // Stages the network's biases, output weights and output bias in LDS
@@ -400,20 +383,34 @@ __device__ inline void mlp_stage_small(const MlpNet& m, float* small, int tid, i
     }
 }
 
-// Forward pass of the wave's 64 feature rows in xs (LDS, kMlpTileElems:
-// 64 x kMlpStride, overwritten by the hidden activations, and the staging
-// slots), with the small operands staged by mlp_stage_small.  Returns row
-// `lane`'s output.  Must be called by all 64 lanes together.
-__device__ inline float mlp_forward_wave(const MlpNet& m, uint16_t* xs, const float* small, int lane)
+// Forward pass of the first n_rows (0..64) feature rows of the wave's tile xs
+// (LDS, kMlpTileElems: 64 x kMlpStride, overwritten by the hidden
+// activations, then 64 floats of outputs), in ceil(n_rows / 16) blocks of 16
+// rows, with the small operands staged by mlp_stage_small.  Returns the
+// output of row `row` (any lane; rows at or beyond n_rows give 0).  Must be
+// called by all 64 lanes together with the same n_rows.
+__device__ inline float mlp_forward_rows(const MlpNet& m, uint16_t* xs, const float* small, int lane, int n_rows, int row)
 {
+    float* ys = (float*)(xs + 64 * kMlpStride);
+    wave_sync();  // the feature rows written by other lanes
+    const int nb = (n_rows + 15) >> 4;
+#pragma unroll 1
+    for (int b = 0; b < nb; ++b) {
+        uint16_t* block = xs + 16 * b * kMlpStride;
+        // the weights are reloaded for every block (from the caches) rather
+        // than held across the loop: holding all three layers' fragments
+        // would take about 100 registers per lane
+        const uint16_t* W1 = m.W1;
+        const uint16_t* W2 = m.W2;
+        const uint16_t* W3 = m.W3;
+        asm volatile("" : "+s"(W1), "+s"(W2), "+s"(W3));
+        mlp_layer<kMlpInPad>(block, W1, small + kMlpSmallB1, lane);
+        mlp_layer<kMlpHid>(block, W2, small + kMlpSmallB2, lane);
+        const float y = mlp_last_layer(block, W3, small, lane);
+        if (lane < 16) ys[16 * b + lane] = y;
+    }
     wave_sync();
-    mlp_layer<kMlpInPad>(xs, m.W1, small + kMlpSmallB1, lane);
-    wave_sync();
-    mlp_layer<kMlpHid>(xs, m.W2, small + kMlpSmallB2, lane);
-    wave_sync();
-    const float y = mlp_last_layer(xs, m, small, lane);
-    wave_sync();
-    return y;
+    return row < n_rows ? ys[row] : 0.0f;
 }
 
 }  // namespace frw

@@ -375,7 +375,7 @@ __global__ void gather_rows(const uint16_t* __restrict__ X, const float* __restr
 __global__ void __launch_bounds__(64) forward_waves(MlpNet m, const uint16_t* __restrict__ X, int64_t n,
                                                     float* __restrict__ y)
 {
-    __shared__ uint16_t xs[kMlpTileElems];
+    __shared__ __align__(16) uint16_t xs[kMlpTileElems];
     __shared__ __align__(16) float small[kMlpSmallFloats];
     const int lane = threadIdx.x;
     const int64_t row = (int64_t)blockIdx.x * 64 + lane;
@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(64) forward_waves(MlpNet m, const uint16_t* __
     __syncthreads();
     for (int k = 0; k < kMlpInPad; ++k)
         xs[lane * kMlpStride + k] = row < n ? X[row * kMlpInPad + k] : (uint16_t)0;
-    const float out = mlp_forward_wave(m, xs, small, lane);
+    const float out = mlp_forward_rows(m, xs, small, lane, 64, lane);
     if (row < n) y[row] = out;
 }
 

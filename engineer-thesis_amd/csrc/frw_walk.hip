@@ -427,17 +427,26 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
 
         float y_mlp = 0.0f;
         double cell = 1.0;
+        bool want = false;  // learned: this hop asks the network for a square
         if (kMode == kModeLearned) {
-            // all 64 lanes take part in the MFMA forward pass
-            uint64_t bits = 0;
-            int level = 0;
-            float fx = 0.0f, fy = 0.0f;
-            if (active && !escaped) {
-                level = mlp_level(pyr, hprev, &cell);
-                bits = mlp_window(pyr, level, cell, px, py, &fx, &fy);
+            // A proposal is asked for where the previous square was at least
+            // the finest verification cell: below it a proposal would be
+            // probed at cells as large as itself and is hardly ever
+            // certified.  The lanes that ask write their feature rows to the
+            // first rows of the wave's tile, and the network runs on those
+            // rows only, in blocks of 16 (all 64 lanes take part).
+            want = active && !escaped && hprev >= ldexp(1.0, pyr.K0 - pyr.Vmax);
+            const unsigned long long ask = __ballot(want);
+            if (ask) {
+                const int row = __popcll(ask & (lane ? (~0ull) >> (64 - lane) : 0ull));
+                if (want) {
+                    float fx = 0.0f, fy = 0.0f;
+                    const int level = mlp_level(pyr, hprev, &cell);
+                    const uint64_t bits = mlp_window(pyr, level, cell, px, py, &fx, &fy);
+                    mlp_feature_row(mlp_xs + row * kMlpStride, bits, level, fx, fy);
+                }
+                y_mlp = mlp_forward_rows(net, mlp_xs, s_small, lane, __popcll(ask), want ? row : 64);
             }
-            mlp_feature_row(mlp_xs + lane * kMlpStride, bits, level, fx, fy);
-            y_mlp = mlp_forward_wave(net, mlp_xs, s_small, lane);
         }
         if (!active) continue;
 
@@ -449,7 +458,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
             result = -1;
         } else {
             bool proposed = false;
-            if (kMode == kModeLearned) {
+            if (kMode == kModeLearned && want) {
                 // outputs beyond the training targets' range [-4, 4] are not trusted
                 const float y = fminf(fmaxf(y_mlp, -4.0f), 4.0f);
                 const double d = cell * exp2((double)y) * (double)net.alpha;
@@ -460,6 +469,8 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
                         proposed = true;
                     }
                 }
+            }
+            if (kMode == kModeLearned) {
                 if (proposed) my_acc++; else my_fb++;
             }
             if (!proposed) {
