@@ -579,7 +579,36 @@ __host__ __device__ inline DBox biggest_square(const DevTree& t, const Stack& st
     h = 1.0;
     double dist = t.dist0;
     bool max_reached = false;
-    for (int it = 0;; ++it) {
+    int it = 0;
+    if (!kLiteral) {
+        // the steps whose test h decides alone (step < 60, h outside the
+        // band), in a loop of their own: no exact test in its body, so the
+        // common case runs without the general loop's masking.  The first
+        // ambiguous step hands over to the general loop below, which takes
+        // the schedule on from there.
+        // One exit (ambiguous, or the schedule's end) and selects only: the
+        // body compiles to straight-line code.  (dist * 0.5 is dist / 2
+        // exactly.)
+#pragma unroll 1
+        for (; it < 60; ++it) {
+            const bool c = h >= h_hit;
+            const bool big = dist > MIN_DIST;
+            const bool amb = !c & !(h < h_free);
+            const bool fin = !c & max_reached & !big;
+            if (amb | fin) break;
+            // halving rule c ? big : max_reached; on a step that goes on,
+            // !c with max_reached implies big, so it is big & (c | max_reached)
+            max_reached = max_reached | c;
+            const double half = dist * 0.5;
+            dist = (big & max_reached) ? half : dist;
+            const double sd = c ? dist : -dist;
+            o.x0 += sd; o.y0 += sd; o.x1 -= sd; o.y1 -= sd;
+            h -= sd;
+        }
+        // left before step 60 on a decided step: the schedule's end
+        if (it < 60 && (h >= h_hit || h < h_free)) return o;
+    }
+    for (;; ++it) {
         step = it;
         const bool c = collides(o.x0, o.y0, o.x1, o.y1);
         if (!(c || !max_reached || dist > MIN_DIST)) break;
