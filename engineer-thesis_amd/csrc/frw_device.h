@@ -174,6 +174,21 @@ __host__ __device__ inline void raise_error(int* err, int bit)
 #endif
 }
 
+// True when the predicate holds on any active lane of the wave.  A host
+// build answers for the one lane it runs, or through FRW_HOST_ANY_LANE when
+// the includer defines it (a test harness that lets a loop run on after the
+// lane is done, as the other lanes of a wave would).
+__host__ __device__ inline bool any_lane(bool v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __ballot(v) != 0ull;
+#elif defined(FRW_HOST_ANY_LANE)
+    return FRW_HOST_ANY_LANE(v);
+#else
+    return v;
+#endif
+}
+
 // ---------------------------------------------------------------- tree view
 // Loads from the tree's arrays as global-memory loads: the arrays live in
 // device memory, and a global load (unlike the generic "flat" load the
@@ -581,32 +596,43 @@ __host__ __device__ inline DBox biggest_square(const DevTree& t, const Stack& st
     bool max_reached = false;
     int it = 0;
     if (!kLiteral) {
-        // the steps whose test h decides alone (step < 60, h outside the
-        // band), in a loop of their own: no exact test in its body, so the
-        // common case runs without the general loop's masking.  The first
-        // ambiguous step hands over to the general loop below, which takes
-        // the schedule on from there.
-        // One exit (ambiguous, or the schedule's end) and selects only: the
-        // body compiles to straight-line code.  (dist * 0.5 is dist / 2
-        // exactly.)
+        // The steps whose test h decides alone (step < 60, h outside the
+        // band) run in a loop of their own, in step with the other lanes of
+        // the wave: a lane that has stopped adds 0 (x + 0 == x: a box
+        // coordinate is never -0, which only -0 + -0 gives), the loop ends
+        // when no lane goes on, and the body is straight-line code with no
+        // per-lane exit.  A lane stops at the schedule's end (done) or at
+        // its first ambiguous step, from which the general loop below takes
+        // the schedule on.  The loop carries no boolean: max_reached is
+        // dist < dist0 (when dist0 > MIN_DIST, dist halves at the first
+        // collision and at no step before it; a smaller dist0 leaves the
+        // whole schedule to the general loop), go and done are integers.
+        // (dist * 0.5 is dist / 2 exactly.)
+        int go = dist > MIN_DIST ? 1 : 0, done = 0;
+        const double d0 = dist;
 #pragma unroll 1
-        for (; it < 60; ++it) {
+        for (int s = 0; s < 60; ++s) {
+            const bool g = go != 0;
+            const bool mr = dist < d0;
             const bool c = h >= h_hit;
             const bool big = dist > MIN_DIST;
             const bool amb = !c & !(h < h_free);
-            const bool fin = !c & max_reached & !big;
-            if (amb | fin) break;
-            // halving rule c ? big : max_reached; on a step that goes on,
-            // !c with max_reached implies big, so it is big & (c | max_reached)
-            max_reached = max_reached | c;
+            const bool fin = !c & mr & !big;
+            const bool stop = g & (amb | fin);
+            it = stop ? s : it;
+            done = stop ? (int)(fin & !amb) : done;
+            go = (g & !(amb | fin)) ? 1 : 0;
+            if (!any_lane(go != 0)) break;
+            const bool g2 = go != 0;
             const double half = dist * 0.5;
-            dist = (big & max_reached) ? half : dist;
-            const double sd = c ? dist : -dist;
+            dist = (g2 & big & (mr | c)) ? half : dist;
+            const double sd = g2 ? (c ? dist : -dist) : 0.0;
             o.x0 += sd; o.y0 += sd; o.x1 -= sd; o.y1 -= sd;
             h -= sd;
         }
-        // left before step 60 on a decided step: the schedule's end
-        if (it < 60 && (h >= h_hit || h < h_free)) return o;
+        if (go) it = 60;
+        if (done) return o;
+        max_reached = dist < d0;
     }
     for (;; ++it) {
         step = it;

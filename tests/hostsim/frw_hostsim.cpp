@@ -8,6 +8,22 @@
 #include <cstring>
 #include <vector>
 
+// Wave-wide "any lane" tests of the traversal code answer true for
+// g_pad further calls after the lane's own answer turns false, as when
+// other lanes of its wave go on (0: the lane alone).
+static int g_pad = 0, g_pad_left = 0;
+static bool host_any_lane(bool v)
+{
+    if (v) return true;
+    if (g_pad_left > 0) {
+        --g_pad_left;
+        return true;
+    }
+    g_pad_left = g_pad;
+    return false;
+}
+#define FRW_HOST_ANY_LANE(v) host_any_lane(v)
+
 #include "../../engineer-thesis_amd/csrc/frw_device.h"
 
 using namespace frw;
@@ -174,6 +190,8 @@ struct HostStack {
 extern "C" {
 
 void hs_set_stack(int runs) { g_runs = runs; }
+
+void hs_set_lane_pad(int pad) { g_pad = g_pad_left = pad; }
 
 void* hs_build(const int32_t* xyxy, int n, const int32_t* bounds, int cap)
 {

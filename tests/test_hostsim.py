@@ -38,6 +38,7 @@ def hostsim():
     L.hs_walk.argtypes = [vp, C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64,
                           C.c_uint32, vp, vp, vp, vp]
     L.hs_set_stack.argtypes = [C.c_int]
+    L.hs_set_lane_pad.argtypes = [C.c_int]
     L.hs_walk_hinted.restype = C.c_int64
     L.hs_walk_hinted.argtypes = [vp, C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64,
                                  C.c_uint32, vp, vp, vp, vp, C.c_int]
@@ -119,6 +120,26 @@ def test_hostsim_walks_replay_oracle(oracle_lib, name, master, cap, runs):
     for mode, hint in ((frw.MODE_EXACT, False), (frw.MODE_EXACT, True), (frw.MODE_BISECT, False)):
         hit, hp, tot = ht.walk(master, 1500, mode=mode, use_hint=hint)
         assert np.array_equal(hit, ohit) and np.array_equal(hp, ohp) and tot == tot_o
+
+
+@pytest.mark.parametrize("name,master", FIXTURES)
+@pytest.mark.parametrize("pad", [1, 3, 20])
+def test_hostsim_walks_lockstep_replay(oracle_lib, name, master, pad):
+    """The square replay's decided-steps loop runs in step with the other
+    lanes of its wave: here every lane that stops goes on for `pad` more
+    steps (adding 0), as beside lanes that still go; walks still replay the
+    oracle (a lane's late steps change nothing)."""
+    lay = frw.Layout.load(layout_path(name))
+    ht = HostTree(lay, 8, 1)
+    ix = O.Index(lay.rects().astype(np.float64), tuple(float(v) for v in lay.bounds()),
+                 O.REFTREE_DFS)
+    _, tot_o, ohit, ohp = ix.walk(master, 1500, per_walk=True)
+    ht.L.hs_set_lane_pad(pad)
+    try:
+        hit, hp, tot = ht.walk(master, 1500, mode=frw.MODE_EXACT, use_hint=True)
+    finally:
+        ht.L.hs_set_lane_pad(0)
+    assert np.array_equal(hit, ohit) and np.array_equal(hp, ohp) and tot == tot_o
 
 
 @pytest.mark.parametrize("name,master", FIXTURES[:4])
