@@ -406,29 +406,30 @@ __host__ __device__ inline double nearest_rect(const DevTree& t, const Stack& st
         // w = 2 best + ext2_max of 2 (p - corner) on each axis, so the
         // subtrees of the (at most 4) cells of that window at the deepest
         // level where it spans at most 2 cells per axis hold all of them
+        // (doubled coordinates are below 2^morton_bits <= 2^31: int32 cells;
+        // the clamped window edges are integer-valued doubles, so the
+        // conversion is exact)
         const double w = 2.0 * best + (double)t.ext2_max;
         const double ux = 2.0 * (px - (double)t.bounds.x), uy = 2.0 * (py - (double)t.bounds.y);
-        const int64_t kmax = ((int64_t)1 << t.morton_bits) - 1;
-        auto clampk = [&](double v) -> int64_t {
-            return v <= 0.0 ? 0 : (v >= (double)kmax ? kmax : (int64_t)v);
-        };
-        const int64_t x0 = clampk(floor(ux - w) - 1.0), x1 = clampk(ceil(ux + w) + 1.0);
-        const int64_t y0 = clampk(floor(uy - w) - 1.0), y1 = clampk(ceil(uy + w) + 1.0);
-        const int64_t span = (x1 - x0 > y1 - y0 ? x1 - x0 : y1 - y0) + 1;
-        int s = 0;
-        while (((int64_t)1 << s) < span) ++s;
-        int l = t.morton_bits - s;
-        if (l < 0) l = 0;
-        if (l > t.start_levels - 1) l = t.start_levels - 1;
+        const double kmax = (double)(((int64_t)1 << t.morton_bits) - 1);
+        auto clampk = [&](double v) -> int { return (int)fmin(fmax(v, 0.0), kmax); };
+        const int x0 = clampk(floor(ux - w) - 1.0), x1 = clampk(ceil(ux + w) + 1.0);
+        const int y0 = clampk(floor(uy - w) - 1.0), y1 = clampk(ceil(uy + w) + 1.0);
+        // the fewest bits s with 2^s cells covering the window's span
+        // max(x1 - x0, y1 - y0) + 1: the bit length of the larger difference
+        const unsigned m = (unsigned)max(x1 - x0, y1 - y0);
+        const int s = m ? 32 - __builtin_clz(m) : 0;
+        const int l = min(max(t.morton_bits - s, 0), t.start_levels - 1);
         const int shift = t.morton_bits - l;
-        const int32_t* tab = t.start_tab + start_offset(l);
+        // level l's entries start at (4^l - 1) / 3 = binary 0101...01 (l ones)
+        const int32_t* tab = t.start_tab + (l ? (0x55555555u >> (32 - 2 * l)) : 0u);
         // the window covers at most 2 x 2 cells (cx1 - cx0 <= 1, cy1 - cy0 <= 1)
         // and p's own cell is one of its corners.  The four corner entries are
         // loaded at once (a repeated corner re-reads the same entry) and kept
         // in registers: no per-lane array, which would live in scratch memory.
-        const int cx0 = (int)(x0 >> shift), cx1 = (int)(x1 >> shift);
-        const int cy0 = (int)(y0 >> shift), cy1 = (int)(y1 >> shift);
-        const bool ox = (int)(clampk(floor(ux)) >> shift) != cx0, oy = (int)(clampk(floor(uy)) >> shift) != cy0;
+        const int cx0 = x0 >> shift, cx1 = x1 >> shift;
+        const int cy0 = y0 >> shift, cy1 = y1 >> shift;
+        const bool ox = (clampk(floor(ux)) >> shift) != cx0, oy = (clampk(floor(uy)) >> shift) != cy0;
         const int32_t e0 = gload(tab + ((cy0 << l) + cx0)), e1 = gload(tab + ((cy0 << l) + cx1));
         const int32_t e2 = gload(tab + ((cy1 << l) + cx0)), e3 = gload(tab + ((cy1 << l) + cx1));
         if (cnt) cnt->tabs += (uint32_t)((cx1 != cx0 ? 2 : 1) * (cy1 != cy0 ? 2 : 1));
