@@ -26,21 +26,16 @@ struct frw_mlp {
     uint16_t* d_bfw = nullptr;     // bf16 W2 | W3 with input columns in mlp_col order (the walk's network)
     float* d_adam_m = nullptr;
     float* d_adam_v = nullptr;
-    float* d_grad = nullptr;
-    float* d_gs = nullptr;         // weight-gradient split partials, kMaxSplits x 64 x 72
+    float* d_grad = nullptr;       // summed gradient (frw_mlp_gradient)
+    uint16_t* d_bfT = nullptr;     // bf16 W2^T | W3^T, [in][out] (the backward pass)
+    float* d_part = nullptr;       // partial sums, one row of kPartStride floats per 32 batch rows
+    int64_t part_rows = 0;
+    float* d_loss = nullptr;       // loss sum, batches summed
     int64_t adam_step = 0;
     // training set
     int64_t n_samples = 0;
     uint16_t* d_X = nullptr;       // [n][72] bf16
     float* d_t = nullptr;          // [n] targets
-    // training scratch (batch-sized)
-    int64_t ws_batch = 0;
-    float *d_Z1 = nullptr, *d_Z2 = nullptr, *d_Z3 = nullptr, *d_F = nullptr;
-    float *d_dy = nullptr, *d_lossv = nullptr;
-    uint16_t *d_A1 = nullptr, *d_A2 = nullptr, *d_A3 = nullptr, *d_D = nullptr, *d_Xb = nullptr;
-    float* d_tb = nullptr;
-    float* d_loss = nullptr;
-    int32_t* d_idx = nullptr;
 };
 
 namespace frw {
@@ -51,7 +46,6 @@ constexpr int kBfW2 = kBfW1 + kMlpHid * kMlpInPad;
 constexpr int kBfW3 = kBfW2 + kMlpHid * kMlpHid;
 constexpr int kBfW4 = kBfW3 + kMlpHid * kMlpHid;
 constexpr int kBfCount = kBfW4 + kMlpHid;
-constexpr int kMaxSplits = 16;  // batch slices of the weight-gradient products
 
 // ---------------------------------------------------------------- occupancy
 // One lane per rect: sets the bits of every cell the closed rect meets, one
@@ -178,152 +172,348 @@ __global__ void features_kernel(MlpPyramid m, const double* __restrict__ pts, co
     if (cell_out) cell_out[i] = cell;
 }
 
-// ---------------------------------------------------------------- GEMM
-// C[m][n] = sum_k A(m,k) * B(k,n) with bf16 operands and FP32 accumulation;
-// A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn].  One wave per 32x32
-// tile of C; rows/columns beyond M/N are masked.  Split K: blockIdx.y takes
-// k in [y*kc, (y+1)*kc) and writes its partial product to C + y*cstride
-// (kc a multiple of 8).
-__global__ void __launch_bounds__(64) gemm_bf16(const uint16_t* __restrict__ A, int64_t sam, int64_t sak,
-                                                const uint16_t* __restrict__ B, int64_t sbk, int64_t sbn,
-                                                float* __restrict__ C, int64_t ldc, int M, int N, int K,
-                                                int kc, int64_t cstride)
+// ---------------------------------------------------------------- training
+// One optimiser step is two launches.  train_tile: one wave per 16 rows of
+// the batch runs the forward pass, the loss and the backward pass of its rows
+// on bf16 MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation) entirely in LDS
+// and registers, and writes its share of the gradient (and of the loss) to a
+// row of partial sums.  train_apply: one lane per parameter adds the rows in
+// row order, takes the Adam step and writes the bf16 weight copies.  The
+// result does not depend on scheduling: each partial is a fixed sequence of
+// operations, and the rows are added in a fixed order.
+//
+// Rounding points (those of the FP32/bf16 restatement the tests compare
+// against): inputs, activations and hidden-layer gradients are bf16, every
+// product accumulates in FP32; Z = A W^T + b, A' = bf16(gelu(Z)); y = w4 . A3
+// + b4; dy = (y - t) / B; D3 = bf16(dy w4 gelu'(Z3)); F = D W,
+// D' = bf16(F gelu'(Z)); dW = D^T A; db = column sums of D; dw4 = dy^T A3.
+//
+// MFMA 16x16xK operand layout (lane l, i = l % 16, h = l / 16): the A
+// operand holds row i, k = 8h .. 8h + 7 (K = 32; 4h .. 4h + 3 for K = 16)
+// of a matrix stored [row][k]; the B operand holds column i, the same k, of
+// a matrix stored [column][k]; the result holds rows 4h .. 4h + 3 of column
+// i.  Every product is set up so that both operands are read as rows:
+// forward Z^T = W A^T (weights [out][in], activations [row][in]); backward
+// data F^T = W^T D^T (transposed weights [in][out], gradients [row][out]);
+// weight gradients dW = D^T A over the 16 rows (v_mfma_f32_16x16x16_bf16;
+// gradients and activations transposed, [neuron][row]).  16 rows a wave
+// (64 waves for a 1024-row batch): the step is a chain of dependent layers
+// in one wave per CU, so fewer rows per wave shorten it.
+constexpr int kTrRows = 16;                 // rows per wave
+constexpr int kTrS = 72;                    // row-major LDS stride (bf16)
+constexpr int kTrT = 24;                    // transposed LDS stride (16 rows + 8)
+constexpr int kPartStride = 13120;          // floats per partial row: parameters, loss, padding
+
+static_assert(kMlpParams + 1 <= kPartStride, "partial row too short");
+
+// A 32-bit mix of Wellons' "lowbias32" integer hash (xor-shift-multiply).
+__device__ inline uint32_t mix32(uint32_t x)
 {
-    const int tiles_n = (N + 31) / 32;
-    const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
-    const int lane = threadIdx.x, r32 = lane & 31, q = lane >> 5;
-    const int m = tm * 32 + r32, nn = tn * 32 + r32;
-    const int kb = blockIdx.y * kc;
-    const int ke = min(K, kb + kc);
-    float* Cs = C + blockIdx.y * cstride;
-    f32x16 acc = {0};
-    for (int k0 = kb; k0 < ke; k0 += 8) {
-        bf16x4 a, b;
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// Sample order of an epoch: a 4-round Feistel network on 2 * half bits
+// keyed by the epoch's keys, walked until it lands below n (cycle walking,
+// Black & Rogaway 2002): a bijection of [0, n).  The walk ends, since the
+// orbit of i < n under a bijection returns to i.  half = 0: the identity.
+__device__ inline uint32_t epoch_order(uint32_t i, uint32_t n, int half, uint4 key)
+{
+    if (half == 0) return i;
+    const uint32_t mask = (1u << half) - 1u;
+    uint32_t v = i;
+    do {
+        uint32_t L = v >> half, R = v & mask;
+        const uint32_t k[4] = {key.x, key.y, key.z, key.w};
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int k = k0 + 4 * q + e;
-            a[e] = (short)(m < M ? A[m * sam + k * sak] : 0);
-            b[e] = (short)(nn < N ? B[k * sbk + nn * sbn] : 0);
+        for (int r = 0; r < 4; ++r) {
+            const uint32_t nr = L ^ (mix32(R ^ k[r]) & mask);
+            L = R;
+            R = nr;
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
-    }
+        v = (L << half) | R;
+    } while (v >= n);
+    return v;
+}
+
+__device__ inline f32x4 mfma16(const bf16x8& a, const bf16x8& b, const f32x4& c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8, a), __builtin_bit_cast(bf16v8, b), c,
+                                                  0, 0, 0);
+}
+
+// acc[t] = W[16t .. 16t + 15][:] . In[0 .. 15][:]^T over K inputs: W global
+// bf16 [64][K] (K = 72: a 16x16x16 step takes inputs 64 .. 71), In LDS bf16
+// [16][kTrS].
+template <int K>
+__device__ inline void tr_product(const uint16_t* __restrict__ W, const uint16_t* In, f32x4 acc[4], int i, int h)
+{
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int row = tm * 32 + 8 * (r >> 2) + 4 * q + (r & 3);
-        if (row < M && nn < N) Cs[row * ldc + nn] = acc[r];
+    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        const bf16x8 b = *(const bf16x8*)(In + i * kTrS + 32 * kb + 8 * h);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const bf16x8 a = *(const bf16x8*)(W + (16 * t + i) * K + 32 * kb + 8 * h);
+            acc[t] = mfma16(a, b, acc[t]);
+        }
+    }
+    if (K == 72) {
+        const bool in = h < 2;
+        const bf16x4 zero = {0, 0, 0, 0};
+        const bf16x4 b = in ? *(const bf16x4*)(In + i * kTrS + 64 + 4 * h) : zero;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const bf16x4 a = in ? *(const bf16x4*)(W + (16 * t + i) * K + 64 + 4 * h) : zero;
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
     }
 }
 
-// dst[i] += sum over the split partials src[y*n + i], y = 0..splits-1, in order.
-__global__ void add_splits(float* __restrict__ dst, const float* __restrict__ src, int n, int splits)
+// Weight gradient dW[n][k] = sum over the 16 rows of DT[n][r] AT[k][r] for
+// k < kmax (KT = k tiles of 16), into the partial row at out (row-major,
+// ld columns).
+template <int KT>
+__device__ inline void tr_wgrad(const uint16_t* DT, const uint16_t* AT, float* __restrict__ out, int ld, int kmax,
+                                int i, int h)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const bf16x4 a = *(const bf16x4*)(DT + (16 * t + i) * kTrT + 4 * h);
+#pragma unroll
+        for (int u = 0; u < KT; ++u) {
+            const bf16x4 b = *(const bf16x4*)(AT + (16 * u + i) * kTrT + 4 * h);
+            const f32x4 c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+            const int k = 16 * u + i;
+            if (k < kmax) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) out[(16 * t + 4 * h + q) * ld + k] = c[q];
+            }
+        }
+    }
+}
+
+// Stores bf16 values v of neurons n0 .. n0 + 3, row r: row-major at M and
+// transposed at MT (either may be null).
+__device__ inline void tr_store(uint16_t* M, uint16_t* MT, int r, int n0, const bf16x4& v)
+{
+    if (M) *(bf16x4*)(M + r * kTrS + n0) = v;
+    if (MT) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) MT[(n0 + q) * kTrT + r] = (uint16_t)v[q];
+    }
+}
+
+// Column sums over the 16 rows of a row-major bf16 tile, lane n < 64 sums
+// column n: returns its sum (FP32, rows in order).
+__device__ inline float tr_colsum(const uint16_t* M, int n)
+{
     float acc = 0.0f;
-    for (int y = 0; y < splits; ++y) acc += src[(int64_t)y * n + i];
-    dst[i] += acc;
+    for (int r = 0; r < kTrRows; ++r) acc += bf2f(M[r * kTrS + n]);
+    return acc;
 }
 
-// Z += bias; A = bf16(gelu(Z)).  Z and A are [B][64] / [B][64].
-__global__ void bias_gelu(float* __restrict__ Z, const float* __restrict__ bias,
-                          uint16_t* __restrict__ A, int64_t total)
+__global__ void __launch_bounds__(64) train_tile(const uint16_t* __restrict__ X, const float* __restrict__ tgt,
+                                                 uint32_t n, uint32_t b0, int half, uint4 key, int B,
+                                                 const float* __restrict__ P, const uint16_t* __restrict__ bf,
+                                                 const uint16_t* __restrict__ bfT, float* __restrict__ part)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const float z = Z[i] + bias[i & (kMlpHid - 1)];
-    Z[i] = z;
-    A[i] = f2bf(gelu_f(z));
-}
-
-// Output layer: y = w4.a3 + b4, r = y - t; per sample dy = r/B and the loss
-// term r^2/(2B); D = bf16(dy * w4 * gelu'(z3)).  The sums over samples are
-// taken by out_grads.
-__global__ void output_layer(const uint16_t* __restrict__ A3, const float* __restrict__ Z3,
-                             const uint16_t* __restrict__ w4, const float* __restrict__ b4p,
-                             const float* __restrict__ t, int B, float* __restrict__ y_out,
-                             uint16_t* __restrict__ D, float* __restrict__ dy_out,
-                             float* __restrict__ loss_out)
-{
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= B) return;
-    float y = 0.0f;
-    for (int j = 0; j < kMlpHid; ++j) y += bf2f(w4[j]) * bf2f(A3[s * kMlpHid + j]);
-    y += *b4p;
-    if (y_out) y_out[s] = y;
-    if (!t) return;
-    const float r = y - t[s];
-    const float dy = r / (float)B;
-    dy_out[s] = dy;
-    loss_out[s] = 0.5f * r * r / (float)B;
-    for (int j = 0; j < kMlpHid; ++j)
-        D[s * kMlpHid + j] = f2bf(dy * bf2f(w4[j]) * gelu_grad_f(Z3[s * kMlpHid + j]));
-}
-
-// Block sum of one value per lane (256 lanes).
-__device__ inline float block_sum256(float v, float* sh)
-{
-    sh[threadIdx.x] = v;
+    __shared__ __align__(16) uint16_t sX[kTrRows * kTrS];
+    __shared__ __align__(16) uint16_t sXT[80 * kTrT];
+    __shared__ __align__(16) uint16_t sA[3][kTrRows * kTrS];
+    __shared__ __align__(16) uint16_t sAT[2][kMlpHid * kTrT];
+    __shared__ __align__(16) uint16_t sD[kTrRows * kTrS];
+    __shared__ __align__(16) uint16_t sDT[kMlpHid * kTrT];
+    __shared__ float s_dy[kTrRows], s_loss[kTrRows];
+    __shared__ uint32_t s_idx[kTrRows];
+    const int lane = threadIdx.x, i = lane & 15, h = lane >> 4;
+    float* out = part + (int64_t)blockIdx.x * kPartStride;
+    const uint32_t row0 = b0 + (uint32_t)blockIdx.x * kTrRows;
+    if (lane < kTrRows) s_idx[lane] = epoch_order(row0 + (uint32_t)lane, n, half, key);
+    // inputs 72 .. 79 of the transposed copy are zero (the fifth k tile of dW1)
+    for (int e = lane; e < 8 * kTrT; e += 64) sXT[72 * kTrT + e] = 0;
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    // gather: 16 rows of 72 bf16 as 9 16-byte pieces each, and the targets
+    float tr = 0.0f;
+    if (lane < kTrRows) tr = tgt[s_idx[lane]];
+    for (int c = lane; c < kTrRows * 9; c += 64) {
+        const int r = c / 9, p = c % 9;
+        const bf16x8 v = *(const bf16x8*)(X + (int64_t)s_idx[r] * kMlpInPad + 8 * p);
+        *(bf16x8*)(sX + r * kTrS + 8 * p) = v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) sXT[(8 * p + e) * kTrT + r] = (uint16_t)v[e];
+    }
+    tr = __shfl(tr, i);  // row i's target in each of its four lanes
+    __syncthreads();
+    // forward; Z of each layer stays in registers for the backward pass
+    f32x4 Z[3][4], acc[4];
+    const float* bias[3] = {P + kOffB1, P + kOffB2, P + kOffB3};
+    const uint16_t* W[3] = {bf + kBfW1, bf + kBfW2, bf + kBfW3};
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        if (l == 0) tr_product<kMlpInPad>(W[0], sX, acc, i, h);
+        else tr_product<kMlpHid>(W[l], sA[l - 1], acc, i, h);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int n0 = 16 * t + 4 * h;
+            const float4 b = *(const float4*)(bias[l] + n0);
+            f32x4 z = acc[t];
+            z[0] += b.x;
+            z[1] += b.y;
+            z[2] += b.z;
+            z[3] += b.w;
+            Z[l][t] = z;
+            tr_store(sA[l], l < 2 ? sAT[l] : nullptr, i, n0,
+                     to_bf16x4(gelu_f(z[0]), gelu_f(z[1]), gelu_f(z[2]), gelu_f(z[3])));
+        }
         __syncthreads();
     }
-    return sh[0];
-}
-
-// Sums over the batch, one block per output: blocks 0..63 dw4[j] = sum dy a3[j],
-// block 64 db4 = sum dy, block 65 the loss.
-__global__ void __launch_bounds__(256) out_grads(const uint16_t* __restrict__ A3, const float* __restrict__ dy,
-                                                 const float* __restrict__ lossv, int B,
-                                                 float* __restrict__ grad, float* __restrict__ loss)
-{
-    __shared__ float sh[256];
-    const int j = blockIdx.x;
-    float acc = 0.0f;
-    for (int s = threadIdx.x; s < B; s += 256) {
-        if (j < kMlpHid) acc += dy[s] * bf2f(A3[s * kMlpHid + j]);
-        else if (j == kMlpHid) acc += dy[s];
-        else acc += lossv[s];
+    // output neuron, loss, D3 = bf16(dy w4 gelu'(Z3)); row i in lanes i + 16 h
+    float w4v[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w4v[t][q] = bf2f(bf[kBfW4 + 16 * t + 4 * h + q]);
+    {
+        float y = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const bf16x4 a = *(const bf16x4*)(sA[2] + i * kTrS + 16 * t + 4 * h);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) y += w4v[t][q] * bf2f((uint16_t)a[q]);
+        }
+        y += __shfl_xor(y, 16);
+        y += __shfl_xor(y, 32);
+        y += P[kOffB4];
+        const float res = y - tr;
+        const float dy = res / (float)B;
+        if (h == 0) {
+            s_dy[i] = dy;
+            s_loss[i] = 0.5f * res * res / (float)B;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const f32x4 z = Z[2][t];
+            tr_store(sD, sDT, i, 16 * t + 4 * h,
+                     to_bf16x4(dy * w4v[t][0] * gelu_grad_f(z[0]), dy * w4v[t][1] * gelu_grad_f(z[1]),
+                               dy * w4v[t][2] * gelu_grad_f(z[2]), dy * w4v[t][3] * gelu_grad_f(z[3])));
+        }
     }
-    const float tot = block_sum256(acc, sh);
-    if (threadIdx.x == 0) {
-        if (j < kMlpHid) grad[kOffW4 + j] += tot;
-        else if (j == kMlpHid) grad[kOffB4] += tot;
-        else *loss += tot;
+    __syncthreads();
+    // output layer gradients: dw4 = dy^T A3, db4 = sum dy, loss
+    {
+        float acc4 = 0.0f;
+        for (int r = 0; r < kTrRows; ++r) acc4 += s_dy[r] * bf2f(sA[2][r * kTrS + lane]);
+        out[kOffW4 + lane] = acc4;
+        if (lane == 0) {
+            float sdy = 0.0f, sl = 0.0f;
+            for (int r = 0; r < kTrRows; ++r) {
+                sdy += s_dy[r];
+                sl += s_loss[r];
+            }
+            out[kOffB4] = sdy;
+            out[kMlpParams] = sl;
+        }
+    }
+    // layers 3, 2, 1: dW = D^T A_prev, db = colsum D, then D_prev =
+    // bf16((D W) gelu'(Z_prev)) for the layer below
+    const int offW[3] = {kOffW1, kOffW2, kOffW3}, offB[3] = {kOffB1, kOffB2, kOffB3};
+#pragma unroll
+    for (int l = 2; l >= 0; --l) {
+        if (l == 0) tr_wgrad<5>(sDT, sXT, out + offW[0], kMlpInPad, kMlpInPad, i, h);
+        else tr_wgrad<4>(sDT, sAT[l - 1], out + offW[l], kMlpHid, kMlpHid, i, h);
+        out[offB[l] + lane] = tr_colsum(sD, lane);
+        if (l == 0) break;
+        tr_product<kMlpHid>(bfT + (l - 1) * kMlpHid * kMlpHid, sD, acc, i, h);
+        __syncthreads();  // every lane has read D and DT
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const f32x4 z = Z[l - 1][t], f = acc[t];
+            tr_store(sD, sDT, i, 16 * t + 4 * h,
+                     to_bf16x4(f[0] * gelu_grad_f(z[0]), f[1] * gelu_grad_f(z[1]), f[2] * gelu_grad_f(z[2]),
+                               f[3] * gelu_grad_f(z[3])));
+        }
+        __syncthreads();
     }
 }
 
-// D = bf16(F * gelu'(Z)) for the hidden layer below.
-__global__ void back_gelu(const float* __restrict__ F, const float* __restrict__ Z, uint16_t* __restrict__ D,
-                          int64_t total)
+// bf16 copies of parameter i (FP32 value p): W1 (padding columns zero), W2
+// and W3 row-major, in the walk's column order (mlp_col) and transposed (the
+// backward pass's copies), w4.
+__device__ inline void write_copies(int i, float p, uint16_t* __restrict__ bf, uint16_t* __restrict__ bfw,
+                                    uint16_t* __restrict__ bfT)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    D[i] = f2bf(F[i] * gelu_grad_f(Z[i]));
+    const uint16_t v = f2bf(p);
+    if (i < kOffB1) {
+        bf[kBfW1 + i] = (i % kMlpInPad) >= kMlpIn ? (uint16_t)0 : v;
+    } else if (i >= kOffW2 && i < kOffB2) {
+        const int j = i - kOffW2, o = j / kMlpHid, c = j % kMlpHid;
+        bf[kBfW2 + j] = v;
+        bfw[o * kMlpHid + mlp_col(c)] = v;
+        bfT[c * kMlpHid + o] = v;
+    } else if (i >= kOffW3 && i < kOffB3) {
+        const int j = i - kOffW3, o = j / kMlpHid, c = j % kMlpHid;
+        bf[kBfW3 + j] = v;
+        bfw[kMlpHid * kMlpHid + o * kMlpHid + mlp_col(c)] = v;
+        bfT[kMlpHid * kMlpHid + c * kMlpHid + o] = v;
+    } else if (i >= kOffW4 && i < kOffB4) {
+        bf[kBfW4 + i - kOffW4] = v;
+    }
 }
 
-// Bias gradient: column sums of D [B][64], one block per column.
-__global__ void __launch_bounds__(256) bias_grad(const uint16_t* __restrict__ D, int B, float* __restrict__ g)
-{
-    __shared__ float sh[256];
-    const int c = blockIdx.x;
-    float acc = 0.0f;
-    for (int s = threadIdx.x; s < B; s += 256) acc += bf2f(D[s * kMlpHid + c]);
-    const float tot = block_sum256(acc, sh);
-    if (threadIdx.x == 0) g[c] += tot;
-}
-
-__global__ void adam_step(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ mom,
-                          float* __restrict__ vel, int n, float lr, float b1c, float b2c)
+// Sums the G partial rows of each parameter in row order.  adam: the Adam
+// step on the FP32 parameters and their bf16 copies, the loss added to
+// loss[0] (and 1 to loss[1]) when record is set; otherwise the gradient goes
+// to grad and the loss to loss[0].
+__global__ void train_apply(const float* __restrict__ part, int G, float* __restrict__ P, float* __restrict__ mom,
+                            float* __restrict__ vel, float lr, float b1c, float b2c, int adam, int record,
+                            float* __restrict__ grad, float* __restrict__ loss, uint16_t* __restrict__ bf,
+                            uint16_t* __restrict__ bfw, uint16_t* __restrict__ bfT)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float gi = g[i];
-    const float m = 0.9f * mom[i] + 0.1f * gi;
-    const float v = 0.999f * vel[i] + 0.001f * gi * gi;
+    if (i > kMlpParams) return;
+    // the rows' loads issued 16 at a time, then added in row order
+    float g = 0.0f;
+    for (int w0 = 0; w0 < G; w0 += 16) {
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = w0 + u < G ? part[(int64_t)(w0 + u) * kPartStride + i] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (w0 + u < G) g += v[u];
+    }
+    if (i == kMlpParams) {
+        if (!adam) {
+            loss[0] = g;
+        } else if (record) {
+            loss[0] += g;
+            loss[1] += 1.0f;
+        }
+        return;
+    }
+    if (!adam) {
+        grad[i] = g;
+        return;
+    }
+    const float m = 0.9f * mom[i] + 0.1f * g;
+    const float v = 0.999f * vel[i] + 0.001f * g * g;
     mom[i] = m;
     vel[i] = v;
-    p[i] -= lr * (m / b1c) / (sqrtf(v / b2c) + 1e-8f);
+    const float p = P[i] - lr * (m / b1c) / (sqrtf(v / b2c) + 1e-8f);
+    P[i] = p;
+    write_copies(i, p, bf, bfw, bfT);
+}
+
+__global__ void copies_kernel(const float* __restrict__ P, uint16_t* __restrict__ bf, uint16_t* __restrict__ bfw,
+                              uint16_t* __restrict__ bfT)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kMlpParams) write_copies(i, P[i], bf, bfw, bfT);
 }
 
 __global__ void widen_bf16(const uint16_t* __restrict__ X, float* __restrict__ F, int64_t total)
@@ -336,38 +526,6 @@ __global__ void narrow_bf16(const float* __restrict__ F, uint16_t* __restrict__ 
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < total) X[i] = f2bf(F[i]);
-}
-
-// bf16 copies of the weights; the padding columns of W1 stay zero.  The walk
-// copy bfw holds W2 and W3 with input column n at mlp_col(n), the column of
-// the activation tile where the walk kernel's previous layer stores neuron n.
-__global__ void to_bf16(const float* __restrict__ p, uint16_t* __restrict__ bf, uint16_t* __restrict__ bfw)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < kMlpHid * kMlpInPad) {
-        const bool pad = (i % kMlpInPad) >= kMlpIn;
-        bf[kBfW1 + i] = pad ? (uint16_t)0 : f2bf(p[kOffW1 + i]);
-    }
-    if (i < kMlpHid * kMlpHid) {
-        const uint16_t w2 = f2bf(p[kOffW2 + i]), w3 = f2bf(p[kOffW3 + i]);
-        bf[kBfW2 + i] = w2;
-        bf[kBfW3 + i] = w3;
-        const int o = i / kMlpHid, n = i % kMlpHid;
-        bfw[o * kMlpHid + mlp_col(n)] = w2;
-        bfw[kMlpHid * kMlpHid + o * kMlpHid + mlp_col(n)] = w3;
-    }
-    if (i < kMlpHid) bf[kBfW4 + i] = f2bf(p[kOffW4 + i]);
-}
-
-__global__ void gather_rows(const uint16_t* __restrict__ X, const float* __restrict__ t,
-                            const int32_t* __restrict__ idx, int B, uint16_t* __restrict__ Xb,
-                            float* __restrict__ tb)
-{
-    const int s = blockIdx.x;
-    const int k = threadIdx.x;
-    const int64_t src = idx[s];
-    if (k < kMlpInPad) Xb[(int64_t)s * kMlpInPad + k] = X[src * kMlpInPad + k];
-    if (k == 0) tb[s] = t[src];
 }
 
 // Wave-level MFMA forward over rows of X (the walk kernel's inference path):
@@ -425,93 +583,18 @@ const frw_tree* mlp_tree(const frw_mlp* m) { return m->tree; }
 
 namespace {
 
-int ensure_ws(frw_mlp* m, int64_t B)
+// Partial-sum rows for batches of up to B rows (B / 32 rows of kPartStride
+// floats).
+int ensure_part(frw_mlp* m, int64_t B)
 {
-    if (m->ws_batch >= B) return FRW_OK;
-    (void)hipFree(m->d_Z1); (void)hipFree(m->d_Z2); (void)hipFree(m->d_Z3); (void)hipFree(m->d_F);
-    (void)hipFree(m->d_A1); (void)hipFree(m->d_A2); (void)hipFree(m->d_A3); (void)hipFree(m->d_D); (void)hipFree(m->d_Xb);
-    (void)hipFree(m->d_tb); (void)hipFree(m->d_idx); (void)hipFree(m->d_dy); (void)hipFree(m->d_lossv);
-    const size_t h = (size_t)B * kMlpHid;
-    int rc = FRW_OK;
-    rc = rc ? rc : check_hip(hipMalloc(&m->d_Z1, h * 4), "mlp ws");
-    rc = rc ? rc : check_hip(hipMalloc(&m->d_Z2, h * 4), "mlp ws");
-    rc = rc ? rc : check_hip(hipMalloc(&m->d_Z3, h * 4), "mlp ws");
-    rc = rc ? rc : check_hip(hipMalloc(&m->d_F, (size_t)B * kMlpInPad * 4), "mlp ws");
-    rc = rc ? rc : check_hip(hipMalloc(&m->d_A1, h * 2), "mlp ws");
-    rc = rc ? rc : check_hip(hipMalloc(&m->d_A2, h * 2), "mlp ws");
-    rc = rc ? rc : check_hip(hipMalloc(&m->d_A3, h * 2), "mlp ws");
-    rc = rc ? rc : check_hip(hipMalloc(&m->d_D, h * 2), "mlp ws");
-    rc = rc ? rc : check_hip(hipMalloc(&m->d_Xb, (size_t)B * kMlpInPad * 2), "mlp ws");
-    rc = rc ? rc : check_hip(hipMalloc(&m->d_tb, (size_t)B * 4), "mlp ws");
-    rc = rc ? rc : check_hip(hipMalloc(&m->d_idx, (size_t)B * 4), "mlp ws");
-    rc = rc ? rc : check_hip(hipMalloc(&m->d_dy, (size_t)B * 4), "mlp ws");
-    rc = rc ? rc : check_hip(hipMalloc(&m->d_lossv, (size_t)B * 4), "mlp ws");
-    if (rc == FRW_OK) m->ws_batch = B;
+    const int64_t G = B / kTrRows;
+    if (m->part_rows >= G) return FRW_OK;
+    (void)hipFree(m->d_part);
+    m->d_part = nullptr;
+    m->part_rows = 0;
+    int rc = check_hip(hipMalloc(&m->d_part, (size_t)G * kPartStride * 4), "hipMalloc partial sums");
+    if (rc == FRW_OK) m->part_rows = G;
     return rc;
-}
-
-void gemm(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk, int64_t sbn, float* C,
-          int64_t ldc, int M, int N, int K, hipStream_t s)
-{
-    const int tiles = ((M + 31) / 32) * ((N + 31) / 32);
-    gemm_bf16<<<dim3(tiles, 1), 64, 0, s>>>(A, sam, sak, B, sbk, sbn, C, ldc, M, N, K, K, 0);
-}
-
-// Weight gradient dst (M x N, row-major) += A^T-shaped product over the batch
-// (K = B rows), split over kMaxSplits slices of the batch into the scratch and
-// summed in a fixed order.
-void gemm_wgrad(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk, int64_t sbn,
-                float* dst, float* scratch, int M, int N, int K, hipStream_t s)
-{
-    int splits = std::max(1, std::min(kMaxSplits, K / 64));
-    int kc = (K + splits - 1) / splits;
-    kc = (kc + 7) / 8 * 8;
-    splits = (K + kc - 1) / kc;
-    const int tiles = ((M + 31) / 32) * ((N + 31) / 32);
-    gemm_bf16<<<dim3(tiles, splits), 64, 0, s>>>(A, sam, sak, B, sbk, sbn, scratch, N, M, N, K, kc,
-                                                 (int64_t)M * N);
-    add_splits<<<blocks((int64_t)M * N, 256), 256, 0, s>>>(dst, scratch, M * N, splits);
-}
-
-// Forward + backward of batch (Xb, tb) of size B (a multiple of 32): adds the
-// gradient into d_grad and the mean loss into d_loss.  Hidden-layer gradients
-// are rounded to bf16 before they enter the MFMA weight-gradient products.
-int gradient(frw_mlp* m, const uint16_t* Xb, const float* tb, int B, hipStream_t s)
-{
-    const uint16_t* W1 = m->d_bf + kBfW1;
-    const uint16_t* W2 = m->d_bf + kBfW2;
-    const uint16_t* W3 = m->d_bf + kBfW3;
-    const uint16_t* w4 = m->d_bf + kBfW4;
-    float* P = m->d_params;
-    float* G = m->d_grad;
-    float* GS = m->d_gs;  // split-K partials of the weight gradients
-    const int64_t h = (int64_t)B * kMlpHid;
-    const int H = kMlpHid, IN = kMlpInPad;
-    // forward: Z = A_prev * W^T + b, A = bf16(gelu(Z))
-    gemm(Xb, IN, 1, W1, 1, IN, m->d_Z1, H, B, H, IN, s);
-    bias_gelu<<<blocks(h, 256), 256, 0, s>>>(m->d_Z1, P + kOffB1, m->d_A1, h);
-    gemm(m->d_A1, H, 1, W2, 1, H, m->d_Z2, H, B, H, H, s);
-    bias_gelu<<<blocks(h, 256), 256, 0, s>>>(m->d_Z2, P + kOffB2, m->d_A2, h);
-    gemm(m->d_A2, H, 1, W3, 1, H, m->d_Z3, H, B, H, H, s);
-    bias_gelu<<<blocks(h, 256), 256, 0, s>>>(m->d_Z3, P + kOffB3, m->d_A3, h);
-    // output layer: loss, dw4, db4 and D3
-    output_layer<<<blocks(B, 256), 256, 0, s>>>(m->d_A3, m->d_Z3, w4, P + kOffB4, tb, B, nullptr, m->d_D,
-                                                m->d_dy, m->d_lossv);
-    out_grads<<<kMlpHid + 2, 256, 0, s>>>(m->d_A3, m->d_dy, m->d_lossv, B, G, m->d_loss);
-    // layer 3: dW3 = D3^T A2, db3 = colsum D3, F = D3 W3, D2 = bf16(F gelu'(Z2))
-    gemm_wgrad(m->d_D, 1, H, m->d_A2, H, 1, G + kOffW3, GS, H, H, B, s);
-    bias_grad<<<H, 256, 0, s>>>(m->d_D, B, G + kOffB3);
-    gemm(m->d_D, H, 1, W3, H, 1, m->d_F, H, B, H, H, s);
-    back_gelu<<<blocks(h, 256), 256, 0, s>>>(m->d_F, m->d_Z2, m->d_D, h);
-    // layer 2
-    gemm_wgrad(m->d_D, 1, H, m->d_A1, H, 1, G + kOffW2, GS, H, H, B, s);
-    bias_grad<<<H, 256, 0, s>>>(m->d_D, B, G + kOffB2);
-    gemm(m->d_D, H, 1, W2, H, 1, m->d_F, H, B, H, H, s);
-    back_gelu<<<blocks(h, 256), 256, 0, s>>>(m->d_F, m->d_Z1, m->d_D, h);
-    // layer 1: dW1 = D1^T X ([64][72]), db1
-    gemm_wgrad(m->d_D, 1, H, Xb, IN, 1, G + kOffW1, GS, H, IN, B, s);
-    bias_grad<<<H, 256, 0, s>>>(m->d_D, B, G + kOffB1);
-    return check_hip(hipGetLastError(), "mlp gradient");
 }
 
 }  // namespace
@@ -597,18 +680,19 @@ int frw_mlp_create(const frw_tree* t, uint64_t seed, int32_t max_level, frw_mlp*
     fill(kOffW4, 1, kMlpHid, kMlpHid, 1, 4);
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_params, kMlpParams * 4), "hipMalloc params");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_grad, kMlpParams * 4), "hipMalloc grad");
-    if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_gs, (size_t)kMaxSplits * kMlpHid * kMlpInPad * 4), "hipMalloc grad scratch");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_bfT, 2 * kMlpHid * kMlpHid * 2), "hipMalloc bf16 transposed copy");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_adam_m, kMlpParams * 4), "hipMalloc adam");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_adam_v, kMlpParams * 4), "hipMalloc adam");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_bf, kBfCount * 2), "hipMalloc bf16");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_bfw, 2 * kMlpHid * kMlpHid * 2), "hipMalloc bf16 walk copy");
-    if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_loss, 4), "hipMalloc loss");
+    if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_loss, 2 * 4), "hipMalloc loss");
     if (rc == FRW_OK) rc = check_hip(hipMemset(m->d_adam_m, 0, kMlpParams * 4), "memset");
     if (rc == FRW_OK) rc = check_hip(hipMemset(m->d_adam_v, 0, kMlpParams * 4), "memset");
     if (rc == FRW_OK) rc = check_hip(hipMemcpy(m->d_params, P.data(), kMlpParams * 4, hipMemcpyHostToDevice), "copy params");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(m->d_bf, 0, kBfCount * 2), "memset bf16");
     if (rc == FRW_OK) {
-        to_bf16<<<blocks(kMlpHid * kMlpInPad, 256), 256>>>(m->d_params, m->d_bf, m->d_bfw);
-        rc = check_hip(hipDeviceSynchronize(), "to_bf16");
+        copies_kernel<<<blocks(kMlpParams, 256), 256>>>(m->d_params, m->d_bf, m->d_bfw, m->d_bfT);
+        rc = check_hip(hipDeviceSynchronize(), "bf16 copies");
     }
     if (rc != FRW_OK) {
         frw_mlp_free(m);
@@ -621,11 +705,9 @@ int frw_mlp_create(const frw_tree* t, uint64_t seed, int32_t max_level, frw_mlp*
 void frw_mlp_free(frw_mlp* m)
 {
     if (!m) return;
-    for (void* p : {(void*)m->d_occ, (void*)m->d_sat, (void*)m->d_occ_off, (void*)m->d_sat_off, (void*)m->d_params, (void*)m->d_bf, (void*)m->d_bfw, (void*)m->d_adam_m,
-                    (void*)m->d_adam_v, (void*)m->d_grad, (void*)m->d_gs, (void*)m->d_X, (void*)m->d_t, (void*)m->d_Z1,
-                    (void*)m->d_Z2, (void*)m->d_Z3, (void*)m->d_F, (void*)m->d_A1, (void*)m->d_A2,
-                    (void*)m->d_A3, (void*)m->d_D, (void*)m->d_Xb, (void*)m->d_tb, (void*)m->d_loss,
-                    (void*)m->d_idx, (void*)m->d_dy, (void*)m->d_lossv})
+    for (void* p : {(void*)m->d_occ, (void*)m->d_sat, (void*)m->d_occ_off, (void*)m->d_sat_off, (void*)m->d_params,
+                    (void*)m->d_bf, (void*)m->d_bfw, (void*)m->d_bfT, (void*)m->d_adam_m, (void*)m->d_adam_v,
+                    (void*)m->d_grad, (void*)m->d_part, (void*)m->d_X, (void*)m->d_t, (void*)m->d_loss})
         if (p) (void)hipFree(p);
     delete m;
 }
@@ -643,8 +725,8 @@ int frw_mlp_set_params(frw_mlp* m, const float* in, int32_t n)
     if (!m || !in || n != kMlpParams) return set_error(FRW_E_INVALID, "bad parameter buffer");
     int rc = check_hip(hipMemcpy(m->d_params, in, kMlpParams * 4, hipMemcpyHostToDevice), "copy params");
     if (rc != FRW_OK) return rc;
-    to_bf16<<<blocks(kMlpHid * kMlpInPad, 256), 256>>>(m->d_params, m->d_bf, m->d_bfw);
-    return check_hip(hipDeviceSynchronize(), "to_bf16");
+    copies_kernel<<<blocks(kMlpParams, 256), 256>>>(m->d_params, m->d_bf, m->d_bfw, m->d_bfT);
+    return check_hip(hipDeviceSynchronize(), "bf16 copies");
 }
 
 int frw_mlp_occupancy(const frw_mlp* m, int32_t level, uint32_t* out_words, int64_t n_words)
@@ -709,17 +791,20 @@ int frw_mlp_forward(const frw_mlp* m, const float* X, int64_t n, float* y, void*
 
 int frw_mlp_gradient(frw_mlp* m, const float* X, const float* t, int64_t n, float* grad, float* loss)
 {
-    if (!m || !X || !t || !grad || n <= 0 || n % 32) return set_error(FRW_E_INVALID, "n must be a positive multiple of 32");
-    int rc = ensure_ws(m, n);
+    if (!m || !X || !t || !grad || n <= 0 || n % 32 || n > ((int64_t)1 << 31))
+        return set_error(FRW_E_INVALID, "n must be a positive multiple of 32, at most 2^31");
+    int rc = ensure_part(m, n);
     if (rc != FRW_OK) return rc;
     uint16_t* Xb = nullptr;
     rc = check_hip(hipMalloc(&Xb, (size_t)n * kMlpInPad * 2), "hipMalloc gradient");
     if (rc != FRW_OK) return rc;
     narrow_bf16<<<blocks(n * kMlpInPad, 256), 256>>>(X, Xb, n * kMlpInPad);
-    rc = check_hip(hipMemset(m->d_grad, 0, kMlpParams * 4), "memset grad");
-    if (rc == FRW_OK) rc = check_hip(hipMemset(m->d_loss, 0, 4), "memset loss");
-    if (rc == FRW_OK) rc = gradient(m, Xb, t, (int)n, nullptr);
-    if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "mlp gradient");
+    train_tile<<<(unsigned)(n / kTrRows), 64>>>(Xb, t, (uint32_t)n, 0u, 0, make_uint4(0, 0, 0, 0), (int)n,
+                                                m->d_params, m->d_bf, m->d_bfT, m->d_part);
+    train_apply<<<blocks(kMlpParams + 1, 256), 256>>>(m->d_part, (int)(n / kTrRows), m->d_params, m->d_adam_m,
+                                                      m->d_adam_v, 0.0f, 1.0f, 1.0f, 0, 0, m->d_grad, m->d_loss,
+                                                      m->d_bf, m->d_bfw, m->d_bfT);
+    rc = check_hip(hipDeviceSynchronize(), "mlp gradient");
     if (rc == FRW_OK) rc = check_hip(hipMemcpy(grad, m->d_grad, kMlpParams * 4, hipMemcpyDeviceToHost), "copy grad");
     if (rc == FRW_OK && loss) rc = check_hip(hipMemcpy(loss, m->d_loss, 4, hipMemcpyDeviceToHost), "copy loss");
     (void)hipFree(Xb);
@@ -773,49 +858,36 @@ int frw_mlp_train(frw_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t 
 {
     if (!m || epochs <= 0 || batch <= 0 || batch % 32 || lr <= 0) return set_error(FRW_E_INVALID, "bad training arguments");
     if (m->n_samples < batch) return set_error(FRW_E_RANGE, "fewer samples than one batch");
-    int rc = ensure_ws(m, batch);
+    if (m->n_samples > ((int64_t)1 << 31)) return set_error(FRW_E_RANGE, "more than 2^31 samples");
+    int rc = ensure_part(m, batch);
     if (rc != FRW_OK) return rc;
     const int64_t n = m->n_samples;
-    std::vector<int32_t> perm((size_t)n);
-    std::iota(perm.begin(), perm.end(), 0);
-    int32_t* d_perm = nullptr;
-    rc = check_hip(hipMalloc(&d_perm, (size_t)n * 4), "hipMalloc perm");
-    float loss_sum = 0.f;
-    int64_t loss_n = 0;
+    // the epoch order's Feistel network works on 2 * half >= log2(n) bits
+    int k = 0;
+    while (((int64_t)1 << k) < n) ++k;
+    const int half = std::max(1, (k + 1) / 2);
+    const int G = batch / kTrRows;
+    rc = check_hip(hipMemsetAsync(m->d_loss, 0, 2 * 4, nullptr), "memset loss");
     for (int ep = 0; ep < epochs && rc == FRW_OK; ++ep) {
-        // Fisher-Yates with Philox draws: a fresh order every epoch
-        for (int64_t i = n - 1; i > 0; --i) {
-            uint32_t w[4] = {(uint32_t)i, (uint32_t)ep, 0u, 0x5348464Cu};
-            philox4x32_10(w, (uint32_t)seed, (uint32_t)(seed >> 32));
-            const int64_t j = (int64_t)(((uint64_t)w[0] * (uint64_t)(i + 1)) >> 32);
-            std::swap(perm[i], perm[j]);
-        }
-        rc = check_hip(hipMemcpy(d_perm, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice), "copy perm");
-        loss_sum = 0.f;
-        loss_n = 0;
-        for (int64_t b0 = 0; b0 + batch <= n && rc == FRW_OK; b0 += batch) {
-            gather_rows<<<batch, 128>>>(m->d_X, m->d_t, d_perm + b0, batch, m->d_Xb, m->d_tb);
-            rc = check_hip(hipMemsetAsync(m->d_grad, 0, kMlpParams * 4, nullptr), "memset grad");
-            if (rc == FRW_OK) rc = check_hip(hipMemsetAsync(m->d_loss, 0, 4, nullptr), "memset loss");
-            if (rc == FRW_OK) rc = gradient(m, m->d_Xb, m->d_tb, batch, nullptr);
-            if (rc != FRW_OK) break;
+        uint32_t w[4] = {(uint32_t)ep, 0u, 0u, 0x5348464Cu};  // 'SHFL'
+        philox4x32_10(w, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const uint4 key = make_uint4(w[0], w[1], w[2], w[3]);
+        for (int64_t b0 = 0; b0 + batch <= n; b0 += batch) {
+            train_tile<<<G, 64>>>(m->d_X, m->d_t, (uint32_t)n, (uint32_t)b0, half, key, batch, m->d_params,
+                                  m->d_bf, m->d_bfT, m->d_part);
             m->adam_step++;
             const float b1c = 1.0f - std::pow(0.9f, (float)m->adam_step);
             const float b2c = 1.0f - std::pow(0.999f, (float)m->adam_step);
-            adam_step<<<blocks(kMlpParams, 256), 256>>>(m->d_params, m->d_grad, m->d_adam_m, m->d_adam_v,
-                                                        kMlpParams, lr, b1c, b2c);
-            to_bf16<<<blocks(kMlpHid * kMlpInPad, 256), 256>>>(m->d_params, m->d_bf, m->d_bfw);
-            if (((b0 / batch) & 63) == 63 || b0 + 2 * batch > n) {
-                float l = 0.f;
-                rc = check_hip(hipMemcpy(&l, m->d_loss, 4, hipMemcpyDeviceToHost), "read loss");
-                loss_sum += l;
-                loss_n++;
-            }
+            train_apply<<<blocks(kMlpParams + 1, 256), 256>>>(m->d_part, G, m->d_params, m->d_adam_m, m->d_adam_v,
+                                                              lr, b1c, b2c, 1, ep == epochs - 1 ? 1 : 0, nullptr,
+                                                              m->d_loss, m->d_bf, m->d_bfw, m->d_bfT);
         }
+        rc = check_hip(hipGetLastError(), "training step");
     }
-    if (d_perm) (void)hipFree(d_perm);
+    float l[2] = {0.0f, 0.0f};
     if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "train");
-    if (rc == FRW_OK && last_loss) *last_loss = loss_n ? loss_sum / loss_n : 0.f;
+    if (rc == FRW_OK) rc = check_hip(hipMemcpy(l, m->d_loss, 2 * 4, hipMemcpyDeviceToHost), "read loss");
+    if (rc == FRW_OK && last_loss) *last_loss = l[1] > 0.0f ? l[0] / l[1] : 0.0f;
     return rc;
 }
 

@@ -310,7 +310,13 @@ int frw_mlp_set_samples(frw_mlp* m, const double* pts, const double* h, const do
  * (features_of; NULL = m's own): one network trained on several layouts. */
 int frw_mlp_add_samples(frw_mlp* m, const frw_mlp* features_of, const double* pts, const double* h,
                         const double* dstar, int64_t n);
-/* Adam over the training set (batch a multiple of 32). */
+/* Adam over the training set (batch a multiple of 32, at most 2^31
+ * samples).  Each epoch visits the samples in an order fixed by (seed,
+ * epoch): a keyed Feistel bijection of the sample indices.  One optimiser
+ * step is two launches (forward and backward of 16-row tiles with per-tile
+ * partial gradients, then their sum in a fixed order and the Adam update),
+ * so the same samples and seed give the same parameters bit for bit.
+ * last_loss = mean batch loss of the last epoch. */
 int frw_mlp_train(frw_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* last_loss);
 /* Runs exact walks and records, for up to `capacity` hops that did not end
  * the walk, the point, the previous half-size and d* (device arrays pts
