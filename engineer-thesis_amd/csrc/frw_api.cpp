@@ -412,8 +412,8 @@ int frw::check_walk_params(const frw_tree* t, const frw_walk_params* p)
     if (!t || !p) return set_error(FRW_E_INVALID, "null argument");
     if (p->master < 0 || p->master >= t->n_rects) return set_error(FRW_E_RANGE, "master out of range");
     if (p->n_walks <= 0) return set_error(FRW_E_RANGE, "walk count must be positive");
-    if (p->first_walk < 0 || p->first_walk + p->n_walks > ((int64_t)1 << 32))
-        return set_error(FRW_E_RANGE, "walk ids must fit in 32 bits");
+    if (p->first_walk < 0 || p->first_walk + p->n_walks > kWalkIdLimit)
+        return set_error(FRW_E_RANGE, "walk ids must lie below 2^48");
     if (p->mode != FRW_MODE_EXACT && p->mode != FRW_MODE_BISECT && p->mode != FRW_MODE_REFCOMPAT)
         return set_error(FRW_E_INVALID, "unsupported walk mode");
     if (p->mode == FRW_MODE_REFCOMPAT && !t->ref)
@@ -516,7 +516,7 @@ int frw_walk_adaptive(const frw_tree* t, const frw_walk_params* p, double rel_er
     if (!t || !p || !counts_host || !r) return set_error(FRW_E_INVALID, "null argument");
     if (!(rel_err > 0.0) || !std::isfinite(rel_err)) return set_error(FRW_E_INVALID, "rel_err must be positive");
     const bool weighted = p->mode == FRW_MODE_WEIGHTED;
-    const int64_t id_space = ((int64_t)1 << 32) - p->first_walk;
+    const int64_t id_space = kWalkIdLimit - p->first_walk;
     if (max_walks <= 0 || max_walks > id_space) max_walks = id_space;
     frw_walk_params q = *p;
     if (q.n_walks == 0) q.n_walks = 65536;

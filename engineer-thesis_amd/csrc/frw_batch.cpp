@@ -64,7 +64,7 @@ int upload_jobs(frw_batch* b, int64_t* counts, int64_t* hops, hipStream_t s)
         w.hops = (unsigned long long*)(hops + j);
         w.walk_begin = begin;
         w.n_walks = (uint64_t)q.n_walks;
-        w.first_walk = (uint32_t)q.first_walk;
+        w.first_walk = (uint64_t)q.first_walk;
         w.job = q.job;
         w.pyr = b->pyrs[q.tree];
         begin += (uint64_t)q.n_walks;
@@ -134,7 +134,7 @@ int frw_batch_create(const frw_tree* const* trees, int32_t n_trees, const frw_ba
                 return set_error(FRW_E_INVALID, "all trees of a batch must live on the current device");
             }
             if (q.master < 0 || q.master >= t->n_rects || q.n_walks <= 0 || q.first_walk < 0 ||
-                q.first_walk + q.n_walks > ((int64_t)1 << 32)) {
+                q.first_walk + q.n_walks > kWalkIdLimit) {
                 delete b;
                 return set_error(FRW_E_RANGE, "job " + std::to_string(j) + ": bad master or walk range");
             }

@@ -19,8 +19,10 @@
 namespace frw {
 
 // ---------------------------------------------------------------- Philox
-// Philox4x32-10 (Salmon et al., SC'11).  Walk stream: counter =
-// (walk, job, hop/4, 'WALK'), key = seed; word hop%4 is the hop's uniform.
+// Philox4x32-10 (Salmon et al., SC'11).  Walk stream: counter = (walk id's
+// low word, job, hop/4, 'WALK' ^ walk id's high word), key = seed; word
+// hop%4 is the hop's uniform.  Walk ids go up to 2^48 (kWalkIdLimit); below
+// 2^32 the domain word is 'WALK' itself.
 __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
 {
 #pragma unroll
@@ -39,6 +41,18 @@ __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
 }
 
 constexpr uint32_t kWalkDomain = 0x57414C4Bu;  // 'WALK'
+constexpr int64_t kWalkIdLimit = (int64_t)1 << 48;  // walk ids per (seed, job)
+
+// Counter of walk id `id` in a stream whose domain word is `domain` ('WALK',
+// or 'GAUS' for weighted starts: the two stay apart for ids below 2^48,
+// since only the low 16 bits of the domain word take the id's high word).
+__host__ __device__ inline void walk_counter(uint32_t c[4], uint64_t id, uint32_t job, uint32_t word2, uint32_t domain)
+{
+    c[0] = (uint32_t)id;
+    c[1] = job;
+    c[2] = word2;
+    c[3] = domain ^ (uint32_t)(id >> 32);
+}
 
 // ---------------------------------------------------------------- sampling
 // Panel index = largest i with T[i] <= u (T[0] = 0): an 8-step branch-free

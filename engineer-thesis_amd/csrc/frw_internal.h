@@ -24,7 +24,7 @@ struct WalkJob {
     int32_t* hops_out;
     uint64_t walk_begin;           // global index of the job's first walk
     uint64_t n_walks;
-    uint32_t first_walk;           // Philox walk id of the job's first walk
+    uint64_t first_walk;           // Philox walk id of the job's first walk
     uint32_t job;                  // Philox counter word 1
     int32_t master_id;             // original index of the master (its hits are counted per lane)
     int32_t start_hint;            // sorted index of a rect at the master's centre (prologue): the

@@ -970,7 +970,7 @@ int check_learned(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p,
     if (!t || !m || !p) return set_error(FRW_E_INVALID, "bad arguments");
     if (m->tree != t) return set_error(FRW_E_INVALID, "network built for another tree");
     if (p->master < 0 || p->master >= t->n_rects || p->n_walks <= 0 || p->first_walk < 0 ||
-        p->first_walk + p->n_walks > ((int64_t)1 << 32))
+        p->first_walk + p->n_walks > kWalkIdLimit)
         return set_error(FRW_E_RANGE, "bad master or walk range");
     if (!(alpha > 0.0f)) return set_error(FRW_E_INVALID, "alpha must be positive");
     if (p->max_hops < 0 || p->max_hops > 0xFFFFFFFFll) return set_error(FRW_E_RANGE, "bad max_hops");

@@ -123,7 +123,7 @@ RefView device_view(const frw_tree* t)
 __global__ void __launch_bounds__(256) ref_walk_kernel(RefView v, const uint32_t* __restrict__ thresholds,
                                                        DBox start, const double* __restrict__ surface,
                                                        uint64_t seed, uint32_t job,
-                                                       uint32_t first_walk, int64_t n_walks, uint32_t max_hops,
+                                                       uint64_t first_walk, int64_t n_walks, uint32_t max_hops,
                                                        int32_t n_rects, unsigned long long* counts,
                                                        unsigned long long* hops, int32_t* hit_out,
                                                        int32_t* hops_out)
@@ -147,10 +147,7 @@ __global__ void __launch_bounds__(256) ref_walk_kernel(RefView v, const uint32_t
                 break;
             }
             if ((h & 3u) == 0u) {
-                rnd[0] = first_walk + (uint32_t)w;
-                rnd[1] = job;
-                rnd[2] = h >> 2;
-                rnd[3] = kWalkDomain;
+                walk_counter(rnd, first_walk + (uint64_t)w, job, h >> 2, kWalkDomain);
                 philox4x32_10(rnd, k0, k1);
             }
             double px, py;
@@ -306,7 +303,7 @@ int launch_ref_walk(const frw_tree* t, const frw_walk_params* p, const frw_walk_
     const uint32_t max_hops = (uint32_t)(p->max_hops > 0 ? p->max_hops : 10000000);
     ref_walk_kernel<<<grid, 256, 0, s>>>(device_view(t), t->d_thresholds, start, surface ? nullptr : t->d_surface,
                                          p->seed, p->job,
-                                         (uint32_t)p->first_walk, p->n_walks, max_hops, (int32_t)t->n_rects,
+                                         (uint64_t)p->first_walk, p->n_walks, max_hops, (int32_t)t->n_rects,
                                          (unsigned long long*)o->counts, (unsigned long long*)o->hops,
                                          o->hit_out, o->hops_out);
     return check_hip(hipGetLastError(), "ref_walk_kernel");

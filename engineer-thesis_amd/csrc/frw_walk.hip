@@ -342,7 +342,8 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
                 // normal n, perimeter L), the largest empty square at r inside
                 // the bounds (half-side h), a Green-CDF panel of it (its centre
                 // is the first point) and the weight -eps L (n . g_k) / (p_k 2h)
-                uint32_t c[4] = {jobs[j].first_walk + (uint32_t)w, jobs[j].job, 0u, kGausDomain};
+                uint32_t c[4];
+                walk_counter(c, jobs[j].first_walk + w, jobs[j].job, 0u, kGausDomain);
                 philox4x32_10(c, k0, k1);
                 const double gw = sq.x1 - sq.x0, gh = sq.y1 - sq.y0;
                 const double L = 2.0 * (gw + gh);
@@ -366,10 +367,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
                 start_stuck = !(hh > 0.0);
                 first_pt = false;
                 if (!start_stuck) {
-                    rnd[0] = jobs[j].first_walk + (uint32_t)w;
-                    rnd[1] = jobs[j].job;
-                    rnd[2] = 0u;
-                    rnd[3] = kWalkDomain;
+                    walk_counter(rnd, jobs[j].first_walk + w, jobs[j].job, 0u, kWalkDomain);
                     philox4x32_10(rnd, k0, k1);
                     // the first panel: from the Green CDF (probability p_k),
                     // or, with importance sampling, proportional to |n . g_k|
@@ -411,10 +409,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
                 first_pt = false;
             } else {
                 if ((hop & 3u) == 0u) {
-                    rnd[0] = jobs[j].first_walk + (uint32_t)w;
-                    rnd[1] = jobs[j].job;
-                    rnd[2] = hop >> 2;
-                    rnd[3] = kWalkDomain;
+                    walk_counter(rnd, jobs[j].first_walk + w, jobs[j].job, hop >> 2, kWalkDomain);
                     philox4x32_10(rnd, k0, k1);
                 }
                 const int panel = sample_panel(sT, rnd[hop & 3u]);
@@ -784,7 +779,7 @@ int launch_walk_in(const frw_tree* t, const frw_walk_params* p, const frw_walk_o
     jb.hops_out = o->hops_out;
     jb.walk_begin = 0;
     jb.n_walks = (uint64_t)p->n_walks;
-    jb.first_walk = (uint32_t)p->first_walk;
+    jb.first_walk = (uint64_t)p->first_walk;
     jb.job = p->job;
     jb.master = t->host_rects[p->master];
     jb.master_id = p->master;

@@ -164,7 +164,10 @@ typedef struct {
     int32_t master;      /* RECT_ID */
     int32_t mode;        /* FRW_MODE_* */
     int64_t n_walks;     /* ITER_NUM */
-    int64_t first_walk;  /* id of the first walk (Philox counter) */
+    int64_t first_walk;  /* id of the first walk (Philox counter; ids
+                          * first_walk .. first_walk + n_walks - 1 lie below
+                          * 2^48: the low word is counter word 0, the high
+                          * word is XORed into the stream's domain word) */
     uint64_t seed;       /* Philox key */
     uint32_t job;        /* Philox counter word 1 (layout/master id in a batch) */
     int32_t block;       /* threads per block, 0 = tuned default */
@@ -230,7 +233,8 @@ typedef struct {
     int32_t tree;        /* index into the trees array */
     int32_t master;      /* RECT_ID in that layout */
     int64_t n_walks;
-    int64_t first_walk;  /* Philox walk id of the job's first walk */
+    int64_t first_walk;  /* Philox walk id of the job's first walk (the
+                          * job's ids below 2^48) */
     uint32_t job;        /* Philox counter word 1 */
     int32_t reserved;
 } frw_batch_job;

@@ -56,7 +56,7 @@ typedef struct { double x0, y0, x1, y1; } orc_box;
 /* ---- RNG ---------------------------------------------------------------- */
 void     orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* One uniform u32 for hop `hop` of walk `walk` in job `job`. */
-uint32_t orc_walk_u32(uint64_t seed, uint32_t walk, uint32_t job, uint32_t hop);
+uint32_t orc_walk_u32(uint64_t seed, uint64_t walk, uint32_t job, uint32_t hop);
 
 /* ---- Green CDF -------------------------------------------------------- */
 void orc_green_cdf(double intg[ORC_NSAMPLE + 1]);
@@ -108,7 +108,7 @@ orc_box orc_gauss_from(const orc_index* ix, int master);
 #define ORC_HIT_STUCK   (-2)
 
 int orc_walk_one(const orc_index* ix, const orc_box* surface, const uint32_t T[ORC_NSAMPLE],
-                 uint64_t seed, uint32_t job, uint32_t walk, int64_t max_hops,
+                 uint64_t seed, uint32_t job, uint64_t walk, int64_t max_hops,
                  int64_t* hops);
 
 /* Runs walks first_walk .. first_walk+n_walks-1 from `master`.
@@ -136,13 +136,13 @@ float orc_mlp_backward(const float* params, const float* X, const float* t, int 
 double  orc_square_density(double s, double u, double v);
 void    orc_green_grad(double gx[ORC_NSAMPLE], double gy[ORC_NSAMPLE]);
 void    orc_panel_centre(double x0, double y0, double a, int k, double* px, double* py);
-void    orc_gauss_point(const orc_box* G, uint64_t seed, uint32_t walk, uint32_t job, double* rx, double* ry,
+void    orc_gauss_point(const orc_box* G, uint64_t seed, uint64_t walk, uint32_t job, double* rx, double* ry,
                         double* nx, double* ny, double* L);
 void    orc_importance_table(const double g[ORC_NSAMPLE], uint32_t T[ORC_NSAMPLE], double q[ORC_NSAMPLE]);
 int     orc_walk_weighted_one(const orc_index* ix, const orc_box* G, const uint32_t T[ORC_NSAMPLE],
                               const double gx[ORC_NSAMPLE], const double gy[ORC_NSAMPLE],
                               const double pw[ORC_NSAMPLE], const uint32_t* tx, const uint32_t* ty,
-                              const double* qx, const double* qy, uint64_t seed, uint32_t job, uint32_t walk,
+                              const double* qx, const double* qy, uint64_t seed, uint32_t job, uint64_t walk,
                               int64_t max_hops, double eps, int64_t* hops, double* weight);
 int64_t orc_walk_weighted_many(const orc_index* ix, int master, uint64_t seed, uint32_t job,
                                int64_t first_walk, int64_t n_walks, int64_t max_hops, double eps,

@@ -32,11 +32,13 @@ void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-/* Walk stream: counter = (walk, job, hop/4, 'WALK'), key = seed; one 32-bit
- * word per hop. */
-uint32_t orc_walk_u32(uint64_t seed, uint32_t walk, uint32_t job, uint32_t hop)
+/* Walk stream: counter = (walk low word, job, hop/4, 'WALK' ^ walk high
+ * word), key = seed; one 32-bit word per hop.  Walk ids below 2^32 leave the
+ * domain word 'WALK'; ids up to 2^48 stay apart from the 'GAUS' start words,
+ * whose high half differs. */
+uint32_t orc_walk_u32(uint64_t seed, uint64_t walk, uint32_t job, uint32_t hop)
 {
-    uint32_t ctr[4] = { walk, job, hop >> 2, 0x57414C4Bu };
+    uint32_t ctr[4] = { (uint32_t)walk, job, hop >> 2, 0x57414C4Bu ^ (uint32_t)(walk >> 32) };
     uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
     uint32_t out[4];
     orc_philox(ctr, key, out);

@@ -390,7 +390,7 @@ orc_box orc_gauss_from(const orc_index* ix, int master)
  * next square is the biggest square at p and the walk ends when p lies in a
  * rect. */
 int orc_walk_one(const orc_index* ix, const orc_box* surface, const uint32_t T[ORC_NSAMPLE],
-                 uint64_t seed, uint32_t job, uint32_t walk, int64_t max_hops, int64_t* hops)
+                 uint64_t seed, uint32_t job, uint64_t walk, int64_t max_hops, int64_t* hops)
 {
     orc_box sq = *surface;
     int64_t h = 0;
@@ -433,7 +433,7 @@ int64_t orc_walk_many(const orc_index* ix, int master, uint64_t seed, uint32_t j
 #pragma omp for schedule(dynamic, 64)
         for (int64_t i = 0; i < n_walks; ++i) {
             int64_t hops = 0;
-            int hit = orc_walk_one(ix, &surf, T, seed, job, (uint32_t)(first_walk + i),
+            int hit = orc_walk_one(ix, &surf, T, seed, job, (uint64_t)(first_walk + i),
                                    max_hops, &hops);
             int bin = hit >= 0 ? hit : (hit == ORC_HIT_ESCAPED ? ix->n : ix->n + 1);
             mine[bin]++;

@@ -128,10 +128,10 @@ void orc_panel_centre(double x0, double y0, double a, int k, double* px, double*
 
 /* Start of a weighted walk: the point on the Gaussian surface G from the
  * walk's 'GAUS' Philox word, its outward normal, and the perimeter. */
-void orc_gauss_point(const orc_box* G, uint64_t seed, uint32_t walk, uint32_t job, double* rx, double* ry,
+void orc_gauss_point(const orc_box* G, uint64_t seed, uint64_t walk, uint32_t job, double* rx, double* ry,
                      double* nx, double* ny, double* L)
 {
-    uint32_t ctr[4] = {walk, job, 0u, 0x47415553u}; /* 'GAUS' */
+    uint32_t ctr[4] = {(uint32_t)walk, job, 0u, 0x47415553u ^ (uint32_t)(walk >> 32)}; /* 'GAUS' ^ high word */
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     uint32_t out[4];
     orc_philox(ctr, key, out);
@@ -155,7 +155,7 @@ void orc_gauss_point(const orc_box* G, uint64_t seed, uint32_t walk, uint32_t jo
 int orc_walk_weighted_one(const orc_index* ix, const orc_box* G, const uint32_t T[ORC_NSAMPLE],
                           const double gx[ORC_NSAMPLE], const double gy[ORC_NSAMPLE],
                           const double pw[ORC_NSAMPLE], const uint32_t* tx, const uint32_t* ty,
-                          const double* qx, const double* qy, uint64_t seed, uint32_t job, uint32_t walk,
+                          const double* qx, const double* qy, uint64_t seed, uint32_t job, uint64_t walk,
                           int64_t max_hops, double eps, int64_t* hops, double* weight)
 {
     const orc_box B = orc_index_bounds(ix);
@@ -219,7 +219,7 @@ int64_t orc_walk_weighted_many(const orc_index* ix, int master, uint64_t seed, u
         double w = 0.0;
         const int hit = orc_walk_weighted_one(ix, &G, T, gx, gy, pw, importance ? tx : NULL, importance ? ty : NULL,
                                               importance ? qx : NULL, importance ? qy : NULL, seed, job,
-                                              (uint32_t)(first_walk + i), max_hops, eps, &h, &w);
+                                              (uint64_t)(first_walk + i), max_hops, eps, &h, &w);
         if (hit_out) hit_out[i] = hit;
         if (hops_out) hops_out[i] = (int32_t)h;
         if (w_out) w_out[i] = w;

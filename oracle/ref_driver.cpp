@@ -111,7 +111,7 @@ int64_t ref_walk_many(void* t, const double* master, uint64_t seed, uint32_t job
     RectHost start = mk(master);
     int64_t total = 0, other = 0;
     for (int64_t i = 0; i < n_walks; ++i) {
-        uint32_t walk = (uint32_t)(first_walk + i);
+        const uint64_t walk = (uint64_t)(first_walk + i);
         RectHost output;
         RectHost square = tr->creatGaussianSurfFrom(start, 1.5);
         int64_t hops = 0;

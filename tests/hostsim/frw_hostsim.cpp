@@ -359,10 +359,7 @@ static int64_t hs_walk_hinted_impl(void* h, int master, int mode, uint64_t seed,
         int4 hint_r = start_rect;
         while (result == -3) {
             if ((hop & 3u) == 0u) {
-                rnd[0] = first_walk + (uint32_t)w;
-                rnd[1] = job;
-                rnd[2] = hop >> 2;
-                rnd[3] = kWalkDomain;
+                walk_counter(rnd, (uint64_t)first_walk + (uint64_t)w, job, hop >> 2, kWalkDomain);
                 philox4x32_10(rnd, (uint32_t)seed, (uint32_t)(seed >> 32));
             }
             const int panel = sample_panel(T, rnd[hop & 3u]);

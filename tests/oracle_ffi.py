@@ -61,7 +61,7 @@ def oracle():
     L = C.CDLL(ORACLE_SO)
     L.orc_philox.argtypes = [_U32, _U32, _U32]
     L.orc_walk_u32.restype = C.c_uint32
-    L.orc_walk_u32.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
+    L.orc_walk_u32.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]
     L.orc_green_cdf.argtypes = [_D]
     L.orc_green_thresholds.argtypes = [_U32]
     L.orc_sample_index.argtypes = [_U32, C.c_uint32]
@@ -104,7 +104,7 @@ def oracle():
     L.orc_mlp_backward.restype = C.c_float
     L.orc_mlp_backward.argtypes = [_F32, _F32, _F32, C.c_int, _F32]
     L.orc_walk_one.restype = C.c_int
-    L.orc_walk_one.argtypes = [C.c_void_p, C.POINTER(Box), _U32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64,
+    L.orc_walk_one.argtypes = [C.c_void_p, C.POINTER(Box), _U32, C.c_uint64, C.c_uint32, C.c_uint64, C.c_int64,
                                _I64]
     L.orc_walk_many.restype = C.c_int64
     L.orc_walk_many.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_uint32, C.c_int64,

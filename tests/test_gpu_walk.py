@@ -57,6 +57,29 @@ def test_walk_replay_seed_job_offset():
         assert np.array_equal(hit, ohit) and np.array_equal(hp, ohp)
 
 
+def test_walk_ids_beyond_32_bits():
+    """Walk ids run to 2^48: a range across 2^32 replays the oracle walk for
+    walk (the id's high word enters the Philox domain word), walks 2^32 + k
+    are not walks k again, the top of the id range replays too, and a range
+    past 2^48 is refused before anything runs."""
+    lay, rects, bounds = load_fixture("test100.txt")
+    tree = frw.Tree(lay, leaf_cap=4)
+    df = oracle_index(rects, bounds, O.REFTREE_DFS)
+    first = (1 << 32) - 700
+    c, h, hit, hp = gpu_walks(tree, 3, 1400, frw.MODE_EXACT, seed=5, first=first)
+    oc, oh, ohit, ohp = df.walk(3, 1400, seed=5, first_walk=first, per_walk=True)
+    assert np.array_equal(hit, ohit) and np.array_equal(hp, ohp) and h == oh
+    _, _, hit0, hp0 = gpu_walks(tree, 3, 700, frw.MODE_EXACT, seed=5, first=0)
+    assert not (np.array_equal(hit[700:], hit0) and np.array_equal(hp[700:], hp0))
+    top = (1 << 48) - 300
+    c, h, hit, hp = gpu_walks(tree, 3, 300, frw.MODE_BISECT, seed=5, first=top)
+    oc, oh, ohit, ohp = df.walk(3, 300, seed=5, first_walk=top, per_walk=True)
+    assert np.array_equal(hit, ohit) and np.array_equal(hp, ohp)
+    with pytest.raises(frw.FrwError) as e:
+        gpu_walks(tree, 3, 301, frw.MODE_EXACT, seed=5, first=top)
+    assert e.value.code == frw.E_RANGE
+
+
 def test_walk_launch_shape_invariance():
     lay, rects, bounds = load_fixture("test1000.txt")
     tree = frw.Tree(lay)

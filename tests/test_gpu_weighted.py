@@ -95,7 +95,9 @@ def test_weighted_seed_job_offset_and_max_hops():
     lay, rects, bounds = load_fixture("test100.txt")
     tree = frw.Tree(lay, leaf_cap=4)
     ix = oracle_index(rects, bounds, O.REFTREE_DFS)
-    for seed, job, first, mh in [(9, 3, 12345, 0), (1, 0, 0, 2)]:
+    # (the third range crosses walk id 2^32: the ids' high word enters both
+    # the 'GAUS' start and the 'WALK' hop streams)
+    for seed, job, first, mh in [(9, 3, 12345, 0), (1, 0, 0, 2), (7, 1, (1 << 32) - 700, 0)]:
         c, h, ws, wq, w, hit, hp = gpu_weighted(tree, 3, 1500, seed=seed, first=first, job=job, max_hops=mh)
         ohit, ohp, ow = ix.walk_weighted(3, 1500, seed=seed, job=job, first_walk=first,
                                          max_hops=mh if mh else 10 ** 6)
