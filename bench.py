@@ -370,8 +370,10 @@ def bench_batched(args, frw, torch, np):
                       "steps_for_all_jobs_1pct": worst, "time_to_all_1pct_s": worst * wall / steps,
                       "fractions": per_job, "counts": c, "hops_total": total_hops}
         if mode_id == frw.MODE_LEARNED:
-            results[m]["accept_rate"] = int(st[0]) / max(1, int(st[0] + st[1]))
+            # stats = (accepted, rejected) proposals: hops that evaluated the network
+            results[m]["accept_rate"] = int(st[0]) / max(1, total_hops)
             results[m]["proposals"] = int(st[0] + st[1])
+            results[m]["proposal_accept_rate"] = int(st[0]) / max(1, int(st[0] + st[1]))
         log(f"{m}: {results[m]['walk_steps_per_s']:.3e} steps/s, {results[m]['walks_per_s']:.3e} walks/s, "
             f"{results[m]['hops_per_walk']:.2f} hops/walk, launch {results[m]['avg_launch_ms']:.2f} ms")
     head = "learned" if "learned" in results else modes[0]
@@ -407,6 +409,9 @@ def bench_batched(args, frw, torch, np):
         out["learned"] = dict(train_info, alpha=args.alpha)
         if "learned" in results:
             out["learned"]["accept_rate"] = results["learned"]["accept_rate"]
+            out["learned"]["proposal_accept_rate"] = results["learned"]["proposal_accept_rate"]
+            out["learned"]["accept_note"] = ("accept_rate: accepted proposals per hop; proposal_accept_rate: "
+                                             "accepted per network evaluation (hops that asked)")
     if args.compare:
         comp = {}
         for m in runs:
@@ -415,8 +420,10 @@ def bench_batched(args, frw, torch, np):
                                          "autotuned", "steps_for_all_jobs_1pct", "time_to_all_1pct_s")}
             if "accept_rate" in q:
                 comp[m]["accept_rate"] = q["accept_rate"]
+                comp[m]["proposal_accept_rate"] = q["proposal_accept_rate"]
         comp["network_only"]["note"] = ("learned walks whose every proposal is rejected (alpha 1e-9): the "
-                                        "network and its features run every hop, then the exact step")
+                                        "network and its features run on every hop that asks (previous square at "
+                                        "least the finest verification cell), then the exact step")
         comp["plain"] = "bisect: the reference's bisection square search, same batch, same launch form"
         for m in ("exact", "learned"):
             comp[f"{m}_vs_plain_steps_per_s"] = results[m]["walk_steps_per_s"] / results["bisect"]["walk_steps_per_s"]
@@ -653,7 +660,7 @@ def walk_mode(frw, torch, tree, mlp, mode, master, walks, steps, warmup, stream,
     n_walks = walks * steps
     return {
         "wall": wall, "launch_ms": launch_ms, "hops": total_hops, "walks": n_walks,
-        "counts": c, "accepted": int(st[0]), "fallback": int(st[1]),
+        "counts": c, "accepted": int(st[0]), "rejected": int(st[1]),
         "steps_per_s": total_hops / wall, "walks_per_s": n_walks / wall,
     }
 
@@ -821,7 +828,8 @@ def bench_single(args, frw, torch, np):
         out["refcompat"] = refcompat_run(frw, torch, lay, master, stream)
     if mode == "learned":
         out["learned"] = dict(train_info, alpha=args.alpha,
-                              accept_rate=r["accepted"] / max(1, r["accepted"] + r["fallback"]))
+                              accept_rate=r["accepted"] / max(1, r["hops"]),
+                              proposal_accept_rate=r["accepted"] / max(1, r["accepted"] + r["rejected"]))
     if args.compare:
         cw = min(walks, args.compare_walks)
         comp = {}
@@ -835,7 +843,7 @@ def bench_single(args, frw, torch, np):
                        "hops_per_walk": q["hops"] / q["walks"], "fraction_other": fm,
                        "time_to_1pct_s": setup + walks_for_1pct(fm) / q["walks_per_s"]}
             if m == "learned":
-                comp[m]["accept_rate"] = q["accepted"] / max(1, q["accepted"] + q["fallback"])
+                comp[m]["accept_rate"] = q["accepted"] / max(1, q["hops"])
             log(f"compare {m}: {q['steps_per_s']:.3e} steps/s, {q['walks_per_s']:.3e} walks/s")
         comp["walks"] = cw
         comp["plain"] = "bisect: the reference's bisection square search on the same device tree"

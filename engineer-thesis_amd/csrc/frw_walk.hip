@@ -465,7 +465,8 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
                     }
                 }
             }
-            if (kMode == kModeLearned) {
+            // proposals (hops that evaluated the network): accepted or rejected
+            if (kMode == kModeLearned && want) {
                 if (proposed) my_acc++; else my_fb++;
             }
             if (!proposed) {

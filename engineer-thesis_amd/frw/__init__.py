@@ -631,7 +631,8 @@ def walk_harvest(tree, master, n_walks, pts, h, dstar, seed=1, first_walk=0, job
 
 def walk_learned(tree, mlp, master, n_walks, counts, hops, hit_out=None, hops_out=None, seed=1,
                  first_walk=0, job=0, alpha=0.8, block=0, grid=0, max_hops=0):
-    """Walks with the learned proposal; returns (accepted, fallback) counts."""
+    """Walks with the learned proposal; returns (accepted, rejected) proposal
+    counts (their sum: network evaluations; hops - accepted: exact searches)."""
     p = WalkParams(master=master, mode=MODE_LEARNED, n_walks=n_walks, first_walk=first_walk,
                    seed=seed, job=job, block=block, grid=grid, reserved=0, max_hops=max_hops)
     o = WalkOutputs(counts.data_ptr(), hops.data_ptr(),
@@ -646,7 +647,7 @@ def walk_learned_async(tree, mlp, master, n_walks, counts, hops, stats=None, hit
                        hops_out=None, seed=1, first_walk=0, job=0, alpha=0.8, block=0, grid=0,
                        stream=None):
     """Asynchronous learned walks into device tensors; `stats` (device uint64[2]
-    or int64[2]) accumulates (accepted, fallback) when given."""
+    or int64[2]) accumulates (accepted, rejected) proposals when given."""
     p = WalkParams(master=master, mode=MODE_LEARNED, n_walks=n_walks, first_walk=first_walk,
                    seed=seed, job=job, block=block, grid=grid, reserved=0, max_hops=0)
     o = WalkOutputs(counts.data_ptr(), hops.data_ptr(),
@@ -657,7 +658,7 @@ def walk_learned_async(tree, mlp, master, n_walks, counts, hops, stats=None, hit
 
 
 def walk_learned_traffic(tree, mlp, master, n_walks, seed=1, alpha=0.8, first_walk=0, job=0):
-    """Instrumented learned walks: (node records, rect records, hops, accepted, fallback,
+    """Instrumented learned walks: (node records, rect records, hops, accepted, rejected,
     start-table entries)."""
     p = WalkParams(master=master, mode=MODE_LEARNED, n_walks=n_walks, first_walk=first_walk,
                    seed=seed, job=job, block=0, grid=0, reserved=0, max_hops=0)

@@ -329,20 +329,23 @@ int frw_mlp_train(frw_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t 
  * the same arguments give the same records in every run. */
 int frw_walk_harvest(const frw_tree* t, const frw_walk_params* p, double* pts, double* h, double* dstar,
                      int64_t capacity, int64_t* n_out);
-/* Walks with the learned proposal (FRW_MODE_LEARNED semantics): per hop the
- * network proposes alpha * s * 2^y; a probe of the occupancy pyramid that
- * certifies the square free accepts it, otherwise the exact search decides
- * the square.  stats (host, may be NULL)
- * = {accepted proposals, fallbacks}.  Synchronous. */
+/* Walks with the learned proposal (FRW_MODE_LEARNED semantics): a hop whose
+ * previous square reaches the finest verification cell asks the network,
+ * which proposes alpha * s * 2^y; a probe of the occupancy pyramid that
+ * certifies the square free accepts it, otherwise (and on every hop that
+ * does not ask) the exact search decides the square.  stats (host, may be
+ * NULL) = {accepted proposals, rejected proposals}: their sum is the number
+ * of network evaluations, and hops - accepted the number of exact searches.
+ * Synchronous. */
 int frw_walk_learned(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p,
                      const frw_walk_outputs* o, float alpha, int64_t stats[2]);
 /* Asynchronous form on `stream`: no allocation, no synchronisation; stats_dev
- * (device, 2 x uint64, may be NULL) accumulates {accepted, fallbacks}. */
+ * (device, 2 x uint64, may be NULL) accumulates {accepted, rejected}. */
 int frw_walk_learned_async(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p,
                            const frw_walk_outputs* o, float alpha, uint64_t* stats_dev, void* stream);
 /* Measurement hook: learned walks with traffic counters compiled in.
- * out = {node records read, rect records read, hops, accepted, fallbacks,
- * start-table entries read}. */
+ * out = {node records read, rect records read, hops, accepted proposals,
+ * rejected proposals, start-table entries read}. */
 int frw_walk_learned_traffic(const frw_tree* t, const frw_mlp* m, const frw_walk_params* p, float alpha,
                              int64_t out[6]);
 

@@ -133,7 +133,8 @@ def test_batch_learned_equals_single_learned():
     torch.cuda.synchronize()
     batch.check()
     ref_c, ref_h, ref_s = counts.cpu().numpy(), hops.cpu().numpy(), stats.cpu().numpy()
-    assert ref_s[0] > 0 and ref_s[0] + ref_s[1] == ref_h.sum()
+    # (accepted, rejected) proposals: at most one per hop
+    assert ref_s[0] > 0 and ref_s[0] + ref_s[1] <= ref_h.sum()
     acc_sum = 0
     for j, q in enumerate(jobs):
         t = trees[q["tree"]]

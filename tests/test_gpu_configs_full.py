@@ -147,7 +147,7 @@ def test_c4_learned_batched_conserves(c4):
         off = batch.counts_offset(j)
         nr = trees[j].info().n_rects
         assert c[off:off + nr + 2].sum() == q["n_walks"], j
-    assert st[0] + st[1] == hp.sum() and st[0] > 0
+    assert st[0] > 0 and st[0] + st[1] <= hp.sum()  # (accepted, rejected): at most one proposal a hop
 
 
 # ---------------------------------------------------------------- C5
@@ -185,10 +185,11 @@ def test_c5_learned_walks_conserve_and_match_exact(c5):
     torch.cuda.synchronize()
     c_l, c_e = cl.cpu().numpy(), ce.cpu().numpy()
     acc, fb = (int(v) for v in st.cpu().numpy())
-    print(f"C5 learned: {int(hl.item()) / n:.2f} hops/walk, {acc / (acc + fb):.3f} accepted; "
+    print(f"C5 learned: {int(hl.item()) / n:.2f} hops/walk, {acc / (acc + fb):.3f} of proposals and "
+          f"{acc / int(hl.item()):.3f} of hops accepted; "
           f"exact {int(he.item()) / n:.2f} hops/walk", flush=True)
     assert c_l.sum() == n and c_e.sum() == n
-    assert acc + fb == int(hl.item()) and acc > 0
+    assert acc > 0 and acc + fb <= int(hl.item())
     assert c_l[nr + 1] == 0  # no walk reached the hop cap
     p = _homogeneity_p(c_l, c_e)
     assert p > 1e-4, p

@@ -304,7 +304,7 @@ def test_learned_walk_distribution(setup, trained):
     c_l, hops_l, acc, fb = _learned(tree, mlp2, 0, n, seed=21)
     c_e, hops_e = _exact(tree, 0, n, seed=22)
     assert c_l.sum() == n and c_e.sum() == n
-    assert acc + fb == hops_l
+    assert acc + fb <= hops_l  # (accepted, rejected) proposals: at most one a hop
     assert acc > 0.3 * hops_l, (acc, fb)
     p = _homogeneous(c_l, c_e)
     assert p > 1e-4, p
@@ -357,7 +357,8 @@ def _walk_rates(tree, mlp2, n, alpha=0.8):
     """(walks/s, hops/walk) of bisection, exact and learned walks over the
     same walk ids, and of learned walks whose every proposal is rejected
     (alpha so small that no proposal reaches the 2-unit minimum: the
-    network and its features run every hop, then the exact step)."""
+    network and its features run on every hop that asks, then the exact
+    step)."""
     torch = torch_dev()
     nr = tree.info().n_rects
     counts = torch.zeros(nr + 2, dtype=torch.int64, device="cuda")
@@ -365,10 +366,13 @@ def _walk_rates(tree, mlp2, n, alpha=0.8):
     stream = torch.cuda.current_stream().cuda_stream
     out = {}
 
+    total = {}
+
     def run(name, launch):
         hops.zero_()
         ms = _timed_walks(launch)
-        per = int(hops.item()) / 4 / n  # warm-up + 3 timed launches
+        total[name] = int(hops.item())
+        per = total[name] / 4 / n  # warm-up + 3 timed launches
         out[name] = (n / (ms * 1e-3), per)
 
     run("bisect", lambda: tree.walk_async(0, n, counts, hops, seed=41, mode=frw.MODE_BISECT, stream=stream))
@@ -377,7 +381,9 @@ def _walk_rates(tree, mlp2, n, alpha=0.8):
     run("learned", lambda: frw.walk_learned_async(tree, mlp2, 0, n, counts, hops, stats, seed=41, alpha=alpha,
                                                   stream=stream))
     st = stats.cpu().numpy()
-    out["accept_rate"] = int(st[0]) / max(1, int(st[0] + st[1]))
+    # accepted proposals per hop (the stats count the hops that asked the
+    # network: accepted + rejected), over the same four launches as the hops
+    out["accept_rate"] = int(st[0]) / max(1, total["learned"])
     run("network_only", lambda: frw.walk_learned_async(tree, mlp2, 0, n, counts, hops, seed=41, alpha=1e-9,
                                                        stream=stream))
     for k, v in out.items():
@@ -438,7 +444,7 @@ def test_learned_alpha_sweep(setup, trained):
         b = 32 * tl[0] + 16 * tl[1] + 16 * (tl[3] + tl[4]) + 4 * tl[5]
         print(f"alpha {alpha}: hops/walk {tl[2] / 20_000:.2f} accept {tl[3] / max(1, tl[2]):.3f} "
               f"bytes/walk {b / 20_000:.0f}")
-        assert tl[3] + tl[4] == tl[2]
+        assert tl[3] + tl[4] <= tl[2]  # proposals: at most one a hop
 
 
 def test_untrained_network_falls_back(setup):
@@ -450,7 +456,7 @@ def test_untrained_network_falls_back(setup):
     n = 100_000
     c_l, hops_l, acc, fb = _learned(tree, mlp, 3, n, seed=31, alpha=1.0)
     c_e, hops_e = _exact(tree, 3, n, seed=32)
-    assert acc + fb == hops_l
+    assert acc + fb <= hops_l
     assert _homogeneous(c_l, c_e) > 1e-4
 
 
