@@ -6,8 +6,8 @@
 //      quadtree cell holding its centre at every level (loose quadtree).  The
 //      key layout (code << 6 | level) also admits rects that stop at an inner
 //      cell; with centre keys every rect sits at the deepest level.
-//   2. one device radix sort of (key, rect id).  Sorted by key, the rects of
-//      any cell form one contiguous run.
+//   2. one device radix sort of (key, rect id) (stable LSD radix sort, this
+//      file).  Sorted by key, the rects of any cell form one contiguous run.
 //   3. level by level: every frontier node splits its run into its four
 //      quadrant runs by binary search over the sorted keys; a node becomes a
 //      leaf when its run holds <= leaf_cap rects (or it is a unit cell).
@@ -16,8 +16,6 @@
 //   4. each node's box = tight bounding box of its run (block reduction); the
 //      boxes of siblings may overlap.
 // Query answers depend only on the rect set, never on this structure.
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 #include <climits>
 #include <cstdio>
@@ -222,6 +220,203 @@ struct DevBuf {
 
 inline int blocks_for(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
+// ---------------------------------------------------------------- scan
+// Exclusive prefix sums of uint32 arrays, in three launches: per-tile sums
+// (tiles of 256 threads x 16 items), one block scanning the tile sums, and
+// each tile's scan offset by its tile's sum.  Integer sums: the result does
+// not depend on the order of the additions.
+constexpr int kScanItems = 16;
+constexpr int kScanTile = 256 * kScanItems;
+
+// Exclusive scan of one value per thread over a 256-thread block; *total
+// receives the block's sum.  sh: 4 words of LDS.
+__device__ inline uint32_t block_excl_scan256(uint32_t v, uint32_t* sh, uint32_t* total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(inc, off);
+        if (lane >= off) inc += o;
+    }
+    if (lane == 63) sh[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int w = 0; w < wave; ++w) before += sh[w];
+    *total = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return before + inc - v;
+}
+
+__global__ void __launch_bounds__(256) scan_tile_sums(const uint32_t* __restrict__ in, int64_t m,
+                                                      uint32_t* __restrict__ sums)
+{
+    __shared__ uint32_t sh[4];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < m) s += in[base + k];
+    uint32_t tot;
+    (void)block_excl_scan256(s, sh, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// Exclusive scan in place of the nb tile sums, one 256-thread block in
+// chunks of 256 with a running carry.
+__global__ void __launch_bounds__(256) scan_sums(uint32_t* __restrict__ sums, int nb)
+{
+    __shared__ uint32_t sh[4];
+    uint32_t carry = 0;
+    for (int c0 = 0; c0 < nb; c0 += 256) {
+        const int i = c0 + (int)threadIdx.x;
+        const uint32_t v = i < nb ? sums[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan256(v, sh, &tot);
+        if (i < nb) sums[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ void __launch_bounds__(256) scan_apply(const uint32_t* __restrict__ in, int64_t m,
+                                                  const uint32_t* __restrict__ sums, uint32_t* __restrict__ out)
+{
+    __shared__ uint32_t sh[4];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = base + k < m ? in[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = sums[blockIdx.x] + block_excl_scan256(s, sh, &tot);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < m) out[base + k] = run;
+        run += v[k];
+    }
+}
+
+// out[i] = sum of in[j] for j < i (out may alias in); sums: scratch of
+// scan_tile_count(m) words.
+inline int64_t scan_tile_count(int64_t m) { return (m + kScanTile - 1) / kScanTile; }
+
+int exclusive_scan(const uint32_t* in, uint32_t* out, int64_t m, uint32_t* sums, hipStream_t s)
+{
+    if (m <= 0) return FRW_OK;
+    const int nb = (int)scan_tile_count(m);
+    scan_tile_sums<<<nb, 256, 0, s>>>(in, m, sums);
+    scan_sums<<<1, 256, 0, s>>>(sums, nb);
+    scan_apply<<<nb, 256, 0, s>>>(in, m, sums, out);
+    return check_hip(hipGetLastError(), "exclusive scan");
+}
+
+// ---------------------------------------------------------------- sort
+// Stable LSD radix sort of (uint64 key, int32 value) pairs on key bits
+// [lo, hi), 8 bits a pass.  A pass: per-tile digit histograms (tiles of
+// 256 threads x 16 items, LDS counters), the exclusive scan of the
+// histograms laid out digit-major (so a tile's items of digit d land after
+// every earlier tile's), and a scatter that ranks the items of each digit in
+// input order: a tile goes in 16 rounds of 256 consecutive items; in a round
+// each wave finds the lanes holding its digit by 8 ballots, the waves' counts
+// per digit go through LDS, and the per-digit running offsets advance after
+// the round.  Equal keys keep their input order.
+constexpr int kSortItems = 16;
+constexpr int kSortTile = 256 * kSortItems;
+
+__device__ inline uint32_t sort_digit(uint64_t k, int shift) { return (uint32_t)(k >> shift) & 255u; }
+
+__global__ void __launch_bounds__(256) sort_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                                                 uint32_t* __restrict__ hist, int nb)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const int64_t tile = (int64_t)blockIdx.x * kSortTile;
+#pragma unroll 4
+    for (int j = 0; j < kSortItems; ++j) {
+        const int64_t i = tile + (int64_t)j * 256 + threadIdx.x;
+        if (i < n) atomicAdd(&h[sort_digit(keys[i], shift)], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(256) sort_scatter(const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin,
+                                                    uint64_t* __restrict__ kout, int32_t* __restrict__ vout,
+                                                    int64_t n, int shift, const uint32_t* __restrict__ offs, int nb)
+{
+    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_cnt[4][256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    s_base[tid] = offs[(int64_t)tid * nb + blockIdx.x];
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const int64_t tile = (int64_t)blockIdx.x * kSortTile;
+    for (int j = 0; j < kSortItems; ++j) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) s_cnt[w][tid] = 0u;
+        __syncthreads();
+        const int64_t i = tile + (int64_t)j * 256 + tid;
+        const bool valid = i < n;
+        const uint64_t key = valid ? kin[i] : 0ull;
+        const uint32_t d = sort_digit(key, shift);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t ones = __ballot(((d >> b) & 1u) != 0u);
+            peers &= ((d >> b) & 1u) ? ones : ~ones;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & below);
+        if (valid && rank == 0u) s_cnt[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = s_base[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += s_cnt[w][d];
+            if ((int64_t)pos < n) {  // always, for a consistent histogram
+                kout[pos] = key;
+                vout[pos] = vin[i];
+            }
+        }
+        __syncthreads();
+        s_base[tid] += s_cnt[0][tid] + s_cnt[1][tid] + s_cnt[2][tid] + s_cnt[3][tid];
+        __syncthreads();
+    }
+}
+
+// Sorts (ka, va) by key bits [lo, hi) into (kb, vb); ka and va serve as the
+// other half of the passes' ping-pong and are clobbered.  hist: 256 x
+// (tiles) words, sums: scan scratch for them.
+int sort_pairs(uint64_t* ka, int32_t* va, uint64_t* kb, int32_t* vb, int64_t n, int lo, int hi, uint32_t* hist,
+               uint32_t* sums, hipStream_t s)
+{
+    if (n <= 0) return FRW_OK;
+    const int nb = (int)((n + kSortTile - 1) / kSortTile);
+    const int passes = (hi - lo + 7) / 8;
+    uint64_t* kin = ka;
+    int32_t* vin = va;
+    uint64_t* kout = kb;
+    int32_t* vout = vb;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = lo + 8 * p;
+        sort_hist<<<nb, 256, 0, s>>>(kin, n, shift, hist, nb);
+        int rc = exclusive_scan(hist, hist, (int64_t)256 * nb, sums, s);
+        if (rc != FRW_OK) return rc;
+        sort_scatter<<<nb, 256, 0, s>>>(kin, vin, kout, vout, n, shift, hist, nb);
+        rc = check_hip(hipGetLastError(), "radix sort pass");
+        if (rc != FRW_OK) return rc;
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    if (kin != kb) {  // an even pass count ends in (ka, va)
+        int rc = check_hip(hipMemcpyAsync(kb, kin, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s), "sort copy");
+        if (rc == FRW_OK) rc = check_hip(hipMemcpyAsync(vb, vin, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToDevice, s), "sort copy");
+        return rc;
+    }
+    return FRW_OK;
+}
+
 // One thread per start-table entry (all levels): the node of its cell.
 __global__ void start_table(const Node* __restrict__ nodes, int levels, int64_t entries, int32_t* __restrict__ tab)
 {
@@ -258,8 +453,7 @@ int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 B, int leaf_cap
     int4* d_in = nullptr;
     uint64_t *d_keys = nullptr, *d_keys_sorted = nullptr;
     int32_t* d_vals = nullptr;
-    void* d_tmp = nullptr;
-    size_t tmp_bytes = 0;
+    DevBuf<uint32_t> hist, scan_sums;
     DevBuf<Frontier> fa, fb;
     DevBuf<int32_t> internal, group, own_end;
     DevBuf<int2> runs;
@@ -288,12 +482,12 @@ int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 B, int leaf_cap
     if (n > 0) {
         rect_keys<<<blocks_for(n, 256), 256, 0, s>>>(d_in, n, B, K, d_keys, d_vals);
         FRW_TRY(hipGetLastError());
-        const int end_bit = 2 * K + 6;
-        FRW_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, d_keys, d_keys_sorted, d_vals,
-                                                   t->d_ids, n, 0, end_bit, s));
-        FRW_TRY(hipMalloc(&d_tmp, std::max<size_t>(tmp_bytes, 16)));
-        FRW_TRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp_bytes, d_keys, d_keys_sorted, d_vals,
-                                                   t->d_ids, n, 0, end_bit, s));
+        // key bits 6 .. 2K + 6: the Morton code (the low 6 bits hold the
+        // same level tag in every key)
+        const int64_t nb = (n + kSortTile - 1) / kSortTile;
+        FRW_TRYRC(hist.reserve((size_t)(256 * nb), s));
+        FRW_TRYRC(scan_sums.reserve((size_t)scan_tile_count(256 * nb), s));
+        FRW_TRYRC(sort_pairs(d_keys, d_vals, d_keys_sorted, t->d_ids, n, 6, 2 * K + 6, hist.p, scan_sums.p, s));
         gather_rects<<<blocks_for(n, 256), 256, 0, s>>>(d_in, t->d_ids, n, t->d_rects);
         FRW_TRY(hipGetLastError());
     }
@@ -314,18 +508,9 @@ int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 B, int leaf_cap
         split_level<<<blocks_for(nf, 256), 256, 0, s>>>(d_keys_sorted, fa.p, nf, K, leaf_cap,
                                                         nodes.p, runs.p, internal.p, own_end.p);
         FRW_TRY(hipGetLastError());
-        size_t need = 0;
-        FRW_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, need, internal.p, group.p, nf, s));
-        if (need > tmp_bytes || !d_tmp) {
-            if (d_tmp) {
-                FRW_TRY(hipStreamSynchronize(s));
-                (void)hipFree(d_tmp);
-                d_tmp = nullptr;
-            }
-            tmp_bytes = std::max<size_t>(need, 16);
-            FRW_TRY(hipMalloc(&d_tmp, tmp_bytes));
-        }
-        FRW_TRY(hipcub::DeviceScan::ExclusiveSum(d_tmp, need, internal.p, group.p, nf, s));
+        FRW_TRYRC(scan_sums.reserve((size_t)scan_tile_count(nf), s));
+        FRW_TRYRC(exclusive_scan(reinterpret_cast<const uint32_t*>(internal.p), reinterpret_cast<uint32_t*>(group.p),
+                                 nf, scan_sums.p, s));
         int last_g = 0, last_i = 0;
         FRW_TRY(hipMemcpyAsync(&last_g, group.p + nf - 1, sizeof(int), hipMemcpyDeviceToHost, s));
         FRW_TRY(hipMemcpyAsync(&last_i, internal.p + nf - 1, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -376,7 +561,8 @@ done:
     if (d_keys) (void)hipFree(d_keys);
     if (d_keys_sorted) (void)hipFree(d_keys_sorted);
     if (d_vals) (void)hipFree(d_vals);
-    if (d_tmp) (void)hipFree(d_tmp);
+    hist.release();
+    scan_sums.release();
     fa.release();
     fb.release();
     internal.release();
