@@ -40,6 +40,10 @@ BF16_PEAK_TFLOPS = 2500.0
 ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 # network FLOP per proposal: 2 (67*64 + 64*64 + 64*64 + 64)
 FLOP_PER_PROPOSAL = 2 * (67 * 64 + 64 * 64 + 64 * 64 + 64)
+# training FLOP per sample: the forward pass, the weight gradients (as many
+# multiply-adds again) and the data gradients of layers 3, 2 and the output
+# (64 + 2 * 64*64)
+TRAIN_FLOP_PER_SAMPLE = FLOP_PER_PROPOSAL * 2 + 2 * (64 + 2 * 64 * 64)
 COUNTERS_MD = os.path.join(ROOT, "profiles", "walk_kernel_batched_1e6rects_16masters.md")
 
 WORKLOADS = {
@@ -407,6 +411,10 @@ def bench_batched(args, frw, torch, np):
     }
     if train_info:
         out["learned"] = dict(train_info, alpha=args.alpha)
+        out["learned"]["train_tflops"] = (train_info["pairs"] // 1024 * 1024 * train_info["epochs"] *
+                                          TRAIN_FLOP_PER_SAMPLE / max(train_info["train_s"], 1e-9) / 1e12)
+        out["learned"]["train_note"] = ("algorithmic training FLOP/s (forward, weight and data gradients; "
+                                        "batches of 1024, one optimiser step per batch: latency-bound)")
         if "learned" in results:
             out["learned"]["accept_rate"] = results["learned"]["accept_rate"]
             out["learned"]["proposal_accept_rate"] = results["learned"]["proposal_accept_rate"]
@@ -787,7 +795,7 @@ def bench_single(args, frw, torch, np):
     stream = torch.cuda.Stream()
     mlp, train_info = None, None
     if mode == "learned" or args.compare:
-        mlp, train_info = train_proposal(frw, torch, tree, master, min(args.pairs, 2_000_000), args.epochs)
+        mlp, train_info = train_proposal(frw, torch, tree, master, args.pairs, args.epochs)
     r = walk_mode(frw, torch, tree, mlp, mode, master, walks, args.steps, args.warmup, stream,
                   args.alpha, block=args.block)
     log(f"timed {args.steps} steps of {walks} walks ({mode}) in {r['wall']:.3f} s")
