@@ -543,7 +543,8 @@ def bench_batched(args, frw, torch, np):
         if m not in results:
             continue
         q = results[m]
-        roof = {"bound": "hbm", "kernel": f"walk_kernel<{m}> (batched)", "avg_launch_ms": q["avg_launch_ms"],
+        # the search and replay run from the caches: instruction issue binds
+        roof = {"bound": "issue", "kernel": f"walk_kernel<{m}> (batched)", "avg_launch_ms": q["avg_launch_ms"],
                 "traffic": None}
         add_counter_roofs(roof, cnt, m, q, walks_per_step)
         out.setdefault("rooflines_other", {})[m] = roof
@@ -591,6 +592,9 @@ def add_counter_roofs(roof, cnt, mode, q, walks_per_step):
                          "peak_per_s": ISSUE_PEAK, "frac": max(valu, salu) * hops_per_s / ISSUE_PEAK}
         if k.get("SQ_ACTIVE_INST_VALU"):
             roof["issue"]["valu_lane_utilisation"] = k["SQ_THREAD_CYCLES_VALU"] / (k["SQ_ACTIVE_INST_VALU"] * 64)
+        if roof.get("bound") == "issue":
+            roof.update(achieved=valu * hops_per_s, peak=ISSUE_PEAK, unit="VALU wave instructions/s",
+                        frac=roof["issue"]["frac"])
     if "bytes_per_hop" in k:
         roof["algorithmic_bytes_per_hop"] = k["bytes_per_hop"]
         ab = k["bytes_per_hop"] * hops_per_launch / (q["avg_launch_ms"] * 1e-3) / 1e9
