@@ -1,6 +1,7 @@
 // Learned square-size proposal: occupancy-window features and a 67-64-64-64-1
-// GELU network evaluated with bf16 MFMA (v_mfma_f32_32x32x8_bf16) and FP32
-// accumulation, and the occupancy-pyramid probe that verifies a proposal.
+// GELU network evaluated with bf16 MFMA (v_mfma_f32_16x16x32_bf16 on 16-row
+// blocks) and FP32 accumulation, and the occupancy-pyramid probe that
+// verifies a proposal.
 // No counterpart in the reference: the reference sizes every square by
 // bisection (randomWalk/CPU/quadTree.cpp:264-312).
 //
