@@ -266,36 +266,41 @@ int autotune(frw_batch* b, const frw_batch_params* base, frw_batch_params* best,
     cand.use_graph = 0;
     int cus = 0;
     if (rc == FRW_OK) rc = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device), "CUs");
-    const int blocks[] = {64, 128, 192, 256};
-    const int waves_per_cu[] = {4, 8, 16, 32};
-    for (int bi = 0; bi < 4 && rc == FRW_OK; ++bi) {
-        for (int wi = 0; wi < 4 && rc == FRW_OK; ++wi) {
-            cand.block = blocks[bi];
-            cand.grid = std::max(1, cus * waves_per_cu[wi] / (blocks[bi] / kWaveSize));
-            WalkLaunch L = launch_of(b, &cand);
-            int blk = 0, grd = 0;
-            size_t lds = 0;
-            if (plan_walk(L, &blk, &grd, &lds) != FRW_OK) continue;  // does not fit: skip
-            rc = check_hip(hipMemsetAsync(d_counts, 0, sizeof(int64_t) * b->counts_size, nullptr), "memset counts");
-            if (rc == FRW_OK) rc = check_hip(hipMemsetAsync(d_hops, 0, sizeof(int64_t) * b->jobs.size(), nullptr), "memset hops");
-            if (rc == FRW_OK) rc = check_hip(hipEventRecord(e0, nullptr), "hipEventRecord");
-            if (rc == FRW_OK) rc = run(&cand, d_counts, d_hops);
-            if (rc == FRW_OK) rc = check_hip(hipEventRecord(e1, nullptr), "hipEventRecord");
-            if (rc != FRW_OK) break;
-            if ((rc = check_hip(hipDeviceSynchronize(), "autotune run")) != FRW_OK) break;
-            float ms = 0.f;
-            if ((rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) != FRW_OK) break;
-            std::vector<int64_t> h(b->jobs.size());
-            if ((rc = check_hip(hipMemcpy(h.data(), d_hops, sizeof(int64_t) * h.size(), hipMemcpyDeviceToHost),
-                                "copy hops")) != FRW_OK)
-                break;
-            int64_t tot = 0;
-            for (int64_t v : h) tot += v;
-            const double rate = tot / (ms * 1e-3);
-            if (rate > best_rate) {
-                best_rate = rate;
-                *best = cand;
-            }
+    // (block, waves per CU) shapes: the learned mode's blocks hold the
+    // network's weights in LDS, so it also tries blocks of 8 and 12 waves
+    std::vector<std::pair<int, int>> shapes;
+    for (int blk : {64, 128, 192, 256})
+        for (int wv : {4, 8, 16, 32}) shapes.emplace_back(blk, wv);
+    if (base->mode == FRW_MODE_LEARNED)
+        for (const std::pair<int, int>& sh : {std::make_pair(512, 8), std::make_pair(512, 16), std::make_pair(768, 12),
+                                              std::make_pair(768, 24)})
+            shapes.push_back(sh);
+    for (size_t si = 0; si < shapes.size() && rc == FRW_OK; ++si) {
+        cand.block = shapes[si].first;
+        cand.grid = std::max(1, cus * shapes[si].second / (shapes[si].first / kWaveSize));
+        WalkLaunch L = launch_of(b, &cand);
+        int blk = 0, grd = 0;
+        size_t lds = 0;
+        if (plan_walk(L, &blk, &grd, &lds) != FRW_OK) continue;  // does not fit: skip
+        rc = check_hip(hipMemsetAsync(d_counts, 0, sizeof(int64_t) * b->counts_size, nullptr), "memset counts");
+        if (rc == FRW_OK) rc = check_hip(hipMemsetAsync(d_hops, 0, sizeof(int64_t) * b->jobs.size(), nullptr), "memset hops");
+        if (rc == FRW_OK) rc = check_hip(hipEventRecord(e0, nullptr), "hipEventRecord");
+        if (rc == FRW_OK) rc = run(&cand, d_counts, d_hops);
+        if (rc == FRW_OK) rc = check_hip(hipEventRecord(e1, nullptr), "hipEventRecord");
+        if (rc != FRW_OK) break;
+        if ((rc = check_hip(hipDeviceSynchronize(), "autotune run")) != FRW_OK) break;
+        float ms = 0.f;
+        if ((rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) != FRW_OK) break;
+        std::vector<int64_t> h(b->jobs.size());
+        if ((rc = check_hip(hipMemcpy(h.data(), d_hops, sizeof(int64_t) * h.size(), hipMemcpyDeviceToHost),
+                            "copy hops")) != FRW_OK)
+            break;
+        int64_t tot = 0;
+        for (int64_t v : h) tot += v;
+        const double rate = tot / (ms * 1e-3);
+        if (rate > best_rate) {
+            best_rate = rate;
+            *best = cand;
         }
     }
     if (e0) (void)hipEventDestroy(e0);

@@ -38,9 +38,14 @@ namespace frw {
 constexpr int kMlpIn = 67;   // features (window 64, level, position in cell x and y)
 constexpr int kMlpInPad = 72;   // K of the first layer, a multiple of 8
 constexpr int kMlpHid = 64;
-constexpr int kMlpStride = 72;  // LDS row stride (bf16 elements) of activations
+constexpr int kMlpStride = 72;  // LDS row stride (bf16 elements) of activations and weights
 // a wave's activation tile: 64 rows of kMlpStride, then the 64 FP32 outputs
 constexpr int kMlpTileElems = 64 * kMlpStride + 64 * 2;
+// a block's copy of the weights in LDS: W1 [64][72], W2 and W3 [64][64] (the
+// walk copies), each row at a stride of kMlpStride (144 bytes, so the 16 rows
+// a wave reads at once fall on distinct banks, as in the activation tile)
+constexpr int kMlpWElems = 64 * kMlpStride;
+constexpr int kMlpWLdsElems = 3 * kMlpWElems;  // 27 KB
 constexpr int kMlpMaxLevel = 15;  // feature-window levels
 constexpr int kPyrMaxLevel = 15;  // bitmap / summed-area-table levels
 constexpr int kCertShift = 4;     // verification cells are at most d / 2^4
@@ -248,6 +253,22 @@ __device__ inline void mlp_feature_row(uint16_t* row, uint64_t bits, int level, 
     *(bf16x4*)(row + 68) = bf16x4{0, 0, 0, 0};
 }
 
+// The walk kernel's feature row in its activation tile: columns 64..67 as in
+// the 72-wide row (level / 16, position in the cell, 0), the 64 window bits
+// packed in columns 68..71 (8 bytes) instead of written as 64 bf16 values;
+// the first layer expands them as it loads its operand (mlp_expand_bits).
+constexpr int kMlpBitsCol = 68;
+__device__ inline void mlp_feature_bits(uint16_t* row, uint64_t bits, int level, float frac_x, float frac_y)
+{
+    bf16x4 a;
+    a[0] = (short)f2bf((float)level / 16.0f);
+    a[1] = (short)f2bf(frac_x);
+    a[2] = (short)f2bf(frac_y);
+    a[3] = 0;
+    *(bf16x4*)(row + 64) = a;
+    *(uint64_t*)(row + kMlpBitsCol) = bits;
+}
+
 // Verification probe: true when the closed square (x0, y0, x1, y1) of
 // half-size d (> 0) lies in the bounds and its covering cell block at level
 // clamp(K0 - floor(log2 d) + 4, 0, Vmax) holds no occupied cell.
@@ -294,32 +315,54 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // walk copy of W2 and W3).
 __host__ __device__ constexpr int mlp_col(int n) { return (n & 0x23) | ((n & 0x0C) << 1) | ((n & 0x10) >> 2); }
 
+// Eight window bits (bit k of b = the k-th column) as a bf16x8 operand, 1.0
+// (0x3F80) where the bit is set, two columns per 32-bit word: for the bit
+// pair x = 2 b1 + b0, x * 0x8001 holds b0 at bits 0 and 15 and b1 at bits 1
+// and 16, the mask keeps bits 0 and 16, and * 0x3F80 puts bf16(1) in the
+// halves whose bit is set (every product below 2^24: 24-bit multiplies).
+__device__ __forceinline__ bf16x8 mlp_expand_bits(uint32_t b)
+{
+    uint32_t w[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const uint32_t x = (b >> (2 * p)) & 3u;
+        w[p] = __umul24(__umul24(x, 0x8001u) & 0x10001u, 0x3F80u);
+    }
+    return __builtin_bit_cast(bf16x8, w);
+}
+
 // The four 16x16 output tiles (neurons 16 t + 4 h + r of row j) of W * rows^T
-// for one block: two 16x16x32 steps over columns 0..63 and, for K = 72, one
-// v_mfma_f32_16x16x16_bf16 step over columns 64..79 of which the lanes with
-// h >= 2 (columns 72..79) supply zeros.
-template <int K>
-__device__ inline void mlp_tiles(const uint16_t* row, const uint16_t* W, int j, int h, f32x4 acc[4])
+// for one block, W in LDS at a row stride of kMlpStride: two 16x16x32 steps
+// over columns 0..63 and, for K = 72, one v_mfma_f32_16x16x16_bf16 step over
+// columns 64..79 of which only columns 64..67 (lanes with h = 0) are non-zero
+// (column 67 and W1's columns 67..71 are zero).  kBits: the row is the walk
+// kernel's (window bits packed in columns 68..71), expanded here as the
+// operand is loaded.
+template <int K, bool kBits = false>
+__device__ __forceinline__ void mlp_tiles(const uint16_t* row, const uint16_t* W, int j, int h, f32x4 acc[4])
 {
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    uint64_t bits = 0;
+    if (kBits) bits = *(const uint64_t*)(row + kMlpBitsCol);
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb) {
-        const bf16x8 x = *(const bf16x8*)(row + 32 * kb + 8 * h);
+        const bf16x8 x = kBits ? mlp_expand_bits((uint32_t)(bits >> (32 * kb + 8 * h)) & 0xFFu)
+                               : *(const bf16x8*)(row + 32 * kb + 8 * h);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const bf16x8 w = *(const bf16x8*)(W + (16 * t + j) * K + 32 * kb + 8 * h);
+            const bf16x8 w = *(const bf16x8*)(W + (16 * t + j) * kMlpStride + 32 * kb + 8 * h);
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8, w),
                                                              __builtin_bit_cast(bf16v8, x), acc[t], 0, 0, 0);
         }
     }
     if (K == 72) {
-        const bool in = h < 2;
+        const bool in = h == 0;
         const bf16x4 zero = {0, 0, 0, 0};
-        const bf16x4 x = in ? *(const bf16x4*)(row + 64 + 4 * h) : zero;
+        const bf16x4 x = in ? *(const bf16x4*)(row + 64) : zero;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const bf16x4 w = in ? *(const bf16x4*)(W + (16 * t + j) * K + 64 + 4 * h) : zero;
+            const bf16x4 w = in ? *(const bf16x4*)(W + (16 * t + j) * kMlpStride + 64) : zero;
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w, x, acc[t], 0, 0, 0);
         }
     }
@@ -327,13 +370,13 @@ __device__ inline void mlp_tiles(const uint16_t* row, const uint16_t* W, int j, 
 
 // One hidden layer for a 16-row block, in place: row = bf16(gelu(row * W^T +
 // b)) with the outputs in mlp_col order.
-template <int K>
-__device__ inline void mlp_layer(uint16_t* block, const uint16_t* W, const float* __restrict__ bias, int lane)
+template <int K, bool kBits = false>
+__device__ __forceinline__ void mlp_layer(uint16_t* block, const uint16_t* W, const float* __restrict__ bias, int lane)
 {
     const int j = lane & 15, h = lane >> 4;
     uint16_t* row = block + j * kMlpStride;
     f32x4 acc[4];
-    mlp_tiles<K>(row, W, j, h, acc);
+    mlp_tiles<K, kBits>(row, W, j, h, acc);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int n0 = 16 * t + 4 * h;
@@ -346,7 +389,7 @@ __device__ inline void mlp_layer(uint16_t* block, const uint16_t* W, const float
 // Third hidden layer and output neuron for a 16-row block: y = w4 .
 // bf16(gelu(row * W3^T + b3)) + b4 of row j, in every lane j + 16 h.  Each
 // lane sums its 16 neurons; the four lanes of a row add their sums.
-__device__ inline float mlp_last_layer(const uint16_t* block, const uint16_t* W3, const float* __restrict__ small, int lane)
+__device__ __forceinline__ float mlp_last_layer(const uint16_t* block, const uint16_t* W3, const float* __restrict__ small, int lane)
 {
     const int j = lane & 15, h = lane >> 4;
     f32x4 acc[4];
@@ -384,13 +427,39 @@ __device__ inline void mlp_stage_small(const MlpNet& m, float* small, int tid, i
     }
 }
 
+// Copies the three weight matrices into a block's LDS (kMlpWLdsElems bf16,
+// rows at a stride of kMlpStride), 16 bytes per copy; the caller
+// synchronises the block afterwards.
+__device__ inline void mlp_stage_weights(const MlpNet& m, uint16_t* w, int tid, int n_threads)
+{
+    constexpr int kW1Chunks = 64 * kMlpInPad / 8;  // 9 chunks of 8 per W1 row
+    constexpr int kWChunks = 64 * kMlpHid / 8;     // 8 per W2 or W3 row
+    for (int i = tid; i < kW1Chunks + 2 * kWChunks; i += n_threads) {
+        const uint16_t* src;
+        uint16_t* dst;
+        if (i < kW1Chunks) {
+            const int r = i / (kMlpInPad / 8), c = i % (kMlpInPad / 8);
+            src = m.W1 + r * kMlpInPad + 8 * c;
+            dst = w + r * kMlpStride + 8 * c;
+        } else {
+            const int k = i - kW1Chunks, mat = k / kWChunks, q = k % kWChunks;
+            const int r = q / (kMlpHid / 8), c = q % (kMlpHid / 8);
+            src = (mat ? m.W3 : m.W2) + r * kMlpHid + 8 * c;
+            dst = w + (1 + mat) * kMlpWElems + r * kMlpStride + 8 * c;
+        }
+        *(uint4*)dst = *(const uint4*)src;
+    }
+}
+
 // Forward pass of the first n_rows (0..64) feature rows of the wave's tile xs
-// (LDS, kMlpTileElems: 64 x kMlpStride, overwritten by the hidden
-// activations, then 64 floats of outputs), in ceil(n_rows / 16) blocks of 16
-// rows, with the small operands staged by mlp_stage_small.  Returns the
-// output of row `row` (any lane; rows at or beyond n_rows give 0).  Must be
-// called by all 64 lanes together with the same n_rows.
-__device__ inline float mlp_forward_rows(const MlpNet& m, uint16_t* xs, const float* small, int lane, int n_rows, int row)
+// (LDS, kMlpTileElems: 64 x kMlpStride rows as mlp_feature_bits writes them,
+// overwritten by the hidden activations, then 64 floats of outputs), in
+// ceil(n_rows / 16) blocks of 16 rows, with the weights staged in LDS by
+// mlp_stage_weights (w) and the small operands by mlp_stage_small.  Returns
+// the output of row `row` (any lane; rows at or beyond n_rows give 0).  Must
+// be called by all 64 lanes together with the same n_rows.
+__device__ __forceinline__ float mlp_forward_rows(const uint16_t* w, uint16_t* xs, const float* small, int lane, int n_rows,
+                                         int row)
 {
     float* ys = (float*)(xs + 64 * kMlpStride);
     wave_sync();  // the feature rows written by other lanes
@@ -398,16 +467,16 @@ __device__ inline float mlp_forward_rows(const MlpNet& m, uint16_t* xs, const fl
 #pragma unroll 1
     for (int b = 0; b < nb; ++b) {
         uint16_t* block = xs + 16 * b * kMlpStride;
-        // the weights are reloaded for every block (from the caches) rather
-        // than held across the loop: holding all three layers' fragments
-        // would take about 100 registers per lane
-        const uint16_t* W1 = m.W1;
-        const uint16_t* W2 = m.W2;
-        const uint16_t* W3 = m.W3;
-        asm volatile("" : "+s"(W1), "+s"(W2), "+s"(W3));
-        mlp_layer<kMlpInPad>(block, W1, small + kMlpSmallB1, lane);
-        mlp_layer<kMlpHid>(block, W2, small + kMlpSmallB2, lane);
-        const float y = mlp_last_layer(block, W3, small, lane);
+        // the weight fragments are read again for every block rather than
+        // held across the loop (all three layers' would take about 100
+        // registers per lane): an offset the compiler cannot see through
+        // keeps the reads in the loop
+        int off = 0;
+        asm volatile("" : "+s"(off));
+        const uint16_t* wb = w + off;
+        mlp_layer<kMlpInPad, true>(block, wb, small + kMlpSmallB1, lane);
+        mlp_layer<kMlpHid>(block, wb + kMlpWElems, small + kMlpSmallB2, lane);
+        const float y = mlp_last_layer(block, wb + 2 * kMlpWElems, small, lane);
         if (lane < 16) ys[16 * b + lane] = y;
     }
     wave_sync();

@@ -529,19 +529,27 @@ __global__ void narrow_bf16(const float* __restrict__ F, uint16_t* __restrict__ 
 }
 
 // Wave-level MFMA forward over rows of X (the walk kernel's inference path):
-// one wave per 64 rows.
+// one wave per 64 rows, each 72-wide row repacked into the walk kernel's tile
+// row (columns 0..63, which features hold as 0 or 1, as packed bits; columns
+// 64..67 as they are).
 __global__ void __launch_bounds__(64) forward_waves(MlpNet m, const uint16_t* __restrict__ X, int64_t n,
                                                     float* __restrict__ y)
 {
     __shared__ __align__(16) uint16_t xs[kMlpTileElems];
     __shared__ __align__(16) float small[kMlpSmallFloats];
+    __shared__ __align__(16) uint16_t w[kMlpWLdsElems];
     const int lane = threadIdx.x;
     const int64_t row = (int64_t)blockIdx.x * 64 + lane;
     mlp_stage_small(m, small, lane, 64);
+    mlp_stage_weights(m, w, lane, 64);
     __syncthreads();
-    for (int k = 0; k < kMlpInPad; ++k)
-        xs[lane * kMlpStride + k] = row < n ? X[row * kMlpInPad + k] : (uint16_t)0;
-    const float out = mlp_forward_rows(m, xs, small, lane, 64, lane);
+    uint16_t* r = xs + lane * kMlpStride;
+    uint64_t bits = 0;
+    for (int k = 0; k < 64; ++k)
+        if (row < n && (X[row * kMlpInPad + k] & 0x7FFFu) != 0u) bits |= 1ull << k;
+    for (int k = 64; k < kMlpBitsCol; ++k) r[k] = row < n ? X[row * kMlpInPad + k] : (uint16_t)0;
+    *(uint64_t*)(r + kMlpBitsCol) = bits;
+    const float out = mlp_forward_rows(w, xs, small, lane, 64, lane);
     if (row < n) y[row] = out;
 }
 

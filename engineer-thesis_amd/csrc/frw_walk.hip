@@ -63,6 +63,9 @@ struct WalkArgs {
 };
 
 constexpr int kLdsJobs = 128;                                   // job walk offsets cached in LDS
+// learned-mode blocks of up to 12 waves: one block per CU holds the 12 waves
+// the registers allow (3 per SIMD) with one LDS copy of the network's weights
+constexpr int kLearnedMaxBlock = 768;
 constexpr int kWaveChunk = 256;                                 // walk indices a wave claims per queue visit
 constexpr int kTailChunk = 64;                                  // ... once the queue nears its end
 constexpr uint32_t kGausDomain = 0x47415553u;                   // 'GAUS': Gaussian-surface start points
@@ -218,7 +221,7 @@ template <int kMode, bool kStats, bool kHarvest, bool kRuns, bool kSingle>
 // are wave-uniform and live in scalar registers), measured faster at 10^6
 // rects despite a few spilled registers.  kRuns: the traversal stack holds
 // runs of siblings (RunStack) instead of single nodes (PlainStack).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+__global__ void __launch_bounds__(kMode == kModeLearned ? kLearnedMaxBlock : 256) __attribute__((amdgpu_waves_per_eu(
     kMode == kModeLearned ? 3 : kMode == kModeWeighted ? (kSingle ? 4 : 3) : (kSingle ? 5 : 4))))
 walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
 {
@@ -237,8 +240,11 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
     constexpr int kBinVals = kMode == kModeWeighted ? 5 : 1;
     __shared__ unsigned long long s_bkey[kBinSlots];
     __shared__ unsigned long long s_bval[kBinSlots * kBinVals];
-    // learned mode: the network's biases and output weights
+    // learned mode: the network's biases and output weights, and its weight
+    // matrices (read from LDS by every block of 16 rows the wave evaluates;
+    // reloaded through the vector caches, they took 4 L2 reads per hop)
     __shared__ __align__(16) float s_small[kMode == kModeLearned ? kMlpSmallFloats : 4];
+    __shared__ __align__(16) uint16_t s_w[kMode == kModeLearned ? kMlpWLdsElems : 8];
     for (int i = threadIdx.x; i < kNSample; i += blockDim.x) sT[i] = a.thresholds[i];
     for (int i = threadIdx.x; i < min(a.n_jobs, kLdsJobs); i += blockDim.x) s_begin[i] = jobs[i].walk_begin;
     for (int i = threadIdx.x; i < 3 * kLdsJobs; i += blockDim.x) s_tally[i] = 0ull;
@@ -246,7 +252,10 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
     for (int i = threadIdx.x; i < kBinSlots * kBinVals; i += blockDim.x) s_bval[i] = 0ull;
     if (kMode == kModeWeighted)
         for (int i = threadIdx.x; i < 8; i += blockDim.x) s_wtally[i] = 0ull;
-    if (kMode == kModeLearned) mlp_stage_small(net, s_small, threadIdx.x, blockDim.x);
+    if (kMode == kModeLearned) {
+        mlp_stage_small(net, s_small, threadIdx.x, blockDim.x);
+        mlp_stage_weights(net, s_w, threadIdx.x, blockDim.x);
+    }
     __syncthreads();
     const int lane = threadIdx.x & (kWaveSize - 1);
     using Stack = typename std::conditional<kRuns, RunStack, PlainStack>::type;
@@ -481,9 +490,9 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
                     float fx = 0.0f, fy = 0.0f;
                     const int level = mlp_level(pyr, hprev, &cell);
                     const uint64_t bits = mlp_window(pyr, level, cell, px, py, &fx, &fy);
-                    mlp_feature_row(mlp_xs + row * kMlpStride, bits, level, fx, fy);
+                    mlp_feature_bits(mlp_xs + row * kMlpStride, bits, level, fx, fy);
                 }
-                y_mlp = mlp_forward_rows(net, mlp_xs, s_small, lane, __popcll(ask), want ? row : 64);
+                y_mlp = mlp_forward_rows(s_w, mlp_xs, s_small, lane, __popcll(ask), want ? row : 64);
             }
         }
         if (!active) continue;
@@ -637,8 +646,9 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
 __global__ void set_job(WalkJob* slot, WalkJob jb) { *slot = jb; }
 
 // Launch shape when the caller gives none: the block size (256, 192, 128 or
-// 64 lanes) that keeps the most lanes resident per CU for this kernel's
-// registers and its LDS stack, and one such full set of blocks per CU.
+// 64 lanes; the learned mode also 768 and 512) that keeps the most lanes
+// resident per CU for this kernel's registers and its LDS, and one such full
+// set of blocks per CU.
 struct AutoShape {
     bool on = false;
     size_t wave_bytes = 0, fixed = 0, lds_block = 0;
@@ -653,7 +663,8 @@ void launch_kernel_as(int grid, int block, size_t lds, const AutoShape& as, hipS
     auto kern = walk_kernel<kMode, kStats, kHarvest, kRuns, kSingle>;
     if (as.on) {
         int best = 0;
-        for (int b : {256, 192, 128, 64}) {
+        for (int b : {kLearnedMaxBlock, 512, 256, 192, 128, 64}) {
+            if (b > (kMode == kModeLearned ? kLearnedMaxBlock : 256)) continue;
             const size_t l = (size_t)(b / kWaveSize) * as.wave_bytes;
             if (l + as.fixed > as.lds_block) continue;
             int nb = 0;
@@ -687,12 +698,13 @@ void launch_kernel(bool runs, int grid, int block, size_t lds, const AutoShape& 
 }  // namespace
 
 // Static LDS of a walk block: Green thresholds, job offsets, job tallies,
-// the table of other bins' tallies, the learned mode's small operands.
+// the table of other bins' tallies, the learned mode's small operands and
+// weights.
 size_t walk_static_lds(int mode)
 {
     return kNSample * 4 + kLdsJobs * 8 + kLdsJobs * 24 + (mode == kModeWeighted ? 64 : 8) +
            (size_t)kBinSlots * 8 * (mode == kModeWeighted ? 6 : 2) +
-           (mode == kModeLearned ? kMlpSmallFloats * 4 : 16);
+           (mode == kModeLearned ? kMlpSmallFloats * 4 + kMlpWLdsElems * 2 : 16 + 16);
 }
 
 // LDS of one wave: its lanes' stacks, or the learned mode's activation
@@ -721,8 +733,9 @@ bool walk_uses_runs(const WalkLaunch& L, size_t lds_cu, size_t fixed)
 int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds, bool* runs)
 {
     int b = L.block;
-    if (b < 0 || b % kWaveSize != 0 || b > 256)
-        return set_error(FRW_E_INVALID, "block must be a multiple of 64 and <= 256");
+    const int bmax = L.mode == kModeLearned ? kLearnedMaxBlock : 256;
+    if (b < 0 || b % kWaveSize != 0 || b > bmax)
+        return set_error(FRW_E_INVALID, "block must be a multiple of 64 and <= 256 (learned mode: 768)");
     const size_t fixed = walk_static_lds(L.mode);
     int lds_block = 0, lds_cu = 0, cus = 0;
     if (hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, L.device) != hipSuccess ||
@@ -736,7 +749,7 @@ int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds, bool* run
     const size_t wave_bytes = walk_wave_bytes(L, use_runs);
     auto need = [&](int bb) { return (size_t)(bb / kWaveSize) * wave_bytes; };
     if (b == 0) {
-        b = 256;
+        b = bmax;
         while (b >= 64 && need(b) + fixed > (size_t)lds_block) b -= 64;
     }
     if (b < 64 || need(b) + fixed > (size_t)lds_block)
