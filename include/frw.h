@@ -170,7 +170,8 @@ typedef struct {
                           * word is XORed into the stream's domain word) */
     uint64_t seed;       /* Philox key */
     uint32_t job;        /* Philox counter word 1 (layout/master id in a batch) */
-    int32_t block;       /* threads per block, 0 = tuned default */
+    int32_t block;       /* threads per block: 0 = tuned default, else a multiple of 64,
+                          * at most 256 (learned walks: 768) */
     int32_t grid;        /* blocks, 0 = fill the device */
     int32_t reserved;
     int64_t max_hops;    /* per-walk hop cap (0 = 10^7); capped walks count as stuck */
@@ -241,7 +242,7 @@ typedef struct {
 
 typedef struct {
     int32_t mode;        /* FRW_MODE_EXACT or FRW_MODE_BISECT (learned: frw_batch_walk_learned) */
-    int32_t block;       /* 0 = default */
+    int32_t block;       /* 0 = default; a multiple of 64, at most 256 (learned: 768) */
     int32_t grid;        /* 0 = fill the device */
     int32_t use_graph;   /* capture the launch as a HIP graph once, then replay */
     uint64_t seed;
