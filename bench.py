@@ -85,6 +85,8 @@ def parse_args():
     ap.add_argument("--alpha", type=float, default=0.9, help="learned: safety factor")
     ap.add_argument("--pairs", type=int, default=10_000_000, help="learned: training pairs")
     ap.add_argument("--epochs", type=int, default=3, help="learned: training epochs")
+    ap.add_argument("--train-batch", type=int, default=4096, help="learned: rows per optimiser step")
+    ap.add_argument("--lr", type=float, default=1.2e-2, help="learned: Adam step size")
     ap.add_argument("--train-seed", type=int, default=1, help="learned: network init and shuffle seed")
     ap.add_argument("--max-level", type=int, default=15, help="learned: finest occupancy-pyramid level")
     ap.add_argument("--compare", action="store_true", default=True,
@@ -227,7 +229,7 @@ def time_walks(torch, launch, steps, warmup, sync_stream):
 
 # ---------------------------------------------------------------- batched (default)
 
-def train_batched(frw, torch, trees, harvest, pyr_of, n_pairs, epochs, seed, net):
+def train_batched(frw, torch, trees, harvest, pyr_of, n_pairs, epochs, seed, net, batch=4096, lr=1.2e-2):
     """Trains `net` (one of the trees' proposal objects, whose weights every
     job uses) on pairs harvested (in walk order) from exact walks of every
     (tree, master) of `harvest`, walk ids from 3 * 2^30 (disjoint from the
@@ -249,10 +251,10 @@ def train_batched(frw, torch, trees, harvest, pyr_of, n_pairs, epochs, seed, net
         got_total += got
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    loss = net.train(epochs=epochs, batch=1024, lr=2e-3, seed=seed)
+    loss = net.train(epochs=epochs, batch=batch, lr=lr, seed=seed)
     t2 = time.perf_counter()
     return net, {"pairs": got_total, "harvest_s": t1 - t0, "train_s": t2 - t1, "epochs": epochs,
-                 "final_loss": loss, "seed": seed}
+                 "batch": batch, "lr": lr, "final_loss": loss, "seed": seed}
 
 
 def bench_batched(args, frw, torch, np):
@@ -307,7 +309,7 @@ def bench_batched(args, frw, torch, np):
         pyr_s = time.perf_counter() - t0
         net = pyrs[int(np.argmax(sizes))]  # its weights serve every job, each job its own pyramid
         net, train_info = train_batched(frw, torch, trees, harvest, pyrs, args.pairs, args.epochs,
-                                        args.train_seed, net)
+                                        args.train_seed, net, args.train_batch, args.lr)
         train_info["pyramid_s"] = pyr_s
         train_info["pyramid_max_level"] = max_level
         batch.set_proposal(pyrs)
@@ -411,10 +413,11 @@ def bench_batched(args, frw, torch, np):
     }
     if train_info:
         out["learned"] = dict(train_info, alpha=args.alpha)
-        out["learned"]["train_tflops"] = (train_info["pairs"] // 1024 * 1024 * train_info["epochs"] *
+        tb = train_info["batch"]
+        out["learned"]["train_tflops"] = (train_info["pairs"] // tb * tb * train_info["epochs"] *
                                           TRAIN_FLOP_PER_SAMPLE / max(train_info["train_s"], 1e-9) / 1e12)
         out["learned"]["train_note"] = ("algorithmic training FLOP/s (forward, weight and data gradients; "
-                                        "batches of 1024, one optimiser step per batch: latency-bound)")
+                                        f"batches of {tb}, one optimiser step per batch: latency-bound)")
         if "learned" in results:
             out["learned"]["accept_rate"] = results["learned"]["accept_rate"]
             out["learned"]["proposal_accept_rate"] = results["learned"]["proposal_accept_rate"]
@@ -611,7 +614,7 @@ def add_counter_roofs(roof, cnt, mode, q, walks_per_step):
 
 # ---------------------------------------------------------------- single layout
 
-def train_proposal(frw, torch, tree, master, n_pairs, epochs, seed=1):
+def train_proposal(frw, torch, tree, master, n_pairs, epochs, seed=1, batch=4096, lr=1.2e-2):
     """Harvests (point, previous half-size, d*) pairs from exact walks (walk ids
     from 3 * 2^30, disjoint from the measured ones) and trains the network."""
     t0 = time.perf_counter()
@@ -629,12 +632,12 @@ def train_proposal(frw, torch, tree, master, n_pairs, epochs, seed=1):
     t1 = time.perf_counter()
     mlp.set_samples(pts[:got].contiguous(), hh[:got].contiguous(), ds[:got].contiguous())
     del pts, hh, ds
-    loss = mlp.train(epochs=epochs, batch=1024, lr=2e-3, seed=seed)
+    loss = mlp.train(epochs=epochs, batch=batch, lr=lr, seed=seed)
     t2 = time.perf_counter()
     log(f"proposal: {got} pairs harvested in {t1 - t0:.2f} s, {epochs} epochs in {t2 - t1:.2f} s, "
         f"loss {loss:.4f}")
     return mlp, {"pairs": got, "harvest_s": t1 - t0, "train_s": t2 - t1, "epochs": epochs,
-                 "final_loss": loss}
+                 "batch": batch, "lr": lr, "final_loss": loss}
 
 
 def walk_mode(frw, torch, tree, mlp, mode, master, walks, steps, warmup, stream, alpha, block=0,
@@ -799,7 +802,8 @@ def bench_single(args, frw, torch, np):
     stream = torch.cuda.Stream()
     mlp, train_info = None, None
     if mode == "learned" or args.compare:
-        mlp, train_info = train_proposal(frw, torch, tree, master, args.pairs, args.epochs)
+        mlp, train_info = train_proposal(frw, torch, tree, master, args.pairs, args.epochs,
+                                         batch=args.train_batch, lr=args.lr)
     r = walk_mode(frw, torch, tree, mlp, mode, master, walks, args.steps, args.warmup, stream,
                   args.alpha, block=args.block)
     log(f"timed {args.steps} steps of {walks} walks ({mode}) in {r['wall']:.3f} s")

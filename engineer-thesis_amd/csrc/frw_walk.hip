@@ -506,9 +506,11 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
         } else {
             bool proposed = false;
             if (kMode == kModeLearned && want) {
-                // outputs beyond the training targets' range [-4, 4] are not trusted
+                // outputs beyond the training targets' range [-4, 4] are not trusted;
+                // 2^y in single precision (a proposal's size needs no more: the
+                // probe decides whether the square is free)
                 const float y = fminf(fmaxf(y_mlp, -4.0f), 4.0f);
-                const double d = cell * exp2((double)y) * (double)net.alpha;
+                const double d = cell * (double)exp2f(y) * (double)net.alpha;
                 if (d > 2.0 && d < 1e300) {
                     const DBox c{px - d, py - d, px + d, py + d};
                     if (mlp_certify(pyr, c.x0, c.y0, c.x1, c.y1, d)) {
