@@ -48,9 +48,11 @@ constexpr int kBfW4 = kBfW3 + kMlpHid * kMlpHid;
 constexpr int kBfCount = kBfW4 + kMlpHid;
 
 // ---------------------------------------------------------------- occupancy
-// One lane per rect: sets the bits of every cell the closed rect meets, one
-// word mask per bitmap row segment.
-__global__ void occ_mark(const int4* __restrict__ rects, int n, int4 B, int K0, int Lmax,
+// One lane per rect: sets the bits of every cell of level L the closed rect
+// meets, one word mask per bitmap row segment.  Only the finest level is
+// marked this way; occ_reduce derives the coarser ones (marking every level
+// here had put every rect's atomics on the few words of the coarse levels).
+__global__ void occ_mark(const int4* __restrict__ rects, int n, int4 B, int K0, int L,
                          const int64_t* __restrict__ off, uint32_t* __restrict__ occ)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -58,25 +60,64 @@ __global__ void occ_mark(const int4* __restrict__ rects, int n, int4 B, int K0, 
     const int4 q = rects[r];
     const int64_t x0 = (int64_t)q.x - B.x, y0 = (int64_t)q.y - B.y;
     const int64_t x1 = (int64_t)q.z - B.x, y1 = (int64_t)q.w - B.y;
-    for (int l = 0; l <= Lmax; ++l) {
-        const int sh = K0 - l;
-        const int64_t nl = (int64_t)1 << l;
-        const int64_t i0 = x0 >> sh, i1 = min(x1 >> sh, nl - 1);
-        const int64_t j0 = y0 >> sh, j1 = min(y1 >> sh, nl - 1);
-        uint32_t* base = occ + off[l];
-        for (int64_t j = j0; j <= j1; ++j) {
-            int64_t a = j * nl + i0;
-            const int64_t e = j * nl + i1;  // inclusive
-            while (a <= e) {
-                const int64_t wd = a >> 5;
-                const int b0 = (int)(a & 31);
-                const int b1 = (int)min<int64_t>(31, b0 + (e - a));
-                const uint32_t mask = (b1 == 31 ? 0xFFFFFFFFu : ((1u << (b1 + 1)) - 1u)) & ~((1u << b0) - 1u);
-                atomicOr(base + wd, mask);
-                a += b1 - b0 + 1;
-            }
+    const int sh = K0 - L;
+    const int64_t nl = (int64_t)1 << L;
+    const int64_t i0 = x0 >> sh, i1 = min(x1 >> sh, nl - 1);
+    const int64_t j0 = y0 >> sh, j1 = min(y1 >> sh, nl - 1);
+    uint32_t* base = occ + off[L];
+    for (int64_t j = j0; j <= j1; ++j) {
+        int64_t a = j * nl + i0;
+        const int64_t e = j * nl + i1;  // inclusive
+        while (a <= e) {
+            const int64_t wd = a >> 5;
+            const int b0 = (int)(a & 31);
+            const int b1 = (int)min<int64_t>(31, b0 + (e - a));
+            const uint32_t mask = (b1 == 31 ? 0xFFFFFFFFu : ((1u << (b1 + 1)) - 1u)) & ~((1u << b0) - 1u);
+            atomicOr(base + wd, mask);
+            a += b1 - b0 + 1;
         }
     }
+}
+
+// One lane per word of level l (n x n cells) from level l + 1 (2n x 2n):
+// cell (i, j) is occupied when one of its four children (2i + a, 2j + b) is.
+// The children's half-open cells tile the parent's, and the finest level's
+// cell ranges, shifted right by one bit, are the parent level's, so this
+// equals marking level l directly.  For n >= 32 a word holds 32 cells of one
+// row: the OR of the two child rows' 64 bits, adjacent pairs ORed, the even
+// bits compacted; smaller levels go cell by cell.
+__global__ void occ_reduce(const uint32_t* __restrict__ child, int64_t n, uint32_t* __restrict__ parent)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t cells = n * n, words = (cells + 31) / 32, nc = 2 * n;
+    if (w >= words) return;
+    uint32_t out = 0u;
+    if (n >= 32) {
+        const int64_t b = 32 * w, j = b / n, i0 = b % n;
+        const uint32_t* r0 = child + ((2 * j) * nc + 2 * i0) / 32;
+        const uint32_t* r1 = child + ((2 * j + 1) * nc + 2 * i0) / 32;
+        uint64_t x = ((uint64_t)(r0[1] | r1[1]) << 32) | (uint64_t)(r0[0] | r1[0]);
+        x = (x | (x >> 1)) & 0x5555555555555555ull;
+        x = (x | (x >> 1)) & 0x3333333333333333ull;
+        x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+        x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+        x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+        x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+        out = (uint32_t)x;
+    } else {
+        for (int k = 0; k < 32; ++k) {
+            const int64_t b = 32 * w + k;
+            if (b >= cells) break;
+            const int64_t j = b / n, i = b % n;
+            uint32_t any = 0u;
+            for (int q = 0; q < 4; ++q) {
+                const int64_t cb = (2 * j + (q >> 1)) * nc + 2 * i + (q & 1);
+                any |= (child[cb >> 5] >> (cb & 31)) & 1u;
+            }
+            out |= any << k;
+        }
+    }
+    parent[w] = out;
 }
 
 // ---------------------------------------------------------------- summed-area tables
@@ -659,6 +700,11 @@ int frw_mlp_create(const frw_tree* t, uint64_t seed, int32_t max_level, frw_mlp*
         occ_mark<<<blocks(t->n_rects, 256), 256>>>(t->d_rects, (int)t->n_rects, t->bounds, m->K0, m->Pmax,
                                                    m->d_occ_off, m->d_occ);
         rc = check_hip(hipGetLastError(), "occ_mark");
+    }
+    for (int l = m->Pmax - 1; l >= 0 && rc == FRW_OK; --l) {
+        const int64_t n = (int64_t)1 << l, words = (n * n + 31) / 32;
+        occ_reduce<<<blocks(words, 256), 256>>>(m->d_occ + m->occ_off[l + 1], n, m->d_occ + m->occ_off[l]);
+        rc = check_hip(hipGetLastError(), "occ_reduce");
     }
     for (int l = 0; l <= m->Vmax && rc == FRW_OK; ++l) {
         const int64_t n = (int64_t)1 << l;
