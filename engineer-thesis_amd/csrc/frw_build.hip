@@ -156,37 +156,34 @@ __global__ void emit_children(const uint64_t* __restrict__ keys, const Frontier*
     }
 }
 
-// One block per node: tight bounding box of its run.
-__global__ void __launch_bounds__(256) node_boxes(const int4* __restrict__ rects,
-                                                  const int2* __restrict__ runs, int n_nodes,
-                                                  Node* __restrict__ nodes)
+// Tight bounding boxes of one level's nodes [begin, end), deepest level
+// first: a leaf's from its run of rects (an empty run gives the inverted box
+// every test fails on), an internal node's as the union of its four
+// children's, which the previous launch has written.  Every rect lives in a
+// leaf and a node's run is the union of its children's, so this is the box
+// of the node's run (one block per node over its whole run had spent 1 ms on
+// the 10^6-rect root alone).
+__global__ void __launch_bounds__(256) node_boxes_level(const int4* __restrict__ rects,
+                                                        const int2* __restrict__ runs, int begin, int end,
+                                                        Node* __restrict__ nodes)
 {
-    const int ni = blockIdx.x;
-    if (ni >= n_nodes) return;
-    const int2 run = runs[ni];
+    const int ni = begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (ni >= end) return;
     int x0 = INT_MAX, y0 = INT_MAX, x1 = INT_MIN, y1 = INT_MIN;
-    for (int r = run.x + (int)threadIdx.x; r < run.y; r += blockDim.x) {
-        const int4 q = rects[r];
-        x0 = min(x0, q.x); y0 = min(y0, q.y); x1 = max(x1, q.z); y1 = max(y1, q.w);
-    }
-    for (int off = 32; off >= 1; off >>= 1) {
-        x0 = min(x0, __shfl_xor(x0, off));
-        y0 = min(y0, __shfl_xor(y0, off));
-        x1 = max(x1, __shfl_xor(x1, off));
-        y1 = max(y1, __shfl_xor(y1, off));
-    }
-    __shared__ int4 part[4];
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    if (lane == 0) part[wave] = make_int4(x0, y0, x1, y1);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int4 b = part[0];
-        for (int w = 1; w < (int)(blockDim.x / 64); ++w) {
-            b.x = min(b.x, part[w].x); b.y = min(b.y, part[w].y);
-            b.z = max(b.z, part[w].z); b.w = max(b.w, part[w].w);
+    const int fc = nodes[ni].first_child;
+    if (fc < 0) {
+        const int2 run = runs[ni];
+        for (int r = run.x; r < run.y; ++r) {
+            const int4 q = rects[r];
+            x0 = min(x0, q.x); y0 = min(y0, q.y); x1 = max(x1, q.z); y1 = max(y1, q.w);
         }
-        nodes[ni].box = b;
+    } else {
+        for (int c = 0; c < 4; ++c) {
+            const int4 b = nodes[fc + c].box;
+            x0 = min(x0, b.x); y0 = min(y0, b.y); x1 = max(x1, b.z); y1 = max(y1, b.w);
+        }
     }
+    nodes[ni].box = make_int4(x0, y0, x1, y1);
 }
 
 template <typename T>
@@ -459,6 +456,7 @@ int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 B, int leaf_cap
     DevBuf<int2> runs;
     DevBuf<Node> nodes;
     std::vector<Frontier> root(4);
+    std::vector<int> level_start;  // nodes of depth d: [level_start[d], level_start[d + 1])
     int n_nodes = 4, nf = 4, depth = 0;
     const int nn = std::max(n, 1);
 
@@ -501,6 +499,7 @@ int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 B, int leaf_cap
     FRW_TRY(hipMemcpyAsync(fa.p, root.data(), sizeof(Frontier) * 4, hipMemcpyHostToDevice, s));
 
     while (nf > 0) {
+        level_start.push_back(n_nodes - nf);
         depth++;
         FRW_TRYRC(internal.reserve(nf, s));
         FRW_TRYRC(group.reserve(nf, s));
@@ -530,8 +529,14 @@ int build_tree(frw_tree* t, const std::vector<int4>& rects, int4 B, int leaf_cap
         nf = nf_next;
     }
 
-    node_boxes<<<n_nodes, 256, 0, s>>>(t->d_rects, runs.p, n_nodes, nodes.p);
-    FRW_TRY(hipGetLastError());
+    level_start.push_back(n_nodes);
+    for (int d = (int)level_start.size() - 2; d >= 0; --d) {
+        const int cnt = level_start[d + 1] - level_start[d];
+        if (cnt <= 0) continue;
+        node_boxes_level<<<blocks_for(cnt, 256), 256, 0, s>>>(t->d_rects, runs.p, level_start[d],
+                                                               level_start[d + 1], nodes.p);
+        FRW_TRY(hipGetLastError());
+    }
     FRW_TRY(hipMalloc(&t->d_nodes, sizeof(Node) * n_nodes));
     FRW_TRY(hipMemcpyAsync(t->d_nodes, nodes.p, sizeof(Node) * n_nodes, hipMemcpyDeviceToDevice, s));
     {
