@@ -218,10 +218,11 @@ __global__ void features_kernel(MlpPyramid m, const double* __restrict__ pts, co
 // the batch runs the forward pass, the loss and the backward pass of its rows
 // on bf16 MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation) entirely in LDS
 // and registers, and writes its share of the gradient (and of the loss) to a
-// row of partial sums.  train_apply: one lane per parameter adds the rows in
-// row order, takes the Adam step and writes the bf16 weight copies.  The
-// result does not depend on scheduling: each partial is a fixed sequence of
-// operations, and the rows are added in a fixed order.
+// row of partial sums.  train_apply: eight lanes per parameter add the rows
+// (each a fixed, interleaved eighth of them, in row order; then the eight
+// sums in lane order), take the Adam step and write the bf16 weight copies.
+// The result does not depend on scheduling: each partial is a fixed sequence
+// of operations, and the rows are added in a fixed order.
 //
 // Rounding points (those of the FP32/bf16 restatement the tests compare
 // against): inputs, activations and hidden-layer gradients are bf16, every
@@ -511,23 +512,45 @@ __device__ inline void write_copies(int i, float p, uint16_t* __restrict__ bf, u
 // step on the FP32 parameters and their bf16 copies, the loss added to
 // loss[0] (and 1 to loss[1]) when record is set; otherwise the gradient goes
 // to grad and the loss to loss[0].
-__global__ void train_apply(const float* __restrict__ part, int G, float* __restrict__ P, float* __restrict__ mom,
-                            float* __restrict__ vel, float lr, float b1c, float b2c, int adam, int record,
-                            float* __restrict__ grad, float* __restrict__ loss, uint16_t* __restrict__ bf,
-                            uint16_t* __restrict__ bfw, uint16_t* __restrict__ bfT)
+// Eight lanes per parameter: lane c of a parameter's eight sums rows c, c +
+// 8, c + 16, ... in that order, then lane 0 adds the eight sums in lane
+// order (a fixed order, so the step does not depend on scheduling).  32
+// parameters per block of 256.  With one lane per parameter the 52 blocks'
+// loads of a 256-row batch had made this launch latency-bound.
+constexpr int kApplyLanes = 8;
+constexpr int kApplyParamsPerBlock = 256 / kApplyLanes;
+
+__global__ void __launch_bounds__(256) train_apply(const float* __restrict__ part, int G, float* __restrict__ P,
+                                                   float* __restrict__ mom, float* __restrict__ vel, float lr,
+                                                   float b1c, float b2c, int adam, int record,
+                                                   float* __restrict__ grad, float* __restrict__ loss,
+                                                   uint16_t* __restrict__ bf, uint16_t* __restrict__ bfw,
+                                                   uint16_t* __restrict__ bfT)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > kMlpParams) return;
-    // the rows' loads issued 16 at a time, then added in row order
+    __shared__ float s_sum[kApplyLanes][kApplyParamsPerBlock];
+    const int pi = threadIdx.x % kApplyParamsPerBlock, c = threadIdx.x / kApplyParamsPerBlock;
+    const int i = blockIdx.x * kApplyParamsPerBlock + pi;
+    // the chunk's loads issued 8 at a time, then added in row order
     float g = 0.0f;
-    for (int w0 = 0; w0 < G; w0 += 16) {
-        float v[16];
+    if (i <= kMlpParams) {
+        for (int w0 = c; w0 < G; w0 += 8 * kApplyLanes) {
+            float v[8];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = w0 + u < G ? part[(int64_t)(w0 + u) * kPartStride + i] : 0.0f;
+            for (int u = 0; u < 8; ++u) {
+                const int w = w0 + u * kApplyLanes;
+                v[u] = w < G ? part[(int64_t)w * kPartStride + i] : 0.0f;
+            }
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
-            if (w0 + u < G) g += v[u];
+            for (int u = 0; u < 8; ++u)
+                if (w0 + u * kApplyLanes < G) g += v[u];
+        }
     }
+    s_sum[c][pi] = g;
+    __syncthreads();
+    if (c != 0 || i > kMlpParams) return;
+    g = s_sum[0][pi];
+#pragma unroll
+    for (int k = 1; k < kApplyLanes; ++k) g += s_sum[k][pi];
     if (i == kMlpParams) {
         if (!adam) {
             loss[0] = g;
@@ -855,9 +878,9 @@ int frw_mlp_gradient(frw_mlp* m, const float* X, const float* t, int64_t n, floa
     narrow_bf16<<<blocks(n * kMlpInPad, 256), 256>>>(X, Xb, n * kMlpInPad);
     train_tile<<<(unsigned)(n / kTrRows), 64>>>(Xb, t, (uint32_t)n, 0u, 0, make_uint4(0, 0, 0, 0), (int)n,
                                                 m->d_params, m->d_bf, m->d_bfT, m->d_part);
-    train_apply<<<blocks(kMlpParams + 1, 256), 256>>>(m->d_part, (int)(n / kTrRows), m->d_params, m->d_adam_m,
-                                                      m->d_adam_v, 0.0f, 1.0f, 1.0f, 0, 0, m->d_grad, m->d_loss,
-                                                      m->d_bf, m->d_bfw, m->d_bfT);
+    train_apply<<<blocks(kMlpParams + 1, kApplyParamsPerBlock), 256>>>(
+        m->d_part, (int)(n / kTrRows), m->d_params, m->d_adam_m, m->d_adam_v, 0.0f, 1.0f, 1.0f, 0, 0, m->d_grad,
+        m->d_loss, m->d_bf, m->d_bfw, m->d_bfT);
     rc = check_hip(hipDeviceSynchronize(), "mlp gradient");
     if (rc == FRW_OK) rc = check_hip(hipMemcpy(grad, m->d_grad, kMlpParams * 4, hipMemcpyDeviceToHost), "copy grad");
     if (rc == FRW_OK && loss) rc = check_hip(hipMemcpy(loss, m->d_loss, 4, hipMemcpyDeviceToHost), "copy loss");
@@ -932,9 +955,9 @@ int frw_mlp_train(frw_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t 
             m->adam_step++;
             const float b1c = 1.0f - std::pow(0.9f, (float)m->adam_step);
             const float b2c = 1.0f - std::pow(0.999f, (float)m->adam_step);
-            train_apply<<<blocks(kMlpParams + 1, 256), 256>>>(m->d_part, G, m->d_params, m->d_adam_m, m->d_adam_v,
-                                                              lr, b1c, b2c, 1, ep == epochs - 1 ? 1 : 0, nullptr,
-                                                              m->d_loss, m->d_bf, m->d_bfw, m->d_bfT);
+            train_apply<<<blocks(kMlpParams + 1, kApplyParamsPerBlock), 256>>>(
+                m->d_part, G, m->d_params, m->d_adam_m, m->d_adam_v, lr, b1c, b2c, 1, ep == epochs - 1 ? 1 : 0,
+                nullptr, m->d_loss, m->d_bf, m->d_bfw, m->d_bfT);
         }
         rc = check_hip(hipGetLastError(), "training step");
     }
