@@ -87,6 +87,9 @@ def parse_args():
     ap.add_argument("--epochs", type=int, default=3, help="learned: training epochs")
     ap.add_argument("--train-batch", type=int, default=4096, help="learned: rows per optimiser step")
     ap.add_argument("--lr", type=float, default=1.2e-2, help="learned: Adam step size")
+    ap.add_argument("--save-net", default="", help="learned: save the trained parameters (.npy)")
+    ap.add_argument("--load-net", default="",
+                    help="learned: load the parameters (.npy) instead of harvesting and training")
     ap.add_argument("--train-seed", type=int, default=1, help="learned: network init and shuffle seed")
     ap.add_argument("--max-level", type=int, default=15, help="learned: finest occupancy-pyramid level")
     ap.add_argument("--compare", action="store_true", default=True,
@@ -308,14 +311,20 @@ def bench_batched(args, frw, torch, np):
         torch.cuda.synchronize()
         pyr_s = time.perf_counter() - t0
         net = pyrs[int(np.argmax(sizes))]  # its weights serve every job, each job its own pyramid
-        net, train_info = train_batched(frw, torch, trees, harvest, pyrs, args.pairs, args.epochs,
-                                        args.train_seed, net, args.train_batch, args.lr)
+        if args.load_net:
+            net.set_params(np.load(args.load_net))
+            train_info = loaded_net_info(args)
+        else:
+            net, train_info = train_batched(frw, torch, trees, harvest, pyrs, args.pairs, args.epochs,
+                                            args.train_seed, net, args.train_batch, args.lr)
+            if args.save_net:
+                np.save(args.save_net, net.params())
         train_info["pyramid_s"] = pyr_s
         train_info["pyramid_max_level"] = max_level
         batch.set_proposal(pyrs)
         log(f"proposal: pyramids in {pyr_s:.2f} s, {train_info['pairs']} pairs harvested in "
-            f"{train_info['harvest_s']:.2f} s, trained {args.epochs} epochs in {train_info['train_s']:.2f} s, "
-            f"loss {train_info['final_loss']:.4f}")
+            f"{train_info['harvest_s']:.2f} s, trained {train_info['epochs']} epochs in {train_info['train_s']:.2f} s, "
+            f"loss {train_info['final_loss']}")
     counts = torch.zeros(batch.counts_size(), dtype=torch.int64, device="cuda")
     hops = torch.zeros(n_jobs, dtype=torch.int64, device="cuda")
     stats = torch.zeros(2, dtype=torch.int64, device="cuda")
@@ -614,6 +623,13 @@ def add_counter_roofs(roof, cnt, mode, q, walks_per_step):
 
 # ---------------------------------------------------------------- single layout
 
+def loaded_net_info(args):
+    """Training record of a network loaded with --load-net (no harvest, no
+    training in this run: a counter run profiles the walks alone)."""
+    return {"pairs": 0, "harvest_s": 0.0, "train_s": 0.0, "epochs": 0, "batch": args.train_batch, "lr": args.lr,
+            "final_loss": None, "seed": args.train_seed, "loaded_from": args.load_net}
+
+
 def train_proposal(frw, torch, tree, master, n_pairs, epochs, seed=1, batch=4096, lr=1.2e-2):
     """Harvests (point, previous half-size, d*) pairs from exact walks (walk ids
     from 3 * 2^30, disjoint from the measured ones) and trains the network."""
@@ -801,9 +817,15 @@ def bench_single(args, frw, torch, np):
     log(f"layout {nr} rects, tree {info.n_nodes} nodes depth {info.depth}, build {info.build_ms:.2f} ms")
     stream = torch.cuda.Stream()
     mlp, train_info = None, None
-    if mode == "learned" or args.compare:
+    if (mode == "learned" or args.compare) and args.load_net:
+        mlp = frw.Mlp(tree, seed=args.train_seed)
+        mlp.set_params(np.load(args.load_net))
+        train_info = loaded_net_info(args)
+    elif mode == "learned" or args.compare:
         mlp, train_info = train_proposal(frw, torch, tree, master, args.pairs, args.epochs,
                                          batch=args.train_batch, lr=args.lr)
+        if args.save_net:
+            np.save(args.save_net, mlp.params())
     r = walk_mode(frw, torch, tree, mlp, mode, master, walks, args.steps, args.warmup, stream,
                   args.alpha, block=args.block)
     log(f"timed {args.steps} steps of {walks} walks ({mode}) in {r['wall']:.3f} s")
