@@ -77,9 +77,18 @@ constexpr uint32_t kGausDomain = 0x47415553u;                   // 'GAUS': Gauss
 // serialised the kernel (10^4 rects, 10^7 walks: exact walks 6.3 ms, 5.1 ms
 // with those atomics taken out; weighted walks, four 64-bit atomics per walk,
 // 25.4 ms and 7.5 ms).  A key whose kBinProbes slots are taken by other keys
-// goes straight to its bin.
-constexpr int kBinSlotsLog2 = 6;
-constexpr int kBinSlots = 1 << kBinSlotsLog2;
+// goes straight to its bin.  Batched launches hold more slots: a block walks
+// jobs one after another, and with 64 slots the first jobs' keys filled the
+// table (64 layouts of 10^2-10^5 rects: exact walks 27.2 ms per launch with
+// 64 slots, 20.8 with 512).  The bisection keeps 64: its walks end far less
+// often than its collision queries run, and the larger table's LDS cost it
+// resident waves (348 -> 375 ms on the same batch).
+constexpr int kBinSlotsLog2Small = 6;
+constexpr int kBinSlotsLog2Batched = 9;
+__host__ __device__ constexpr int bin_slots_log2(int mode, bool single)
+{
+    return single || mode == kModeBisect ? kBinSlotsLog2Small : kBinSlotsLog2Batched;
+}
 constexpr int kBinProbes = 8;
 constexpr unsigned long long kBinEmpty = ~0ull;  // no key: a job index is below 2^31
 
@@ -88,12 +97,14 @@ __device__ inline unsigned long long bin_key(int job, int bin)
     return ((unsigned long long)(uint32_t)job << 32) | (uint32_t)bin;
 }
 
-// Slot of key in the block's table (claimed if the key is new), or -1 when
-// every slot of its probe path holds another key.
+// Slot of key in the block's table of 2^kLog2 slots (claimed if the key is
+// new), or -1 when every slot of its probe path holds another key.
+template <int kLog2>
 __device__ inline int bin_slot(unsigned long long* keys, unsigned long long key)
 {
+    constexpr int kBinSlots = 1 << kLog2;
     const uint32_t h = ((uint32_t)key ^ (uint32_t)(key >> 32) * 0x85EBCA6Bu) * 0x9E3779B1u;
-    int s = (int)(h >> (32 - kBinSlotsLog2));
+    int s = (int)(h >> (32 - kLog2));
     for (int p = 0; p < kBinProbes; ++p) {
         const unsigned long long k = ((volatile unsigned long long*)keys)[s];
         if (k == key) return s;
@@ -238,6 +249,8 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
     // the block's tallies of other bins: keys, then per slot the count (and,
     // weighted, the fixed-point sum and sum of squares, lo and hi words)
     constexpr int kBinVals = kMode == kModeWeighted ? 5 : 1;
+    constexpr int kBinLog2 = bin_slots_log2(kMode, kSingle);
+    constexpr int kBinSlots = 1 << kBinLog2;
     __shared__ unsigned long long s_bkey[kBinSlots];
     __shared__ unsigned long long s_bval[kBinSlots * kBinVals];
     // learned mode: the network's biases and output weights, and its weight
@@ -566,7 +579,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
             } else if (result == -1) {
                 my_escaped++;
             } else {
-                slot = bin_slot(s_bkey, bin_key(j, bin));
+                slot = bin_slot<kBinLog2>(s_bkey, bin_key(j, bin));
                 if (slot >= 0) atomicAdd(&s_bval[kBinVals * slot], 1ull);
                 else atomicAdd(&jobs[j].counts[bin], 1ull);
             }
@@ -702,10 +715,11 @@ void launch_kernel(bool runs, int grid, int block, size_t lds, const AutoShape& 
 // Static LDS of a walk block: Green thresholds, job offsets, job tallies,
 // the table of other bins' tallies, the learned mode's small operands and
 // weights.
-size_t walk_static_lds(int mode)
+size_t walk_static_lds(int mode, bool single)
 {
+    const size_t bin_slots = (size_t)1 << bin_slots_log2(mode, single);
     return kNSample * 4 + kLdsJobs * 8 + kLdsJobs * 24 + (mode == kModeWeighted ? 64 : 8) +
-           (size_t)kBinSlots * 8 * (mode == kModeWeighted ? 6 : 2) +
+           bin_slots * 8 * (mode == kModeWeighted ? 6 : 2) +
            (mode == kModeLearned ? kMlpSmallFloats * 4 + kMlpWLdsElems * 2 : 16 + 16);
 }
 
@@ -738,7 +752,7 @@ int plan_walk(const WalkLaunch& L, int* block, int* grid, size_t* lds, bool* run
     const int bmax = L.mode == kModeLearned ? kLearnedMaxBlock : 256;
     if (b < 0 || b % kWaveSize != 0 || b > bmax)
         return set_error(FRW_E_INVALID, "block must be a multiple of 64 and <= 256 (learned mode: 768)");
-    const size_t fixed = walk_static_lds(L.mode);
+    const size_t fixed = walk_static_lds(L.mode, L.n_jobs == 1);
     int lds_block = 0, lds_cu = 0, cus = 0;
     if (hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, L.device) != hipSuccess ||
         hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, L.device) != hipSuccess ||
@@ -787,7 +801,7 @@ int launch_walk_jobs(WalkJob* d_jobs, const WalkLaunch& L, hipStream_t s)
             return set_error(FRW_E_DEVICE, "device attributes unavailable");
         as.on = true;
         as.wave_bytes = wave_bytes;
-        as.fixed = walk_static_lds(L.mode);
+        as.fixed = walk_static_lds(L.mode, L.n_jobs == 1);
         as.lds_block = (size_t)lds_block;
         as.cus = std::max(cus, 1);
         as.total = L.total_walks;
