@@ -1,8 +1,10 @@
 """Tallies of walks that end on a rect other than their job's master go
-through a per-block table in LDS (64 slots, open addressing on (job, bin),
-8 probes, flushed once per block; a key that finds no slot goes straight to
-its bin).  These tests give the table more distinct bins per block than it
-has slots, so both of its paths run, and check that every bin still holds
+through a per-block table in LDS (open addressing on (job, bin), 8 probes,
+flushed once per block; a key that finds no slot goes straight to its bin):
+64 slots for single-job and bisection launches, 512 for batched exact and
+learned launches, whose blocks walk many jobs in turn.  These tests give the
+table more distinct bins per block than it has slots, so both of its paths
+run, and check that every bin still holds
 exactly the walks that ended there (per-walk outputs and the oracle's replay
 are the reference), for several launch shapes, for the weighted estimator's
 exact fixed-point sums, and for batched jobs whose keys differ only in the
@@ -102,8 +104,10 @@ def test_weighted_sums_exact_with_a_full_table(dense, block, grid):
 
 
 def test_batched_jobs_on_one_layout_keep_their_bins_apart(dense):
-    """Two jobs on the same tree hit the same rects: their table keys differ
-    only in the job, and each job's bins must equal its own single launch."""
+    """Jobs on the same tree hit the same rects: their table keys differ
+    only in the job, and each job's bins must equal its own single launch.
+    The jobs' distinct (job, bin) keys outnumber the 512 slots of the
+    batched table."""
     torch = torch_dev()
     lay, tree, rects, bounds = dense
     nr = len(rects)
@@ -117,6 +121,8 @@ def test_batched_jobs_on_one_layout_keep_their_bins_apart(dense):
     torch.cuda.synchronize()
     batch.check()
     c = counts.cpu().numpy()
+    keys = sum(np.count_nonzero(c[batch.counts_offset(j):batch.counts_offset(j) + nr] > 0) for j in range(len(jobs)))
+    assert keys > 512, keys
     for j, q in enumerate(jobs):
         c1 = torch.zeros(nr + 2, dtype=torch.int64, device="cuda")
         h1 = torch.zeros(1, dtype=torch.int64, device="cuda")
