@@ -66,7 +66,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // Occupancy pyramid of one layout, passed by value (walk jobs carry one each).
 struct MlpPyramid {
-    const uint32_t* occ;      // bitmaps, level l at word occ_off[l]
+    const uint64_t* occ;      // bitmaps in 8x8-cell tiles (occ_tiles), level l at word occ_off[l]
     const uint32_t* sat;      // summed-area tables, level l at entry sat_off[l]
     const int64_t* occ_off;   // device arrays of Pmax + 1 / Vmax + 1 offsets
     const int64_t* sat_off;
@@ -187,9 +187,18 @@ __device__ inline int mlp_level(const MlpPyramid& m, double h, double* cell)
     return l;
 }
 
-// 64 occupancy bits of the 8x8 window (bit v*8+u) at level l around p.  Each
-// window row is one unaligned 8-bit field of the level's row-major bitmap,
-// read as two words (every level carries one padding word at its end).
+// Occupancy bitmaps are stored in tiles of 8x8 cells, one 64-bit word each:
+// bit 8 y + x of tile (tx, ty) is cell (8 tx + x, 8 ty + y), tiles row-major,
+// occ_tiles(l) per side (levels below 3 use the low cells of one tile).
+__host__ __device__ constexpr int64_t occ_tiles(int l) { return l < 3 ? 1 : (int64_t)1 << (l - 3); }
+
+// Bytes 0..k-1 of a 64-bit word set (k = 0..8).
+__host__ __device__ inline uint64_t occ_low_bytes(int k) { return k >= 8 ? ~0ull : (1ull << (8 * k)) - 1ull; }
+
+// 64 occupancy bits of the 8x8 window (bit v*8+u) at level l around p: the
+// cells (c - 4 + u, c' - 4 + v) of p's cell (c, c').  The window meets at
+// most 2x2 tiles: four word loads, merged by byte-wise shifts.  Cells
+// outside the grid read as occupied.
 __device__ inline uint64_t mlp_window(const MlpPyramid& m, int l, double cell, double px, double py, float* frac_x,
                                       float* frac_y)
 {
@@ -207,27 +216,28 @@ __device__ inline uint64_t mlp_window(const MlpPyramid& m, int l, double cell, d
     if (!(fx > -16.0 && fy > -16.0 && fx < (double)(n + 16) && fy < (double)(n + 16))) return ~0ull;
     *frac_x = (float)(qx - fx);
     *frac_y = (float)(qy - fy);
-    const int64_t ci = (int64_t)fx;
-    const int64_t cj = (int64_t)fy;
-    const uint32_t* base = m.occ + m.occ_off[l];
-    const int64_t i0 = ci - 4;
-    const int64_t lo = i0 > 0 ? i0 : 0;
-    const int64_t hi = i0 + 8 < n ? i0 + 8 : n;
-    uint32_t inside = 0;  // window columns inside the grid
-    if (lo < hi) inside = ((1u << (hi - lo)) - 1u) << (lo - i0);
-    uint64_t bits = 0;
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-        const int64_t j = cj - 4 + v;
-        uint32_t row = 0xFFu;
-        if (inside && j >= 0 && j < n) {
-            const int64_t idx = j * n + lo;
-            const uint64_t w = ((uint64_t)base[(idx >> 5) + 1] << 32) | base[idx >> 5];
-            const uint32_t field = (uint32_t)(w >> (idx & 31)) & ((1u << (hi - lo)) - 1u);
-            row = (field << (lo - i0)) | (~inside & 0xFFu);
-        }
-        bits |= (uint64_t)row << (v * 8);
-    }
+    const int64_t i0 = (int64_t)fx - 4, j0 = (int64_t)fy - 4;  // the window's first cell
+    const int64_t T = occ_tiles(l);
+    const int64_t tx = i0 >> 3, ty = j0 >> 3;  // floor(i0 / 8), also below 0
+    const int ox = (int)(i0 & 7), oy = (int)(j0 & 7);
+    const uint64_t* base = m.occ + m.occ_off[l];
+    auto tile = [&](int64_t x, int64_t y) -> uint64_t {
+        return (x >= 0 && x < T && y >= 0 && y < T) ? base[y * T + x] : 0ull;
+    };
+    const uint64_t t00 = tile(tx, ty), t10 = tile(tx + 1, ty), t01 = tile(tx, ty + 1), t11 = tile(tx + 1, ty + 1);
+    // window column u: column ox + u of the left tile (< 8) or ox + u - 8 of
+    // the right one, in every byte (row) at once
+    const uint64_t left = 0x0101010101010101ull * (0xFFu >> ox);
+    auto merge = [&](uint64_t a, uint64_t b) -> uint64_t { return ((a >> ox) & left) | ((b << (8 - ox)) & ~left); };
+    const uint64_t top = merge(t00, t10), bottom = merge(t01, t11);
+    // window row v: row oy + v of the top pair (< 8) or oy + v - 8 of the bottom one
+    uint64_t bits = (top >> (8 * oy)) | (oy ? bottom << (64 - 8 * oy) : 0ull);
+    // window columns [c0, c1) and rows [r0, r1) lie inside the grid
+    const int c0 = (int)(i0 < 0 ? -i0 : 0), c1 = (int)(n - i0 < 8 ? (n - i0 > c0 ? n - i0 : c0) : 8);
+    const int r0 = (int)(j0 < 0 ? -j0 : 0), r1 = (int)(n - j0 < 8 ? (n - j0 > r0 ? n - j0 : r0) : 8);
+    const uint32_t cols = ((1u << c1) - 1u) & ~((1u << c0) - 1u);
+    const uint64_t inside = (0x0101010101010101ull * cols) & occ_low_bytes(r1) & ~occ_low_bytes(r0);
+    bits = (bits & inside) | ~inside;
     return bits;
 }
 
@@ -308,11 +318,12 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // 16 t + 4 h + r (t = 0..3 tiles of 16 neurons, r = 0..3) of row j.
 //
 // Column of the tile that holds hidden neuron n (0..63) of a layer's output:
-// n = 16 t + 4 h + r goes to 32 (t / 2) + 8 h + 4 (t mod 2) + r, so that lane
-// (j, h) writes exactly the columns it reads as the next layer's B operand
-// (no other lane touches them, hence no barrier between layers).  The next
-// layer's weights are stored with their input columns in the same order (the
-// walk copy of W2 and W3).
+// n = 16 t + 4 h + r goes to 32 (t / 2) + 8 h + 4 (t mod 2) + r, so that
+// lane (j, h) holds exactly the columns it supplies as the next layer's B
+// operand: a hidden layer's activations stay in the lane's registers (no
+// other lane needs them, hence no LDS round trip and no barrier between
+// layers).  The next layer's weights are stored with their input columns in
+// the same order (the walk copy of W2 and W3).
 __host__ __device__ constexpr int mlp_col(int n) { return (n & 0x23) | ((n & 0x0C) << 1) | ((n & 0x10) >> 2); }
 
 // Eight window bits (bit k of b = the k-th column) as a bf16x8 operand, 1.0
@@ -332,68 +343,73 @@ __device__ __forceinline__ bf16x8 mlp_expand_bits(uint32_t b)
 }
 
 // The four 16x16 output tiles (neurons 16 t + 4 h + r of row j) of W * rows^T
-// for one block, W in LDS at a row stride of kMlpStride: two 16x16x32 steps
-// over columns 0..63 and, for K = 72, one v_mfma_f32_16x16x16_bf16 step over
-// columns 64..79 of which only columns 64..67 (lanes with h = 0) are non-zero
-// (column 67 and W1's columns 67..71 are zero).  kBits: the row is the walk
-// kernel's (window bits packed in columns 68..71), expanded here as the
-// operand is loaded.
-template <int K, bool kBits = false>
-__device__ __forceinline__ void mlp_tiles(const uint16_t* row, const uint16_t* W, int j, int h, f32x4 acc[4])
+// over 64 input columns, W in LDS at a row stride of kMlpStride, the lane's
+// B operand in registers: x[k] = columns 32 k + 8 h .. + 7 of row j.
+__device__ __forceinline__ void mlp_tiles64(const bf16x8 x[2], const uint16_t* W, int j, int h, f32x4 acc[4])
 {
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    uint64_t bits = 0;
-    if (kBits) bits = *(const uint64_t*)(row + kMlpBitsCol);
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb) {
-        const bf16x8 x = kBits ? mlp_expand_bits((uint32_t)(bits >> (32 * kb + 8 * h)) & 0xFFu)
-                               : *(const bf16x8*)(row + 32 * kb + 8 * h);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const bf16x8 w = *(const bf16x8*)(W + (16 * t + j) * kMlpStride + 32 * kb + 8 * h);
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8, w),
-                                                             __builtin_bit_cast(bf16v8, x), acc[t], 0, 0, 0);
-        }
-    }
-    if (K == 72) {
-        const bool in = h == 0;
-        const bf16x4 zero = {0, 0, 0, 0};
-        const bf16x4 x = in ? *(const bf16x4*)(row + 64) : zero;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const bf16x4 w = in ? *(const bf16x4*)(W + (16 * t + j) * kMlpStride + 64) : zero;
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w, x, acc[t], 0, 0, 0);
+                                                             __builtin_bit_cast(bf16v8, x[kb]), acc[t], 0, 0, 0);
         }
     }
 }
 
-// One hidden layer for a 16-row block, in place: row = bf16(gelu(row * W^T +
-// b)) with the outputs in mlp_col order.
-template <int K, bool kBits = false>
-__device__ __forceinline__ void mlp_layer(uint16_t* block, const uint16_t* W, const float* __restrict__ bias, int lane)
+// The first layer's tiles from the walk kernel's feature row in LDS: the
+// window bits (packed in columns 68..71) expanded to the operand of columns
+// 0..63 as it is loaded, then one v_mfma_f32_16x16x16_bf16 step over
+// columns 64..79 of which only columns 64..67 (lanes with h = 0) are
+// non-zero (column 67 and W1's columns 67..71 are zero).
+__device__ __forceinline__ void mlp_tiles_first(const uint16_t* row, const uint16_t* W, int j, int h, f32x4 acc[4])
 {
-    const int j = lane & 15, h = lane >> 4;
-    uint16_t* row = block + j * kMlpStride;
-    f32x4 acc[4];
-    mlp_tiles<K, kBits>(row, W, j, h, acc);
+    const uint64_t bits = *(const uint64_t*)(row + kMlpBitsCol);
+    bf16x8 x[2];
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) x[kb] = mlp_expand_bits((uint32_t)(bits >> (32 * kb + 8 * h)) & 0xFFu);
+    mlp_tiles64(x, W, j, h, acc);
+    const bool in = h == 0;
+    const bf16x4 zero = {0, 0, 0, 0};
+    const bf16x4 xt = in ? *(const bf16x4*)(row + 64) : zero;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const int n0 = 16 * t + 4 * h;
-        const float4 b = *(const float4*)(bias + n0);  // one LDS read for four biases
-        *(bf16x4*)(row + mlp_col(n0)) = to_bf16x4(gelu_f(acc[t][0] + b.x), gelu_f(acc[t][1] + b.y),
-                                                  gelu_f(acc[t][2] + b.z), gelu_f(acc[t][3] + b.w));
+        const bf16x4 w = in ? *(const bf16x4*)(W + (16 * t + j) * kMlpStride + 64) : zero;
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w, xt, acc[t], 0, 0, 0);
+    }
+}
+
+// A hidden layer's activations bf16(gelu(acc + b)) as the next layer's B
+// operand: x[k] holds neurons 16 (2 k) + 4 h + r, then 16 (2 k + 1) + 4 h + r
+// (r = 0..3), i.e. the columns 32 k + 8 h .. + 7 in mlp_col order.
+__device__ __forceinline__ void mlp_activate(const f32x4 acc[4], const float* __restrict__ bias, int h, bf16x8 x[2])
+{
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        bf16x4 half[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int t = 2 * kb + u;
+            const float4 b = *(const float4*)(bias + 16 * t + 4 * h);  // one LDS read for four biases
+            half[u] = to_bf16x4(gelu_f(acc[t][0] + b.x), gelu_f(acc[t][1] + b.y), gelu_f(acc[t][2] + b.z),
+                                gelu_f(acc[t][3] + b.w));
+        }
+        x[kb] = __builtin_shufflevector(half[0], half[1], 0, 1, 2, 3, 4, 5, 6, 7);
     }
 }
 
 // Third hidden layer and output neuron for a 16-row block: y = w4 .
-// bf16(gelu(row * W3^T + b3)) + b4 of row j, in every lane j + 16 h.  Each
-// lane sums its 16 neurons; the four lanes of a row add their sums.
-__device__ __forceinline__ float mlp_last_layer(const uint16_t* block, const uint16_t* W3, const float* __restrict__ small, int lane)
+// bf16(gelu(x * W3^T + b3)) + b4 of row j, in every lane j + 16 h, x the
+// second layer's activations in the lane's registers.  Each lane sums its 16
+// neurons; the four lanes of a row add their sums.
+__device__ __forceinline__ float mlp_last_layer(const bf16x8 x[2], const uint16_t* W3, const float* __restrict__ small, int j,
+                                                int h)
 {
-    const int j = lane & 15, h = lane >> 4;
     f32x4 acc[4];
-    mlp_tiles<kMlpHid>(block + j * kMlpStride, W3, j, h, acc);
+    mlp_tiles64(x, W3, j, h, acc);
     float part = 0.0f;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -453,8 +469,8 @@ __device__ inline void mlp_stage_weights(const MlpNet& m, uint16_t* w, int tid, 
 
 // Forward pass of the first n_rows (0..64) feature rows of the wave's tile xs
 // (LDS, kMlpTileElems: 64 x kMlpStride rows as mlp_feature_bits writes them,
-// overwritten by the hidden activations, then 64 floats of outputs), in
-// ceil(n_rows / 16) blocks of 16 rows, with the weights staged in LDS by
+// then 64 floats of outputs), in ceil(n_rows / 16) blocks of 16 rows (the
+// hidden activations in registers), with the weights staged in LDS by
 // mlp_stage_weights (w) and the small operands by mlp_stage_small.  Returns
 // the output of row `row` (any lane; rows at or beyond n_rows give 0).  Must
 // be called by all 64 lanes together with the same n_rows.
@@ -462,6 +478,7 @@ __device__ __forceinline__ float mlp_forward_rows(const uint16_t* w, uint16_t* x
                                          int row)
 {
     float* ys = (float*)(xs + 64 * kMlpStride);
+    const int j = lane & 15, h = lane >> 4;
     wave_sync();  // the feature rows written by other lanes
     const int nb = (n_rows + 15) >> 4;
 #pragma unroll 1
@@ -474,9 +491,13 @@ __device__ __forceinline__ float mlp_forward_rows(const uint16_t* w, uint16_t* x
         int off = 0;
         asm volatile("" : "+s"(off));
         const uint16_t* wb = w + off;
-        mlp_layer<kMlpInPad, true>(block, wb, small + kMlpSmallB1, lane);
-        mlp_layer<kMlpHid>(block, wb + kMlpWElems, small + kMlpSmallB2, lane);
-        const float y = mlp_last_layer(block, wb + 2 * kMlpWElems, small, lane);
+        f32x4 acc[4];
+        bf16x8 x[2];
+        mlp_tiles_first(block + j * kMlpStride, wb, j, h, acc);
+        mlp_activate(acc, small + kMlpSmallB1, h, x);
+        mlp_tiles64(x, wb + kMlpWElems, j, h, acc);
+        mlp_activate(acc, small + kMlpSmallB2, h, x);
+        const float y = mlp_last_layer(x, wb + 2 * kMlpWElems, small, j, h);
         if (lane < 16) ys[16 * b + lane] = y;
     }
     wave_sync();

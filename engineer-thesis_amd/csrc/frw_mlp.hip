@@ -17,7 +17,7 @@ struct frw_mlp {
     int K0 = 0, Lmax = 0, Vmax = 0, Pmax = 0;  // Pmax = max(Lmax, Vmax)
     std::vector<int64_t> occ_off;  // bitmap word offsets, levels 0..Pmax
     std::vector<int64_t> sat_off;  // summed-area-table entry offsets, levels 0..Vmax
-    uint32_t* d_occ = nullptr;
+    uint64_t* d_occ = nullptr;     // tiles of 8x8 cells
     uint32_t* d_sat = nullptr;
     int64_t* d_occ_off = nullptr;  // device copies of the offsets
     int64_t* d_sat_off = nullptr;
@@ -48,12 +48,13 @@ constexpr int kBfW4 = kBfW3 + kMlpHid * kMlpHid;
 constexpr int kBfCount = kBfW4 + kMlpHid;
 
 // ---------------------------------------------------------------- occupancy
+// Levels are stored in tiles of 8x8 cells (occ_tiles, one 64-bit word each).
 // One lane per rect: sets the bits of every cell of level L the closed rect
-// meets, one word mask per bitmap row segment.  Only the finest level is
-// marked this way; occ_reduce derives the coarser ones (marking every level
-// here had put every rect's atomics on the few words of the coarse levels).
+// meets, one 64-bit mask per tile.  Only the finest level is marked this
+// way; occ_reduce derives the coarser ones (marking every level here had put
+// every rect's atomics on the few words of the coarse levels).
 __global__ void occ_mark(const int4* __restrict__ rects, int n, int4 B, int K0, int L,
-                         const int64_t* __restrict__ off, uint32_t* __restrict__ occ)
+                         const int64_t* __restrict__ off, uint64_t* __restrict__ occ)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
@@ -64,78 +65,68 @@ __global__ void occ_mark(const int4* __restrict__ rects, int n, int4 B, int K0, 
     const int64_t nl = (int64_t)1 << L;
     const int64_t i0 = x0 >> sh, i1 = min(x1 >> sh, nl - 1);
     const int64_t j0 = y0 >> sh, j1 = min(y1 >> sh, nl - 1);
-    uint32_t* base = occ + off[L];
-    for (int64_t j = j0; j <= j1; ++j) {
-        int64_t a = j * nl + i0;
-        const int64_t e = j * nl + i1;  // inclusive
-        while (a <= e) {
-            const int64_t wd = a >> 5;
-            const int b0 = (int)(a & 31);
-            const int b1 = (int)min<int64_t>(31, b0 + (e - a));
-            const uint32_t mask = (b1 == 31 ? 0xFFFFFFFFu : ((1u << (b1 + 1)) - 1u)) & ~((1u << b0) - 1u);
-            atomicOr(base + wd, mask);
-            a += b1 - b0 + 1;
+    const int64_t T = occ_tiles(L);
+    uint64_t* base = occ + off[L];
+    for (int64_t ty = j0 >> 3; ty <= j1 >> 3; ++ty) {
+        const int r0 = (int)max<int64_t>(j0 - 8 * ty, 0), r1 = (int)min<int64_t>(j1 - 8 * ty, 7);  // rows r0..r1
+        const uint64_t rows = occ_low_bytes(r1 + 1) & ~occ_low_bytes(r0);
+        for (int64_t tx = i0 >> 3; tx <= i1 >> 3; ++tx) {
+            const int c0 = (int)max<int64_t>(i0 - 8 * tx, 0), c1 = (int)min<int64_t>(i1 - 8 * tx, 7);
+            const uint32_t cols = ((2u << c1) - 1u) & ~((1u << c0) - 1u);  // columns c0..c1
+            atomicOr((unsigned long long*)(base + ty * T + tx),
+                     (unsigned long long)(rows & (0x0101010101010101ull * cols)));
         }
     }
 }
 
-// One lane per word of level l (n x n cells) from level l + 1 (2n x 2n):
-// cell (i, j) is occupied when one of its four children (2i + a, 2j + b) is.
-// The children's half-open cells tile the parent's, and the finest level's
-// cell ranges, shifted right by one bit, are the parent level's, so this
-// equals marking level l directly.  For n >= 32 a word holds 32 cells of one
-// row: the OR of the two child rows' 64 bits, adjacent pairs ORed, the even
-// bits compacted; smaller levels go cell by cell.
-__global__ void occ_reduce(const uint32_t* __restrict__ child, int64_t n, uint32_t* __restrict__ parent)
+// One lane per tile of level l from the (up to) 2x2 tiles of level l + 1
+// under it: cell (i, j) is occupied when one of its four children (2i + a,
+// 2j + b) is.  The children's half-open cells tile the parent's, and the
+// finest level's cell ranges, shifted right by one bit, are the parent
+// level's, so this equals marking level l directly.  A child tile's 8x8
+// cells become 4x4: rows ORed in pairs, then columns, the even bits of the
+// even rows compacted into one nibble per row, placed in the parent tile's
+// quadrant.  Child tiles beyond level l + 1's grid (levels below 3) are empty.
+__global__ void occ_reduce(const uint64_t* __restrict__ child, int l, uint64_t* __restrict__ parent)
 {
+    const int64_t Tp = occ_tiles(l), Tc = occ_tiles(l + 1);
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t cells = n * n, words = (cells + 31) / 32, nc = 2 * n;
-    if (w >= words) return;
-    uint32_t out = 0u;
-    if (n >= 32) {
-        const int64_t b = 32 * w, j = b / n, i0 = b % n;
-        const uint32_t* r0 = child + ((2 * j) * nc + 2 * i0) / 32;
-        const uint32_t* r1 = child + ((2 * j + 1) * nc + 2 * i0) / 32;
-        uint64_t x = ((uint64_t)(r0[1] | r1[1]) << 32) | (uint64_t)(r0[0] | r1[0]);
-        x = (x | (x >> 1)) & 0x5555555555555555ull;
-        x = (x | (x >> 1)) & 0x3333333333333333ull;
-        x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
-        x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
-        x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
-        x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
-        out = (uint32_t)x;
-    } else {
-        for (int k = 0; k < 32; ++k) {
-            const int64_t b = 32 * w + k;
-            if (b >= cells) break;
-            const int64_t j = b / n, i = b % n;
-            uint32_t any = 0u;
-            for (int q = 0; q < 4; ++q) {
-                const int64_t cb = (2 * j + (q >> 1)) * nc + 2 * i + (q & 1);
-                any |= (child[cb >> 5] >> (cb & 31)) & 1u;
-            }
-            out |= any << k;
-        }
+    if (w >= Tp * Tp) return;
+    const int64_t TX = w % Tp, TY = w / Tp;
+    uint64_t out = 0ull;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int qx = q & 1, qy = q >> 1;
+        const int64_t cx = 2 * TX + qx, cy = 2 * TY + qy;
+        if (cx >= Tc || cy >= Tc) continue;
+        uint64_t x = child[cy * Tc + cx];
+        x |= x >> 8;  // rows 2v and 2v + 1 in row 2v
+        x |= x >> 1;  // columns 2u and 2u + 1 in column 2u
+        x &= 0x0055005500550055ull;
+        x = (x | (x >> 1)) & 0x0033003300330033ull;
+        x = (x | (x >> 2)) & 0x000F000F000F000Full;  // parent row v: the nibble of byte 2v
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out |= ((x >> (16 * v)) & 0xFull) << ((4 * qy + v) * 8 + 4 * qx);
     }
     parent[w] = out;
 }
 
 // ---------------------------------------------------------------- summed-area tables
-// Row pass, one block of 256 lanes per grid row j: S[(j+1)(n+1) + i + 1] =
+// Row pass, one block of 256 lanes per grid row j of a level's tiles: S[(j+1)(n+1) + i + 1] =
 // occupied cells (a, j) with a <= i; S[(j+1)(n+1)] = 0.  Row 0 is zeroed by
 // the caller.
-__global__ void __launch_bounds__(256) sat_rows(const uint32_t* __restrict__ bits, int64_t n,
+__global__ void __launch_bounds__(256) sat_rows(const uint64_t* __restrict__ tiles, int l,
                                                 uint32_t* __restrict__ S)
 {
     __shared__ uint32_t part[256];
+    const int64_t n = (int64_t)1 << l, T = occ_tiles(l);
     const int64_t j = blockIdx.x;
     const int t = threadIdx.x;
     const int64_t c = (n + 255) / 256;
     const int64_t a = min(n, (int64_t)t * c), b = min(n, a + c);
-    auto bit = [&](int64_t i) -> uint32_t {
-        const int64_t idx = j * n + i;
-        return (bits[idx >> 5] >> (idx & 31)) & 1u;
-    };
+    const uint64_t* trow = tiles + (j >> 3) * T;
+    const int sh = 8 * (int)(j & 7);
+    auto bit = [&](int64_t i) -> uint32_t { return (uint32_t)(trow[i >> 3] >> (sh + (int)(i & 7))) & 1u; };
     uint32_t cnt = 0;
     for (int64_t i = a; i < b; ++i) cnt += bit(i);
     part[t] = cnt;
@@ -701,16 +692,15 @@ int frw_mlp_create(const frw_tree* t, uint64_t seed, int32_t max_level, frw_mlp*
     int64_t words = 0, entries = 0;
     for (int l = 0; l <= m->Pmax; ++l) {
         m->occ_off.push_back(words);
-        const int64_t bits = ((int64_t)1 << l) * ((int64_t)1 << l);
-        words += (bits + 31) / 32 + 1;  // +1: the window reader loads word pairs
+        words += occ_tiles(l) * occ_tiles(l);
     }
     for (int l = 0; l <= m->Vmax; ++l) {
         m->sat_off.push_back(entries);
         entries += (((int64_t)1 << l) + 1) * (((int64_t)1 << l) + 1);
     }
-    int rc = check_hip(hipMalloc(&m->d_occ, words * 4), "hipMalloc occupancy");
+    int rc = check_hip(hipMalloc(&m->d_occ, words * 8), "hipMalloc occupancy");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_sat, entries * 4), "hipMalloc summed-area tables");
-    if (rc == FRW_OK) rc = check_hip(hipMemset(m->d_occ, 0, words * 4), "memset occupancy");
+    if (rc == FRW_OK) rc = check_hip(hipMemset(m->d_occ, 0, words * 8), "memset occupancy");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_occ_off, sizeof(int64_t) * m->occ_off.size()), "hipMalloc off");
     if (rc == FRW_OK) rc = check_hip(hipMalloc(&m->d_sat_off, sizeof(int64_t) * m->sat_off.size()), "hipMalloc off");
     if (rc == FRW_OK)
@@ -725,8 +715,8 @@ int frw_mlp_create(const frw_tree* t, uint64_t seed, int32_t max_level, frw_mlp*
         rc = check_hip(hipGetLastError(), "occ_mark");
     }
     for (int l = m->Pmax - 1; l >= 0 && rc == FRW_OK; --l) {
-        const int64_t n = (int64_t)1 << l, words = (n * n + 31) / 32;
-        occ_reduce<<<blocks(words, 256), 256>>>(m->d_occ + m->occ_off[l + 1], n, m->d_occ + m->occ_off[l]);
+        const int64_t tiles = occ_tiles(l) * occ_tiles(l);
+        occ_reduce<<<blocks(tiles, 256), 256>>>(m->d_occ + m->occ_off[l + 1], l, m->d_occ + m->occ_off[l]);
         rc = check_hip(hipGetLastError(), "occ_reduce");
     }
     for (int l = 0; l <= m->Vmax && rc == FRW_OK; ++l) {
@@ -734,7 +724,7 @@ int frw_mlp_create(const frw_tree* t, uint64_t seed, int32_t max_level, frw_mlp*
         uint32_t* S = m->d_sat + m->sat_off[l];
         rc = check_hip(hipMemset(S, 0, (size_t)(n + 1) * 4), "memset table row 0");
         if (rc != FRW_OK) break;
-        sat_rows<<<(unsigned)n, 256>>>(m->d_occ + m->occ_off[l], n, S);
+        sat_rows<<<(unsigned)n, 256>>>(m->d_occ + m->occ_off[l], l, S);
         sat_cols<<<blocks(n + 1, 256), 256>>>(n, S);
         rc = check_hip(hipGetLastError(), "summed-area tables");
     }
@@ -809,11 +799,28 @@ int frw_mlp_set_params(frw_mlp* m, const float* in, int32_t n)
 int frw_mlp_occupancy(const frw_mlp* m, int32_t level, uint32_t* out_words, int64_t n_words)
 {
     if (!m || !out_words || level < 0 || level > m->Pmax) return set_error(FRW_E_INVALID, "bad level");
-    const int64_t bits = ((int64_t)1 << level) * ((int64_t)1 << level);
-    const int64_t need = (bits + 31) / 32;
+    const int64_t n = (int64_t)1 << level, T = occ_tiles(level);
+    const int64_t need = (n * n + 31) / 32;
     if (n_words < need) return set_error(FRW_E_INVALID, "output too small");
-    return check_hip(hipMemcpy(out_words, m->d_occ + m->occ_off[level], need * 4, hipMemcpyDeviceToHost),
-                     "copy occupancy");
+    std::vector<uint64_t> tiles((size_t)(T * T));
+    const int rc = check_hip(hipMemcpy(tiles.data(), m->d_occ + m->occ_off[level], tiles.size() * 8,
+                                       hipMemcpyDeviceToHost), "copy occupancy");
+    if (rc != FRW_OK) return rc;
+    // the tiles as the row-major bitmap (bit j n + i = cell (i, j))
+    std::memset(out_words, 0, (size_t)need * 4);
+    if (n >= 8) {
+        // a tile's row is one byte of the row-major bitmap (n is a multiple of 8)
+        uint8_t* bytes = reinterpret_cast<uint8_t*>(out_words);
+        for (int64_t ty = 0; ty < T; ++ty)
+            for (int64_t tx = 0; tx < T; ++tx)
+                for (int y = 0; y < 8; ++y)
+                    bytes[((8 * ty + y) * n + 8 * tx) / 8] = (uint8_t)(tiles[(size_t)(ty * T + tx)] >> (8 * y));
+    } else {
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t i = 0; i < n; ++i)
+                if ((tiles[0] >> (8 * j + i)) & 1u) out_words[(j * n + i) >> 5] |= 1u << ((j * n + i) & 31);
+    }
+    return FRW_OK;
 }
 
 int frw_mlp_levels(const frw_mlp* m, int32_t out[4])
