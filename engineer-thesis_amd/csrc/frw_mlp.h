@@ -148,25 +148,26 @@ __device__ inline float erf_as(float x)
 }
 
 // GELU(z) = z Phi(z) = max(z, 0) - t Phi(-t), t = |z|, with Phi(-t) replaced by
-// a degree-10 polynomial in x = 1 - t / 4.5 that vanishes at t = 4.5 (where
-// t Phi(-t) < 1.6e-5), t clamped there: FMA and min/max only, no
-// transcendental; within 1.7e-5 of the exact GELU, far inside a bf16
-// activation's rounding.  Coefficients from fit_gelu.py (a minimax fit).
+// x P(x), P a degree-8 polynomial in x = 1 - t / 4.5 (x P vanishes at t = 4.5,
+// where t Phi(-t) < 1.6e-5), t clamped there: GELU = fma(-(t x), P(x),
+// max(z, 0)), 13 FMA / min / max / mul instructions, no transcendental;
+// within 1.7e-5 of the exact GELU, far inside a bf16 activation's rounding.
+// Coefficients from fit_gelu.py (a minimax fit).
 __device__ inline float gelu_f(float z)
 {
     const float t = fminf(fabsf(z), 4.5f);
     const float x = __builtin_fmaf(t, -1.0f / 4.5f, 1.0f);
-    float p = -7.503951923e-03f;
-    p = __builtin_fmaf(p, x, 1.011371091e+01f);
-    p = __builtin_fmaf(p, x, -3.778662722e+01f);
-    p = __builtin_fmaf(p, x, 5.380414907e+01f);
-    p = __builtin_fmaf(p, x, -3.819883533e+01f);
-    p = __builtin_fmaf(p, x, 1.585222026e+01f);
-    p = __builtin_fmaf(p, x, -3.753199773e+00f);
-    p = __builtin_fmaf(p, x, 5.077477714e-01f);
-    p = __builtin_fmaf(p, x, -3.255581238e-02f);
-    p = __builtin_fmaf(p, x, 9.095615919e-04f);
-    return fmaxf(z, 0.0f) - t * (p * x);
+    const float q = t * x;
+    float p = 1.007601249e+01f;
+    p = __builtin_fmaf(p, x, -3.770581612e+01f);
+    p = __builtin_fmaf(p, x, 5.370768662e+01f);
+    p = __builtin_fmaf(p, x, -3.812867467e+01f);
+    p = __builtin_fmaf(p, x, 1.582026759e+01f);
+    p = __builtin_fmaf(p, x, -3.744195627e+00f);
+    p = __builtin_fmaf(p, x, 5.062563132e-01f);
+    p = __builtin_fmaf(p, x, -3.242632146e-02f);
+    p = __builtin_fmaf(p, x, 9.052214061e-04f);
+    return __builtin_fmaf(-q, p, fmaxf(z, 0.0f));
 }
 
 __device__ inline float gelu_grad_f(float z)

@@ -34,8 +34,9 @@ struct frw_mlp {
     int64_t adam_step = 0;
     // training set
     int64_t n_samples = 0;
-    uint16_t* d_X = nullptr;       // [n][72] bf16
-    float* d_t = nullptr;          // [n] targets
+    int64_t cap_samples = 0;       // rows allocated (grown geometrically)
+    uint16_t* d_X = nullptr;       // [cap][72] bf16
+    float* d_t = nullptr;          // [cap] targets
 };
 
 namespace frw {
@@ -124,11 +125,22 @@ __global__ void __launch_bounds__(256) sat_rows(const uint64_t* __restrict__ til
     const int t = threadIdx.x;
     const int64_t c = (n + 255) / 256;
     const int64_t a = min(n, (int64_t)t * c), b = min(n, a + c);
+    // row j of the level is byte j mod 8 of its tile row's words: the lane
+    // reads one word per 8 cells of its range [a, b)
     const uint64_t* trow = tiles + (j >> 3) * T;
     const int sh = 8 * (int)(j & 7);
-    auto bit = [&](int64_t i) -> uint32_t { return (uint32_t)(trow[i >> 3] >> (sh + (int)(i & 7))) & 1u; };
+    const int64_t k0 = a >> 3, k1 = b > a ? ((b - 1) >> 3) + 1 : k0;
+    auto cells = [&](int64_t k, int* lo, int* hi) -> uint32_t {  // the tile's byte and the lane's cells in it
+        *lo = (int)max<int64_t>(a - 8 * k, 0);
+        *hi = (int)min<int64_t>(b - 8 * k, 8);
+        return (uint32_t)(trow[k] >> sh) & 0xFFu;
+    };
     uint32_t cnt = 0;
-    for (int64_t i = a; i < b; ++i) cnt += bit(i);
+    for (int64_t k = k0; k < k1; ++k) {
+        int lo, hi;
+        const uint32_t by = cells(k, &lo, &hi);
+        cnt += (uint32_t)__popc(by & ((1u << hi) - 1u) & ~((1u << lo) - 1u));
+    }
     part[t] = cnt;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
@@ -140,9 +152,13 @@ __global__ void __launch_bounds__(256) sat_rows(const uint64_t* __restrict__ til
     uint32_t run = part[t] - cnt;
     uint32_t* row = S + (j + 1) * (n + 1);
     if (t == 0) row[0] = 0u;
-    for (int64_t i = a; i < b; ++i) {
-        run += bit(i);
-        row[i + 1] = run;
+    for (int64_t k = k0; k < k1; ++k) {
+        int lo, hi;
+        const uint32_t by = cells(k, &lo, &hi);
+        for (int u = lo; u < hi; ++u) {
+            run += (by >> u) & 1u;
+            row[8 * k + u + 1] = run;
+        }
     }
 }
 
@@ -901,28 +917,34 @@ int frw_mlp_add_samples(frw_mlp* m, const frw_mlp* features_of, const double* pt
     if (!m || !pts || !h || !dstar || n <= 0) return set_error(FRW_E_INVALID, "bad arguments");
     const frw_mlp* src = features_of ? features_of : m;
     const int64_t old_n = m->n_samples, tot = old_n + n;
-    uint16_t* X = nullptr;
-    float* T = nullptr;
-    int rc = check_hip(hipMalloc(&X, (size_t)tot * kMlpInPad * 2), "hipMalloc samples");
-    if (rc == FRW_OK) rc = check_hip(hipMalloc(&T, (size_t)tot * 4), "hipMalloc targets");
-    if (rc == FRW_OK && old_n) {
-        rc = check_hip(hipMemcpy(X, m->d_X, (size_t)old_n * kMlpInPad * 2, hipMemcpyDeviceToDevice), "copy samples");
-        if (rc == FRW_OK) rc = check_hip(hipMemcpy(T, m->d_t, (size_t)old_n * 4, hipMemcpyDeviceToDevice), "copy targets");
+    int rc = FRW_OK;
+    if (tot > m->cap_samples) {
+        // at least double the capacity, so that appending in many calls
+        // copies the rows already held a bounded number of times
+        const int64_t cap = std::max(tot, 2 * m->cap_samples);
+        uint16_t* X = nullptr;
+        float* T = nullptr;
+        rc = check_hip(hipMalloc(&X, (size_t)cap * kMlpInPad * 2), "hipMalloc samples");
+        if (rc == FRW_OK) rc = check_hip(hipMalloc(&T, (size_t)cap * 4), "hipMalloc targets");
+        if (rc == FRW_OK && old_n) {
+            rc = check_hip(hipMemcpy(X, m->d_X, (size_t)old_n * kMlpInPad * 2, hipMemcpyDeviceToDevice), "copy samples");
+            if (rc == FRW_OK) rc = check_hip(hipMemcpy(T, m->d_t, (size_t)old_n * 4, hipMemcpyDeviceToDevice), "copy targets");
+        }
+        if (rc != FRW_OK) {
+            if (X) (void)hipFree(X);
+            if (T) (void)hipFree(T);
+            return rc;
+        }
+        (void)hipFree(m->d_X);
+        (void)hipFree(m->d_t);
+        m->d_X = X;
+        m->d_t = T;
+        m->cap_samples = cap;
     }
-    if (rc == FRW_OK) {
-        features_kernel<<<blocks(n, 256), 256>>>(mlp_pyramid(src), pts, h, n, X + old_n * kMlpInPad, dstar,
-                                                 T + old_n, nullptr);
-        rc = check_hip(hipDeviceSynchronize(), "features");
-    }
-    if (rc != FRW_OK) {
-        if (X) (void)hipFree(X);
-        if (T) (void)hipFree(T);
-        return rc;
-    }
-    (void)hipFree(m->d_X);
-    (void)hipFree(m->d_t);
-    m->d_X = X;
-    m->d_t = T;
+    features_kernel<<<blocks(n, 256), 256>>>(mlp_pyramid(src), pts, h, n, m->d_X + old_n * kMlpInPad, dstar,
+                                             m->d_t + old_n, nullptr);
+    rc = check_hip(hipDeviceSynchronize(), "features");
+    if (rc != FRW_OK) return rc;
     m->n_samples = tot;
     return FRW_OK;
 }
@@ -930,11 +952,7 @@ int frw_mlp_add_samples(frw_mlp* m, const frw_mlp* features_of, const double* pt
 int frw_mlp_set_samples(frw_mlp* m, const double* pts, const double* h, const double* dstar, int64_t n)
 {
     if (!m || !pts || !h || !dstar || n <= 0) return set_error(FRW_E_INVALID, "bad arguments");
-    (void)hipFree(m->d_X);
-    (void)hipFree(m->d_t);
-    m->d_X = nullptr;
-    m->d_t = nullptr;
-    m->n_samples = 0;
+    m->n_samples = 0;  // the rows are rewritten in place (the capacity is kept)
     return frw_mlp_add_samples(m, nullptr, pts, h, dstar, n);
 }
 
@@ -994,16 +1012,21 @@ int frw_walk_harvest(const frw_tree* t, const frw_walk_params* p, double* pts, d
     // The records thus come in walk and hop order whatever the scheduling,
     // and a capacity below the total keeps the first walks' records.
     const int64_t nb = t->n_rects + 2, nw = q.n_walks;
+    // one allocation: walk offsets (8 B each), bins and the hop total (8 B
+    // each), hits and hop counts (4 B each)
+    char* scratch = nullptr;
+    rc = check_hip(hipMalloc(&scratch, (size_t)(nw * 8 + (nb + 1) * 8 + nw * 8)), "hipMalloc harvest scratch");
+    unsigned long long* d_off = nullptr;
     int64_t *d_counts = nullptr, *d_hops = nullptr;
     int32_t *d_hit = nullptr, *d_hp = nullptr;
-    unsigned long long* d_off = nullptr;
-    rc = check_hip(hipMalloc(&d_counts, nb * 8), "hipMalloc");
-    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hops, 8), "hipMalloc");
-    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hit, nw * 4), "hipMalloc");
-    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_hp, nw * 4), "hipMalloc");
-    if (rc == FRW_OK) rc = check_hip(hipMalloc(&d_off, nw * 8), "hipMalloc");
-    if (rc == FRW_OK) rc = check_hip(hipMemset(d_counts, 0, nb * 8), "memset");
-    if (rc == FRW_OK) rc = check_hip(hipMemset(d_hops, 0, 8), "memset");
+    if (rc == FRW_OK) {
+        d_off = reinterpret_cast<unsigned long long*>(scratch);
+        d_counts = reinterpret_cast<int64_t*>(scratch + nw * 8);
+        d_hops = d_counts + nb;
+        d_hit = reinterpret_cast<int32_t*>(d_hops + 1);
+        d_hp = d_hit + nw;
+        rc = check_hip(hipMemset(d_counts, 0, (nb + 1) * 8), "memset");
+    }
     if (rc == FRW_OK) {
         const frw_walk_outputs o{d_counts, d_hops, d_hit, d_hp};
         rc = launch_walk(t, &q, &o, nullptr);
@@ -1043,7 +1066,7 @@ int frw_walk_harvest(const frw_tree* t, const frw_walk_params* p, double* pts, d
     }
     if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "harvest pass 2");
     if (rc == FRW_OK && n_out) *n_out = (int64_t)std::min<uint64_t>(total, (uint64_t)capacity);
-    for (void* x : {(void*)d_counts, (void*)d_hops, (void*)d_hit, (void*)d_hp, (void*)d_off}) (void)hipFree(x);
+    if (scratch) (void)hipFree(scratch);
     return rc;
 }
 

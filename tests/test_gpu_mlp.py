@@ -249,6 +249,43 @@ def trained(setup, harvested):
     return mlp2, first, last
 
 
+def test_samples_in_pieces_train_alike(setup, harvested):
+    """The training set grows geometrically: samples appended in three calls
+    (the second and third beyond the capacity the first left) train to the
+    same parameters as the same samples set at once, and a smaller set
+    written over a larger one (the capacity kept) trains as on a fresh
+    network."""
+    lay, rects, bnds, tree, mlp = setup
+    pts, h, ds = harvested
+    n = pts.shape[0]
+    assert n >= 4096
+    cuts = [0, n // 7, n // 7 + 1000, n]
+
+    def trained_params(feed, seed=3):
+        m = frw.Mlp(tree, seed=7)
+        feed(m)
+        loss = m.train(epochs=1, batch=1024, lr=2e-3, seed=seed)
+        return m.params(), loss
+
+    def pieces(m):
+        for lo, hi in zip(cuts[:-1], cuts[1:]):
+            m.add_samples(pts[lo:hi], h[lo:hi], ds[lo:hi])
+
+    whole, lw = trained_params(lambda m: m.set_samples(pts, h, ds))
+    parts, lp = trained_params(pieces)
+    assert np.array_equal(whole, parts) and lw == lp
+    half = n // 2
+
+    def shrink(m):
+        m.set_samples(pts, h, ds)
+        m.set_samples(pts[:half], h[:half], ds[:half])
+
+    small, ls = trained_params(shrink)
+    fresh, lf = trained_params(lambda m: m.set_samples(pts[:half], h[:half], ds[:half]))
+    assert np.array_equal(small, fresh) and ls == lf
+    assert not np.array_equal(small, whole)
+
+
 def test_training_lowers_loss(trained):
     mlp2, first, last = trained
     assert np.isfinite(first) and np.isfinite(last)
