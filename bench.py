@@ -236,22 +236,36 @@ def train_batched(frw, torch, trees, harvest, pyr_of, n_pairs, epochs, seed, net
     """Trains `net` (one of the trees' proposal objects, whose weights every
     job uses) on pairs harvested (in walk order) from exact walks of every
     (tree, master) of `harvest`, walk ids from 3 * 2^30 (disjoint from the
-    measured ones), features from each tree's own pyramid."""
+    measured ones), features from each tree's own pyramid.  One batched
+    harvest (two launches) serves every (tree, master); a job short of its
+    share of pairs is harvested again with proportionally more walks."""
+    import numpy as np
     t0 = time.perf_counter()
-    per = max(1024, n_pairs // len(harvest))
+    n_h = len(harvest)
+    per = max(1024, n_pairs // n_h)
+    walks = [max(1000, per // 8)] * n_h
+    cap = 3 * per * n_h
+    pts = torch.empty((cap, 2), dtype=torch.float64, device="cuda")
+    hh = torch.empty(cap, dtype=torch.float64, device="cuda")
+    ds = torch.empty(cap, dtype=torch.float64, device="cuda")
+
+    def harvest_all(walks):
+        hb = frw.Batch(trees, [{"tree": i, "master": m, "n_walks": w, "first_walk": 3 << 30, "job": 0}
+                               for (i, m), w in zip(harvest, walks)])
+        n = hb.harvest(pts, hh, ds, seed=1)
+        return n, np.concatenate([[0], np.cumsum(n)])
+
+    n, offs = harvest_all(walks)
+    if (n < per).any():
+        walks = [w if k >= per else int(w * per / max(k, 1) * 1.2) + 1000 for w, k in zip(walks, n)]
+        n, offs = harvest_all(walks)
     got_total = 0
-    pts = torch.empty((per, 2), dtype=torch.float64, device="cuda")
-    hh = torch.empty(per, dtype=torch.float64, device="cuda")
-    ds = torch.empty(per, dtype=torch.float64, device="cuda")
-    for i, m in harvest:
-        walks = max(1000, per // 8)
-        got = frw.walk_harvest(trees[i], m, walks, pts, hh, ds, seed=1, first_walk=3 << 30)
-        if got < per:
-            got = frw.walk_harvest(trees[i], m, int(walks * per / max(got, 1) * 1.2) + 1000, pts, hh, ds,
-                                   seed=1, first_walk=3 << 30)
-        got = min(got, per)
-        net.add_samples(pts[:got], hh[:got], ds[:got], features_of=pyr_of[i])
+    for k, (i, m) in enumerate(harvest):
+        got = int(min(n[k], per))
+        lo = int(offs[k])
+        net.add_samples(pts[lo:lo + got], hh[lo:lo + got], ds[lo:lo + got], features_of=pyr_of[i])
         got_total += got
+    del pts, hh, ds
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     loss = net.train(epochs=epochs, batch=batch, lr=lr, seed=seed)

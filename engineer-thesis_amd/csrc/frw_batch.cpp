@@ -328,6 +328,118 @@ int frw_batch_walk(frw_batch* b, const frw_batch_params* p, int64_t* counts, int
     return batch_walk(b, p, nullptr, 0.0f, nullptr, counts, hops, (hipStream_t)stream);
 }
 
+int frw_batch_harvest(frw_batch* b, const frw_batch_params* p, double* pts, double* h, double* dstar,
+                      int64_t capacity, int64_t* n_per_job)
+{
+    if (!b || !p || !pts || !h || !dstar || capacity <= 0 || !n_per_job)
+        return set_error(FRW_E_INVALID, "bad arguments");
+    if (p->max_hops < 0 || p->max_hops > 0xFFFFFFFFll) return set_error(FRW_E_RANGE, "bad max_hops");
+    frw_batch_params q = *p;
+    q.mode = FRW_MODE_EXACT;
+    q.use_graph = 0;
+    hipStream_t s = nullptr;
+    bool capturing = false;
+    int rc = order_before(b->order, s, &capturing);
+    if (rc != FRW_OK) return rc;
+    if (capturing) {
+        (void)order_after(b->order, s, capturing);
+        return set_error(FRW_E_INVALID, "harvest is synchronous: not under stream capture");
+    }
+    // Two passes over the same walks of every job, as frw_walk_harvest does
+    // for one: the first gives each walk's hop count and end, hence its
+    // number of records (hops - 1 when it ended on a rect, hops otherwise);
+    // the second writes the launch's walk g (the jobs' walks one after
+    // another) at the exclusive prefix sum of those counts.  Records beyond
+    // the capacity are dropped, so a small capacity keeps the first jobs'.
+    const uint64_t nw = b->total_walks;
+    const size_t nj = b->jobs.size();
+    char* scratch = nullptr;
+    unsigned long long* d_off = nullptr;
+    int64_t *d_counts = nullptr, *d_hops = nullptr;
+    int32_t *d_hit = nullptr, *d_hp = nullptr;
+    std::vector<WalkJob> tab(nj);
+    std::vector<int32_t> hit, hp;
+    std::vector<unsigned long long> off;
+    uint64_t total = 0;
+    try {
+        rc = check_hip(hipMalloc(&scratch, (size_t)(nw * 8 + (b->counts_size + (int64_t)nj) * 8 + nw * 8)),
+                       "hipMalloc harvest scratch");
+        if (rc == FRW_OK) {
+            d_off = reinterpret_cast<unsigned long long*>(scratch);
+            d_counts = reinterpret_cast<int64_t*>(scratch + nw * 8);
+            d_hops = d_counts + b->counts_size;
+            d_hit = reinterpret_cast<int32_t*>(d_hops + nj);
+            d_hp = d_hit + nw;
+            rc = check_hip(hipMemset(d_counts, 0, (size_t)(b->counts_size + (int64_t)nj) * 8), "memset");
+        }
+        if (rc == FRW_OK) {
+            // the job table with each job's per-walk outputs
+            uint64_t begin = 0;
+            for (size_t j = 0; j < nj; ++j) {
+                const frw_batch_job& bj = b->jobs[j];
+                const frw_tree* t = b->trees[bj.tree];
+                WalkJob& w = tab[j];
+                std::memset(&w, 0, sizeof(w));
+                w.tree = dev_view(t);
+                w.master = t->host_rects[bj.master];
+                w.master_id = bj.master;
+                w.counts = (unsigned long long*)(d_counts + b->count_off[j]);
+                w.hops = (unsigned long long*)(d_hops + j);
+                w.hit_out = d_hit + begin;
+                w.hops_out = d_hp + begin;
+                w.walk_begin = begin;
+                w.n_walks = (uint64_t)bj.n_walks;
+                w.first_walk = (uint64_t)bj.first_walk;
+                w.job = bj.job;
+                begin += (uint64_t)bj.n_walks;
+            }
+            // the table of later walk launches is uploaded again
+            b->up_counts = b->up_hops = nullptr;
+            drop_graph(b);
+            rc = check_hip(hipMemcpy(b->d_jobs, tab.data(), sizeof(WalkJob) * nj, hipMemcpyHostToDevice),
+                           "upload job table");
+        }
+        if (rc == FRW_OK) rc = launch_walk_jobs(b->d_jobs, launch_of(b, &q), s);
+        if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "harvest pass 1");
+        if (rc == FRW_OK) {
+            hit.resize((size_t)nw);
+            hp.resize((size_t)nw);
+            off.resize((size_t)nw);
+            rc = check_hip(hipMemcpy(hit.data(), d_hit, nw * 4, hipMemcpyDeviceToHost), "copy hits");
+            if (rc == FRW_OK) rc = check_hip(hipMemcpy(hp.data(), d_hp, nw * 4, hipMemcpyDeviceToHost), "copy hops");
+        }
+        if (rc == FRW_OK) {
+            uint64_t g = 0;
+            for (size_t j = 0; j < nj; ++j) {
+                const uint64_t first = total;
+                for (int64_t w = 0; w < b->jobs[j].n_walks; ++w, ++g) {
+                    off[(size_t)g] = total;
+                    total += (uint64_t)hp[(size_t)g] - (hit[(size_t)g] >= 0 ? 1u : 0u);
+                }
+                const uint64_t lo = std::min<uint64_t>(first, (uint64_t)capacity);
+                const uint64_t hi = std::min<uint64_t>(total, (uint64_t)capacity);
+                n_per_job[j] = (int64_t)(hi - lo);
+            }
+            rc = check_hip(hipMemcpy(d_off, off.data(), (size_t)nw * 8, hipMemcpyHostToDevice), "copy offsets");
+        }
+        if (rc == FRW_OK) {
+            WalkLaunch L = launch_of(b, &q);
+            L.h_pts = pts;
+            L.h_prev = h;
+            L.h_dstar = dstar;
+            L.h_off = d_off;
+            L.h_cap = (uint64_t)capacity;
+            rc = launch_walk_jobs(b->d_jobs, L, s);
+        }
+        if (rc == FRW_OK) rc = check_hip(hipDeviceSynchronize(), "harvest pass 2");
+    } catch (...) {
+        rc = set_error(FRW_E_NOMEM, "host allocation failed");
+    }
+    if (scratch) (void)hipFree(scratch);
+    const int rc2 = order_after(b->order, s, capturing);
+    return rc != FRW_OK ? rc : rc2;
+}
+
 int frw_batch_set_proposal(frw_batch* b, const frw_mlp* const* per_tree, int32_t n_trees)
 {
     if (!b || !per_tree || n_trees != (int32_t)b->trees.size())

@@ -45,8 +45,9 @@ struct WalkArgs {
     int32_t wave_ints;  // LDS per wave in 32-bit words: its lanes' traversal stacks, which the
                         // learned mode's activation tile aliases between traversals
     // harvest: (point, previous half-size, d*) of non-final exact hops, the
-    // k-th record of walk w at slot h_off[w] + k (offsets from a first pass
-    // of the same walks, so the records come in walk and hop order)
+    // k-th record of the launch's walk g (its global index: the jobs' walks
+    // one after another) at slot h_off[g] + k (offsets from a first pass of
+    // the same walks, so the records come in job, walk and hop order)
     double* h_pts;
     double* h_prev;
     double* h_dstar;
@@ -312,7 +313,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
     Counters cnt{0u, 0u, 0u};
     Counters* cp = kStats ? &cnt : nullptr;
     unsigned long long my_nodes = 0, my_rects = 0, my_tabs = 0, my_acc = 0, my_fb = 0;
-    uint64_t my_rec = 0;  // harvest: records of the lane's current walk
+    uint64_t my_rec = 0;  // harvest: slot of the lane's next record
 
     // A lane hands in its tallies (master, escape, hops; weighted sums) for
     // its current job and clears them when it moves to another job and when
@@ -391,7 +392,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
         if (got) {
             active = true;
             hop = 0;
-            if (kHarvest) my_rec = 0;
+            if (kHarvest) my_rec = a.h_off[g];  // by the global walk index (batched jobs in order)
             if (change) {
                 j = nj;
                 t = jobs[j].tree;
@@ -546,7 +547,7 @@ walk_kernel(const WalkJob* __restrict__ jobs, WalkArgs a, MlpNet net)
                         result = gload(t.ids + arg);
                     } else {
                         if (kHarvest) {
-                            const uint64_t slot = a.h_off[w] + my_rec++;
+                            const uint64_t slot = my_rec++;
                             if (slot < a.h_cap) {
                                 a.h_pts[2 * slot] = px;
                                 a.h_pts[2 * slot + 1] = py;

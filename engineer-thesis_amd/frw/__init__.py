@@ -96,7 +96,7 @@ EXPORTS = [
     "frw_mlp_levels", "frw_mlp_certify", "frw_mlp_add_samples", "frw_batch_set_proposal",
     "frw_batch_walk_learned", "frw_batch_autotune_learned", "frw_walk_weighted_async", "frw_walk_weighted", "frw_walk_adaptive", "frw_weight_tables", "frw_probe_read_bandwidth",
     "frw_query_refcompat", "frw_refcompat_gauss_surface", "frw_refcompat_info", "frw_refcompat_walk_from",
-    "frw_layout_set_bounds", "frw_walk_noalloc",
+    "frw_layout_set_bounds", "frw_walk_noalloc", "frw_batch_harvest",
 ]
 
 _lib = None
@@ -176,6 +176,7 @@ def lib():
     L.frw_batch_walk_learned.argtypes = [vp, C.POINTER(BatchParams), vp, C.c_float, vp, vp, vp, vp]
     L.frw_batch_autotune_learned.argtypes = [vp, C.POINTER(BatchParams), vp, C.c_float,
                                              C.POINTER(BatchParams), C.POINTER(C.c_double)]
+    L.frw_batch_harvest.argtypes = [vp, C.POINTER(BatchParams), vp, vp, vp, i64, vp]
     L.frw_mlp_free.argtypes = [vp]
     L.frw_mlp_free.restype = None
     L.frw_mlp_get_params.argtypes = [vp, vp, i32]
@@ -529,6 +530,16 @@ class Batch:
         _check(lib().frw_batch_autotune_learned(self.h, C.byref(base), net.h, alpha, C.byref(best),
                                                 C.byref(rate)))
         return {"block": best.block, "grid": best.grid}, rate.value
+
+    def harvest(self, pts, h, dstar, seed=1, block=0, grid=0, max_hops=0):
+        """walk_harvest for every job in one launch per pass: the records of
+        job 0's walks, then job 1's, ... (walk and hop order), up to
+        pts.shape[0] in all; returns each job's record count (numpy int64)."""
+        p = BatchParams(mode=MODE_EXACT, block=block, grid=grid, use_graph=0, seed=seed, max_hops=max_hops)
+        n = np.zeros(self.n_jobs, dtype=np.int64)
+        _check(lib().frw_batch_harvest(self.h, C.byref(p), pts.data_ptr(), h.data_ptr(), dstar.data_ptr(),
+                                       pts.shape[0], n.ctypes.data))
+        return n
 
     def check(self):
         _check(lib().frw_batch_check(self.h))

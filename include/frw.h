@@ -361,6 +361,15 @@ int frw_batch_walk_learned(frw_batch* b, const frw_batch_params* p, const frw_ml
                            int64_t* counts, int64_t* hops, uint64_t* stats_dev, void* stream);
 int frw_batch_autotune_learned(frw_batch* b, const frw_batch_params* base, const frw_mlp* net, float alpha,
                                frw_batch_params* best, double* best_hops_per_s);
+/* frw_walk_harvest for every job of the batch in one launch per pass (the
+ * jobs' exact walks, p->mode ignored): the records of job 0's walks first, in
+ * walk and hop order, then job 1's, and so on; up to `capacity` records in
+ * all (the first in that order).  n_per_job (host int64[n_jobs]) receives each
+ * job's number of records written.  Synchronous; the next walk launch of the
+ * batch uploads its job table again.  No reference counterpart (training
+ * data for the learned proposal). */
+int frw_batch_harvest(frw_batch* b, const frw_batch_params* p, double* pts, double* h, double* dstar,
+                      int64_t capacity, int64_t* n_per_job);
 
 /* Weighted (physical) estimator of the capacitance row of the master (no
  * reference counterpart: the reference computes Green's-function gradients

@@ -214,3 +214,42 @@ def test_batch_beyond_lds_job_table(mode):
         c1, h1 = _single(tree, q["master"], q["n_walks"], q["job"], q["first_walk"], mode)
         assert np.array_equal(counts[off:off + nr + 2], c1), j
         assert hops[j] == h1, j
+
+
+def test_batch_harvest_equals_single_harvests():
+    """One batched harvest (one launch per pass for every job) gives, job
+    after job, the records of each job's own walk_harvest; a capacity below
+    the total keeps a prefix of them; the batch's later walks are unchanged
+    (its job table is uploaded again)."""
+    torch = torch_dev()
+    trees, jobs = _fixture_batch()
+    batch = frw.Batch(trees, jobs)
+    ref_counts, ref_hops = _run(batch)
+    cap = 2_000_000
+
+    def bufs(n):
+        return (torch.zeros((n, 2), dtype=torch.float64, device="cuda"),
+                torch.zeros(n, dtype=torch.float64, device="cuda"),
+                torch.zeros(n, dtype=torch.float64, device="cuda"))
+
+    pts, hh, ds = bufs(cap)
+    n = batch.harvest(pts, hh, ds, seed=5)
+    total = int(n.sum())
+    assert 0 < total < cap and (n > 0).all()
+    p1, h1, d1 = bufs(cap)
+    off = 0
+    for j, q in enumerate(jobs):
+        got = frw.walk_harvest(trees[q["tree"]], q["master"], q["n_walks"], p1, h1, d1, seed=5,
+                               first_walk=q["first_walk"], job=q["job"])
+        assert got == n[j], j
+        assert torch.equal(pts[off:off + got], p1[:got]), j
+        assert torch.equal(hh[off:off + got], h1[:got]) and torch.equal(ds[off:off + got], d1[:got]), j
+        off += got
+    small = total // 3
+    p2, h2, d2 = bufs(small)
+    n2 = batch.harvest(p2, h2, d2, seed=5)
+    assert int(n2.sum()) == small
+    assert np.array_equal(np.cumsum(n2)[-1], small) and (n2 <= n).all()
+    assert torch.equal(p2, pts[:small]) and torch.equal(h2, hh[:small]) and torch.equal(d2, ds[:small])
+    counts, hops = _run(batch)
+    assert np.array_equal(counts, ref_counts) and np.array_equal(hops, ref_hops)
