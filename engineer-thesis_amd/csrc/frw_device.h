@@ -550,6 +550,95 @@ __host__ __device__ inline int q2_point(const DevTree& t, const Stack& st, doubl
     return best_id;
 }
 
+// The decided steps of the square search below in closed form.  While every
+// test is decided by h alone (h >= h_hit collides, h < h_free is free), the
+// search from h = 1 grows by dist0 until its first collision, at h_c = 1 +
+// n1 dist0, then runs a non-restoring binary search with steps dist0 / 2,
+// dist0 / 4, ... down to delta = dist0 / 2^m (the first at most MIN_DIST):
+// h stays on the lattice b + j delta (b = h_c - dist0), each step keeps
+// h_hit in (h - step, h + step], so after the m halvings h is the odd
+// lattice point b + (2q + 1) delta within delta of h_hit (the search's
+// directions are the bits of q, most significant first: 1 grows, 0
+// shrinks), and one more shrink by delta follows if h still collides: the
+// search ends at L, the largest lattice point below h_hit.  The loops track
+// h in FP64 (its running sum may round, by at most 30 ulp of h_c over 60
+// steps), so their decisions are these wherever every visited lattice point
+// lies farther than eps = 64 ulp(h_c) from the band [h_free, h_hit): the
+// two points next to h_hit (L and L + delta) are checked, every other one
+// is delta or more away.  Otherwise a test is ambiguous, or too close to
+// call, and the caller's loops run.  The decided loop stops at 60 steps:
+// longer searches are left to it too.
+struct SquarePath {
+    double L;      // final half-size
+    double d0;     // first step
+    double hc;     // largest half-size of the search
+    int n1;        // growth steps
+    int m;         // halvings
+    int shrink;    // 1: a last shrink by delta
+    int exact;     // 1: every h of the search is exact (delta's lowest set bit 2^eL <= 1, h_c < 2^(53 + eL))
+    int eL;
+    uint32_t q;    // directions of the halving steps
+};
+
+__host__ __device__ inline bool square_path(double d0, double min_dist, double h_hit, double h_free, SquarePath* sp)
+{
+    if (!(d0 > min_dist) || !(h_hit > 1.0) || !(h_free <= h_hit)) return false;
+    int e0 = 0, em = 0;
+    const double f0 = frexp(d0, &e0), fm = frexp(min_dist, &em);  // fractions in [0.5, 1)
+    const int m = e0 - em + (f0 > fm ? 1 : 0);                         // halvings: d0 / 2^m <= min_dist < d0 / 2^(m-1)
+    if (m < 1 || m > 31) return false;
+    const double delta = ldexp(d0, -m);
+    // n1: the first k >= 0 with 1 + k d0 >= h_hit
+    double k = ceil((h_hit - 1.0) / d0);
+    if (!(k >= 1.0 && k < 64.0)) return false;
+    if (k > 1.0 && fma(k - 1.0, d0, 1.0) >= h_hit) k -= 1.0;
+    if (fma(k, d0, 1.0) < h_hit) k += 1.0;
+    if (k + (double)m + 1.0 > 59.0) return false;
+    const double hc = fma(k, d0, 1.0), b = fma(k - 1.0, d0, 1.0);
+    int ehc = 0;
+    (void)frexp(hc, &ehc);
+    const double eps = ldexp(1.0, ehc - 53 + 6);  // 64 ulp(h_c)
+    if (!(hc >= h_hit + eps)) return false;
+    // L: the largest lattice point below h_hit, clear of the band by eps; the next one clear of h_hit
+    double j = ceil((h_hit - b) / delta) - 1.0;
+    if (!(j >= 0.0)) j = 0.0;
+    if (fma(j, delta, b) >= h_hit) j -= 1.0;
+    if (fma(j + 1.0, delta, b) < h_hit) j += 1.0;
+    if (!(j >= 0.0 && j < ldexp(1.0, m))) return false;
+    const double L = fma(j, delta, b);
+    if (!(L < h_free - eps && fma(j + 1.0, delta, b) >= h_hit + eps)) return false;
+    const uint64_t mant = __builtin_bit_cast(uint64_t, d0) & ((1ull << 52) - 1ull);
+    const int tz = mant ? __builtin_ctzll(mant) : 52;
+    const int eL = (e0 - 1) - 52 + tz - m;  // delta's lowest set bit
+    const uint32_t ji = (uint32_t)j;
+    const uint32_t odd = (ji & 1u) ? ji : ji + 1u;  // the halving steps' last point, b + odd delta
+    sp->L = L;
+    sp->d0 = d0;
+    sp->hc = hc;
+    sp->n1 = (int)k;
+    sp->m = m;
+    sp->shrink = (ji & 1u) ? 0 : 1;
+    sp->exact = (eL <= 0 && hc < ldexp(1.0, 53 + eL)) ? 1 : 0;
+    sp->eL = eL;
+    sp->q = (odd - 1u) >> 1;
+    return true;
+}
+
+// True when every box coordinate of the search, p -/+ h for h up to the
+// search's largest half-size, is exactly representable: with every h exact
+// (a multiple of 2^eL) and p a multiple of its ulp, each value is a
+// multiple of 2^g (g the smaller exponent) and representable below
+// 2^(53 + g).  The loops' running sums are then p -/+ h exactly.
+__host__ __device__ inline bool square_path_exact(const SquarePath& sp, double c)
+{
+    if (!sp.exact) return false;
+    int ec = 0;
+    const double a = fabs(c);
+    (void)frexp(a, &ec);  // ulp(a) = 2^(ec - 53)
+    const int g = a > 0.0 ? (ec - 53 < sp.eL ? ec - 53 : sp.eL) : sp.eL;
+    return a + sp.hc < ldexp(1.0, 53 + g);
+}
+
 // ---------------------------------------------------------------- Q3
 // drawBiggestSquareAtPoint (randomWalk/CPU/quadTree.cpp:264-312): returns the
 // p+-1 or p+-2 square when it collides, else runs the reference's grow/shrink
@@ -607,6 +696,38 @@ __host__ __device__ inline DBox biggest_square(const DevTree& t, const Stack& st
     h = 2.0;
     if (collides(init.x0, init.y0, init.x1, init.y1)) return init;
     h = 1.0;
+    if (!kLiteral) {
+        SquarePath sp;
+        if (square_path(t.dist0, MIN_DIST, h_hit, h_free, &sp)) {
+#if !defined(__HIP_DEVICE_COMPILE__) && defined(FRW_HOST_CLOSED_FORM_HOOK)
+            FRW_HOST_CLOSED_FORM_HOOK();
+#endif
+            if (square_path_exact(sp, px) && square_path_exact(sp, py))
+                return DBox{px - sp.L, py - sp.L, px + sp.L, py + sp.L};
+            // the search's steps on the box, in order, as the loops add them
+            // (some coordinates round): n1 growths by dist0, the m halving
+            // steps in the directions of q's bits, the last shrink
+            const int n = sp.n1 + sp.m + sp.shrink;
+            double d = sp.d0;
+            for (int s = 0; s < n; ++s) {
+                double sd = -sp.d0;
+                if (s >= sp.n1) {
+                    const int i = s - sp.n1;  // halving step i + 1, or the last shrink
+                    if (i < sp.m) {
+                        d *= 0.5;
+                        sd = ((sp.q >> (sp.m - 1 - i)) & 1u) ? -d : d;
+                    } else {
+                        sd = d;
+                    }
+                }
+                o.x0 += sd;
+                o.y0 += sd;
+                o.x1 -= sd;
+                o.y1 -= sd;
+            }
+            return o;
+        }
+    }
     double dist = t.dist0;
     bool max_reached = false;
     int it = 0;

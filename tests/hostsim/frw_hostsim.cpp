@@ -23,6 +23,9 @@ static bool host_any_lane(bool v)
     return false;
 }
 #define FRW_HOST_ANY_LANE(v) host_any_lane(v)
+// square searches answered by the closed form of their decided steps
+static long long g_closed_form = 0;
+#define FRW_HOST_CLOSED_FORM_HOOK() (++g_closed_form)
 
 #include "../../engineer-thesis_amd/csrc/frw_device.h"
 
@@ -192,6 +195,7 @@ extern "C" {
 void hs_set_stack(int runs) { g_runs = runs; }
 
 void hs_set_lane_pad(int pad) { g_pad = g_pad_left = pad; }
+long long hs_closed_form_count(void) { return g_closed_form; }
 
 void* hs_build(const int32_t* xyxy, int n, const int32_t* bounds, int cap)
 {

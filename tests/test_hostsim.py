@@ -39,6 +39,7 @@ def hostsim():
                           C.c_uint32, vp, vp, vp, vp]
     L.hs_set_stack.argtypes = [C.c_int]
     L.hs_set_lane_pad.argtypes = [C.c_int]
+    L.hs_closed_form_count.restype = C.c_longlong
     L.hs_walk_hinted.restype = C.c_int64
     L.hs_walk_hinted.argtypes = [vp, C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64,
                                  C.c_uint32, vp, vp, vp, vp, C.c_int]
@@ -181,6 +182,35 @@ def test_hostsim_large_layout(oracle_lib):
     hit, hp, tot = ht.walk(0, 3000)
     _, tot_o, ohit, ohp = ix.walk(0, 3000, per_walk=True)
     assert np.array_equal(hit, ohit) and np.array_equal(hp, ohp)
+
+
+@pytest.mark.parametrize("box", [2_000_000, 1_999_999, 123_457])
+def test_hostsim_square_closed_form(box):
+    """The replayed square search takes its decisions from the closed form
+    of its decided steps (growth by dist0, then a binary search down to
+    dist0 / 2^m) on most queries, and equals the literal bisection on every
+    query, for boxes whose first step 0.05 W is a short binary fraction (h
+    exact: the box is p -/+ L at once) and ones whose is not (h rounds in the
+    loops: the decisions are the lattice's away from the band, and the
+    steps are replayed on the box)."""
+    lay = frw.Layout.generate(20_000, box, 5, max(10, box // 4000), seed=7, cluster_frac=0.3, sigma=box / 50,
+                              n_clusters=10)
+    r = lay.rects()
+    b = tuple(float(v) for v in lay.bounds())
+    ht = HostTree(lay, 8)
+    rng = np.random.default_rng(11)
+    pts = query_points(r, b, 20000, seed=4)
+    # and points with long fractions, as walks produce (panel points of squares)
+    f = rng.random((5000, 2))
+    extra = np.column_stack([b[0] + f[:, 0] * (b[2] - b[0]), b[1] + f[:, 1] * (b[3] - b[1])])
+    pts = np.vstack([pts, extra])
+    before = ht.L.hs_closed_form_count()
+    dist, q2, q3e, q3b, nfb = ht.queries(pts)
+    used = ht.L.hs_closed_form_count() - before
+    assert np.array_equal(q3e, q3b)
+    free = int(np.count_nonzero(dist > 2.0))
+    print(f"\nbox {box}: closed form on {used} of {free} searches past the first two tests")
+    assert used > 0.9 * free
 
 
 def test_hostsim_tree_invariants():
