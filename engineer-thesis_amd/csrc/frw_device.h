@@ -624,19 +624,68 @@ __host__ __device__ inline bool square_path(double d0, double min_dist, double h
     return true;
 }
 
-// True when every box coordinate of the search, p -/+ h for h up to the
-// search's largest half-size, is exactly representable: with every h exact
-// (a multiple of 2^eL) and p a multiple of its ulp, each value is a
-// multiple of 2^g (g the smaller exponent) and representable below
-// 2^(53 + g).  The loops' running sums are then p -/+ h exactly.
-__host__ __device__ inline bool square_path_exact(const SquarePath& sp, double c)
+// The box coordinate c + s h of the search's end (s = -1: x0, y0; +1: x1,
+// y1) as the loops leave it, when every h of the search is exact.  Where
+// every |c + s h| (at most max(|c|, h_c) where it shrinks with h, |c| + h_c
+// where it grows) stays below 2^(53 + g), g the smaller of the exponents
+// of c's ulp and of h's lowest bit, every running sum is exact: c + s L.  Where it grows past the top of c's binade B, the values from
+// there on have an ulp of 2u (u = c's): with h a multiple of 2u and c an
+// even multiple of u they are still exact; with c an odd one, the first
+// value at or above B (a growth step, as growth comes first and ends at
+// h_c) is an odd multiple of u, a tie, rounded to the multiple of 4u (the
+// even significand), and every later value is exact again: the end is
+// sign(c) (|c| + e + L), e = -u or +u by that value's residue mod 4u.
+// False when none of these holds (more than one binade, h not a multiple
+// of 2u).
+__host__ __device__ inline bool square_coord(const SquarePath& sp, double c, double s, double* out)
 {
     if (!sp.exact) return false;
-    int ec = 0;
     const double a = fabs(c);
-    (void)frexp(a, &ec);  // ulp(a) = 2^(ec - 53)
-    const int g = a > 0.0 ? (ec - 53 < sp.eL ? ec - 53 : sp.eL) : sp.eL;
-    return a + sp.hc < ldexp(1.0, 53 + g);
+    int ea = 0;
+    (void)frexp(a, &ea);  // a in [2^(ea-1), 2^ea): ulp u = 2^(ea - 53)
+    const int eu = ea - 53;
+    if (a == 0.0) {  // the values are -/+ h, exact
+        *out = s * sp.L;
+        return true;
+    }
+    const int g = eu < sp.eL ? eu : sp.eL;
+    if ((c > 0.0) != (s > 0.0)) {  // |c + s h| = |a - h| <= max(a, h): multiples of 2^g
+        if (!(fmax(a, sp.hc) < ldexp(1.0, 53 + g))) return false;
+        *out = c + s * sp.L;
+        return true;
+    }
+    if (eu > sp.eL) {  // h has bits below c's ulp: a + h, multiples of 2^eL
+        if (!(a + sp.hc < ldexp(1.0, 53 + sp.eL))) return false;
+        *out = c + s * sp.L;
+        return true;
+    }
+    const double B = ldexp(1.0, ea);
+    if (a + sp.hc < B) {
+        *out = c + s * sp.L;
+        return true;
+    }
+    if (!(a + sp.hc < 2.0 * B) || sp.eL < eu + 1) return false;
+    const uint64_t ma = __builtin_bit_cast(uint64_t, a);
+    if ((ma & 1ull) == 0ull) {
+        *out = c + s * sp.L;
+        return true;
+    }
+    // the first growth step k >= 0 with a + 1 + k d0 >= B (below B every sum is exact)
+    double k = ceil((B - a - 1.0) / sp.d0);
+    if (!(k >= 0.0)) k = 0.0;
+    if (k > 0.0 && a + fma(k - 1.0, sp.d0, 1.0) >= B) k -= 1.0;
+    if (a + fma(k, sp.d0, 1.0) < B) k += 1.0;
+    if (!(k <= (double)sp.n1) || (k > 0.0 && !(a + fma(k - 1.0, sp.d0, 1.0) < B))) return false;
+    // (a + h_k) / u mod 4, from a's significand and h_k / u (an even integer)
+    const double hu = ldexp(fma(k, sp.d0, 1.0), -eu) * 0.25;
+    const uint32_t r = ((uint32_t)(ma & 3ull) + (uint32_t)((hu - floor(hu)) * 4.0)) & 3u;
+    const double u = ldexp(1.0, eu);
+    const double ae = a + (r == 1u ? -u : u);
+#if !defined(__HIP_DEVICE_COMPILE__) && defined(FRW_HOST_CLOSED_FORM_TIE_HOOK)
+    FRW_HOST_CLOSED_FORM_TIE_HOOK();
+#endif
+    *out = c > 0.0 ? ae + sp.L : -(ae + sp.L);
+    return true;
 }
 
 // ---------------------------------------------------------------- Q3
@@ -702,8 +751,29 @@ __host__ __device__ inline DBox biggest_square(const DevTree& t, const Stack& st
 #if !defined(__HIP_DEVICE_COMPILE__) && defined(FRW_HOST_CLOSED_FORM_HOOK)
             FRW_HOST_CLOSED_FORM_HOOK();
 #endif
-            if (square_path_exact(sp, px) && square_path_exact(sp, py))
+            // every box coordinate exact (the common case, one test per axis), or
+            // each one in closed form, or the steps replayed
+            auto quick = [&](double c) {
+                int ec = 0;
+                const double ac = fabs(c);
+                (void)frexp(ac, &ec);
+                const int g = ac > 0.0 && ec - 53 < sp.eL ? ec - 53 : sp.eL;
+                return ac + sp.hc < ldexp(1.0, 53 + g);
+            };
+            if (sp.exact && quick(px) && quick(py)) {
+#if !defined(__HIP_DEVICE_COMPILE__) && defined(FRW_HOST_CLOSED_FORM_DIRECT_HOOK)
+                FRW_HOST_CLOSED_FORM_DIRECT_HOOK();
+#endif
                 return DBox{px - sp.L, py - sp.L, px + sp.L, py + sp.L};
+            }
+            DBox r;
+            if (square_coord(sp, px, -1.0, &r.x0) && square_coord(sp, py, -1.0, &r.y0) &&
+                square_coord(sp, px, 1.0, &r.x1) && square_coord(sp, py, 1.0, &r.y1)) {
+#if !defined(__HIP_DEVICE_COMPILE__) && defined(FRW_HOST_CLOSED_FORM_DIRECT_HOOK)
+                FRW_HOST_CLOSED_FORM_DIRECT_HOOK();
+#endif
+                return r;
+            }
             // the search's steps on the box, in order, as the loops add them
             // (some coordinates round): n1 growths by dist0, the m halving
             // steps in the directions of q's bits, the last shrink
