@@ -568,6 +568,19 @@ __host__ __device__ inline int q2_point(const DevTree& t, const Stack& st, doubl
 // is delta or more away.  Otherwise a test is ambiguous, or too close to
 // call, and the caller's loops run.  The decided loop stops at 60 steps:
 // longer searches are left to it too.
+// 1 / d to within a few ulp (the quotients below only seed searches that
+// check their answer): the hardware reciprocal estimate refined by one
+// Newton step on the device, a division on the host.
+__host__ __device__ inline double approx_rcp(double d)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double r = __builtin_amdgcn_rcp(d);
+    return fma(fma(-d, r, 1.0), r, r);
+#else
+    return 1.0 / d;
+#endif
+}
+
 struct SquarePath {
     double L;      // final half-size
     double d0;     // first step
@@ -589,7 +602,8 @@ __host__ __device__ inline bool square_path(double d0, double min_dist, double h
     if (m < 1 || m > 31) return false;
     const double delta = ldexp(d0, -m);
     // n1: the first k >= 0 with 1 + k d0 >= h_hit
-    double k = ceil((h_hit - 1.0) / d0);
+    const double inv = approx_rcp(d0);
+    double k = ceil((h_hit - 1.0) * inv);
     if (!(k >= 1.0 && k < 64.0)) return false;
     if (k > 1.0 && fma(k - 1.0, d0, 1.0) >= h_hit) k -= 1.0;
     if (fma(k, d0, 1.0) < h_hit) k += 1.0;
@@ -600,7 +614,7 @@ __host__ __device__ inline bool square_path(double d0, double min_dist, double h
     const double eps = ldexp(1.0, ehc - 53 + 6);  // 64 ulp(h_c)
     if (!(hc >= h_hit + eps)) return false;
     // L: the largest lattice point below h_hit, clear of the band by eps; the next one clear of h_hit
-    double j = ceil((h_hit - b) / delta) - 1.0;
+    double j = ceil((h_hit - b) * ldexp(inv, m)) - 1.0;
     if (!(j >= 0.0)) j = 0.0;
     if (fma(j, delta, b) >= h_hit) j -= 1.0;
     if (fma(j + 1.0, delta, b) < h_hit) j += 1.0;
@@ -671,7 +685,7 @@ __host__ __device__ inline bool square_coord(const SquarePath& sp, double c, dou
         return true;
     }
     // the first growth step k >= 0 with a + 1 + k d0 >= B (below B every sum is exact)
-    double k = ceil((B - a - 1.0) / sp.d0);
+    double k = ceil((B - a - 1.0) * approx_rcp(sp.d0));
     if (!(k >= 0.0)) k = 0.0;
     if (k > 0.0 && a + fma(k - 1.0, sp.d0, 1.0) >= B) k -= 1.0;
     if (a + fma(k, sp.d0, 1.0) < B) k += 1.0;
@@ -777,23 +791,29 @@ __host__ __device__ inline DBox biggest_square(const DevTree& t, const Stack& st
             // the search's steps on the box, in order, as the loops add them
             // (some coordinates round): n1 growths by dist0, the m halving
             // steps in the directions of q's bits, the last shrink
-            const int n = sp.n1 + sp.m + sp.shrink;
+            const double g0 = -sp.d0;
+            for (int i = 0; i < sp.n1; ++i) {
+                o.x0 += g0;
+                o.y0 += g0;
+                o.x1 -= g0;
+                o.y1 -= g0;
+            }
             double d = sp.d0;
-            for (int s = 0; s < n; ++s) {
-                double sd = -sp.d0;
-                if (s >= sp.n1) {
-                    const int i = s - sp.n1;  // halving step i + 1, or the last shrink
-                    if (i < sp.m) {
-                        d *= 0.5;
-                        sd = ((sp.q >> (sp.m - 1 - i)) & 1u) ? -d : d;
-                    } else {
-                        sd = d;
-                    }
-                }
+            uint32_t dirs = sp.q << (32 - sp.m);  // the directions from the top bit down (m <= 31)
+            for (int i = 0; i < sp.m; ++i) {
+                d *= 0.5;
+                const double sd = (int32_t)dirs < 0 ? -d : d;
+                dirs <<= 1;
                 o.x0 += sd;
                 o.y0 += sd;
                 o.x1 -= sd;
                 o.y1 -= sd;
+            }
+            if (sp.shrink) {
+                o.x0 += d;
+                o.y0 += d;
+                o.x1 -= d;
+                o.y1 -= d;
             }
             return o;
         }
