@@ -638,83 +638,27 @@ __host__ __device__ inline bool square_path(double d0, double min_dist, double h
     return true;
 }
 
-// The box coordinate c + s h of the search's end (s = -1: x0, y0; +1: x1,
-// y1) as the loops leave it, when every h of the search is exact.  Where
-// every |c + s h| (at most max(|c|, h_c) where it shrinks with h, |c| + h_c
-// where it grows) stays below 2^(53 + g), g the smaller of the exponents
-// of c's ulp and of h's lowest bit, every running sum is exact: c + s L.  Where it grows past the top of c's binade B, the values from
-// there on have an ulp of 2u (u = c's): with h a multiple of 2u and c an
-// even multiple of u they are still exact; with c an odd one, the first
-// value at or above B (a growth step, as growth comes first and ends at
-// h_c) is an odd multiple of u, a tie, rounded to the multiple of 4u (the
-// even significand), and every later value is exact again: the end is
-// sign(c) (|c| + e + L), e = -u or +u by that value's residue mod 4u.
-// False when none of these holds (more than one binade, h not a multiple
-// of 2u).
-__host__ __device__ inline bool square_coord(const SquarePath& sp, double c, double s, double* out)
+// True when the halving steps leave a box coordinate exact: after the
+// growth steps it is z (the loops' own value), and every later value is z
+// plus a multiple of 2^eL (h exact on the lattice), between z and zb, its
+// value at b = h_c - dist0, so a multiple of g0 = min(2^lz, 2^eL) (2^lz:
+// z's lowest set bit) no larger in magnitude than max(|z|, |zb|): exact
+// when that stays below 2^(53 + g0's exponent) (zb computed in FP64:
+// rounding is monotone and the bound a power of two, so the test on the
+// rounded value is safe).
+__host__ __device__ inline bool halving_exact(double z, double zb, int eL)
 {
-    if (!sp.exact) return false;
-    const double a = fabs(c);
-    int ea = 0;
-    (void)frexp(a, &ea);  // a in [2^(ea-1), 2^ea): ulp u = 2^(ea - 53)
-    const int eu = ea - 53;
-    if (a == 0.0) {  // the values are -/+ h, exact
-        *out = s * sp.L;
-        return true;
+    int g = eL;
+    if (z != 0.0) {
+        int ez = 0;
+        (void)frexp(z, &ez);  // |z| in [2^(ez-1), 2^ez): ulp 2^(ez - 53)
+        const uint64_t sig = (__builtin_bit_cast(uint64_t, z) & ((1ull << 52) - 1ull)) | (1ull << 52);
+        const int lz = ez - 53 + __builtin_ctzll(sig);
+        g = lz < eL ? lz : eL;
     }
-    const int g = eu < sp.eL ? eu : sp.eL;
-    if ((c > 0.0) != (s > 0.0)) {  // |c + s h| = |a - h| <= max(a, h): multiples of 2^g
-        if (!(fmax(a, sp.hc) < ldexp(1.0, 53 + g))) return false;
-        *out = c + s * sp.L;
-        return true;
-    }
-    if (eu > sp.eL) {  // h has bits below c's ulp: a + h, multiples of 2^eL
-        if (!(a + sp.hc < ldexp(1.0, 53 + sp.eL))) return false;
-        *out = c + s * sp.L;
-        return true;
-    }
-    const double B = ldexp(1.0, ea);
-    if (a + sp.hc < B) {
-        *out = c + s * sp.L;
-        return true;
-    }
-    if (!(a + sp.hc < 2.0 * B) || sp.eL < eu + 1) return false;
-    const uint64_t ma = __builtin_bit_cast(uint64_t, a);
-    if ((ma & 1ull) == 0ull) {
-        *out = c + s * sp.L;
-        return true;
-    }
-    // the first growth step k >= 0 with a + 1 + k d0 >= B (below B every sum is exact)
-    double k = ceil((B - a - 1.0) * approx_rcp(sp.d0));
-    if (!(k >= 0.0)) k = 0.0;
-    if (k > 0.0 && a + fma(k - 1.0, sp.d0, 1.0) >= B) k -= 1.0;
-    if (a + fma(k, sp.d0, 1.0) < B) k += 1.0;
-    if (!(k <= (double)sp.n1) || (k > 0.0 && !(a + fma(k - 1.0, sp.d0, 1.0) < B))) return false;
-    // (a + h_k) / u mod 4, from a's significand and h_k / u (an even integer)
-    const double hu = ldexp(fma(k, sp.d0, 1.0), -eu) * 0.25;
-    const uint32_t r = ((uint32_t)(ma & 3ull) + (uint32_t)((hu - floor(hu)) * 4.0)) & 3u;
-    const double u = ldexp(1.0, eu);
-    const double ae = a + (r == 1u ? -u : u);
-#if !defined(__HIP_DEVICE_COMPILE__) && defined(FRW_HOST_CLOSED_FORM_TIE_HOOK)
-    FRW_HOST_CLOSED_FORM_TIE_HOOK();
-#endif
-    *out = c > 0.0 ? ae + sp.L : -(ae + sp.L);
-    return true;
+    return fmax(fabs(z), fabs(zb)) < ldexp(1.0, 53 + g);
 }
 
-// ---------------------------------------------------------------- Q3
-// drawBiggestSquareAtPoint (randomWalk/CPU/quadTree.cpp:264-312): returns the
-// p+-1 or p+-2 square when it collides, else runs the reference's grow/shrink
-// bisection.  The bisection's only input is the collision test of each
-// candidate box, so the schedule is replayed here on the four FP64
-// coordinates exactly as the reference updates them, and each test is
-// decided by the nearest-rect distance d*: every box side at least d*+margin
-// from p collides, every side below d*-margin does not, and only boxes inside
-// that band (rounding-ambiguous) fall back to an exact Q1.  kLiteral runs a
-// Q1 for every test instead (the reference algorithm as is).  `near` (the
-// rect at d*, optional) answers an ambiguous test without a traversal when
-// the box meets it, as Q1 would: walkers on integer coordinates, where d*
-// and the first test sizes 1 and 2 coincide, meet it on about 2 % of hops.
 template <bool kLiteral, class Stack>
 __host__ __device__ inline DBox biggest_square(const DevTree& t, const Stack& st, double px, double py,
                                       double dstar, uint32_t* n_q1, Counters* cnt, int* err,
@@ -765,39 +709,25 @@ __host__ __device__ inline DBox biggest_square(const DevTree& t, const Stack& st
 #if !defined(__HIP_DEVICE_COMPILE__) && defined(FRW_HOST_CLOSED_FORM_HOOK)
             FRW_HOST_CLOSED_FORM_HOOK();
 #endif
-            // every box coordinate exact (the common case, one test per axis), or
-            // each one in closed form, or the steps replayed
-            auto quick = [&](double c) {
-                int ec = 0;
-                const double ac = fabs(c);
-                (void)frexp(ac, &ec);
-                const int g = ac > 0.0 && ec - 53 < sp.eL ? ec - 53 : sp.eL;
-                return ac + sp.hc < ldexp(1.0, 53 + g);
-            };
-            if (sp.exact && quick(px) && quick(py)) {
-#if !defined(__HIP_DEVICE_COMPILE__) && defined(FRW_HOST_CLOSED_FORM_DIRECT_HOOK)
-                FRW_HOST_CLOSED_FORM_DIRECT_HOOK();
-#endif
-                return DBox{px - sp.L, py - sp.L, px + sp.L, py + sp.L};
-            }
-            DBox r;
-            if (square_coord(sp, px, -1.0, &r.x0) && square_coord(sp, py, -1.0, &r.y0) &&
-                square_coord(sp, px, 1.0, &r.x1) && square_coord(sp, py, 1.0, &r.y1)) {
-#if !defined(__HIP_DEVICE_COMPILE__) && defined(FRW_HOST_CLOSED_FORM_DIRECT_HOOK)
-                FRW_HOST_CLOSED_FORM_DIRECT_HOOK();
-#endif
-                return r;
-            }
-            // the search's steps on the box, in order, as the loops add them
-            // (some coordinates round): n1 growths by dist0, the m halving
-            // steps in the directions of q's bits, the last shrink
-            const double g0 = -sp.d0;
+            // the growth steps as the loops add them (where a sum rounds, it
+            // rounds here alike), then the halving steps: exact where every
+            // value they visit is exact (halving_exact), the end z -/+ (h_c - L)
+            DBox z = o;
             for (int i = 0; i < sp.n1; ++i) {
-                o.x0 += g0;
-                o.y0 += g0;
-                o.x1 -= g0;
-                o.y1 -= g0;
+                z.x0 -= sp.d0;
+                z.y0 -= sp.d0;
+                z.x1 += sp.d0;
+                z.y1 += sp.d0;
             }
+            if (sp.exact && halving_exact(z.x0, z.x0 + sp.d0, sp.eL) && halving_exact(z.y0, z.y0 + sp.d0, sp.eL) &&
+                halving_exact(z.x1, z.x1 - sp.d0, sp.eL) && halving_exact(z.y1, z.y1 - sp.d0, sp.eL)) {
+#if !defined(__HIP_DEVICE_COMPILE__) && defined(FRW_HOST_CLOSED_FORM_DIRECT_HOOK)
+                FRW_HOST_CLOSED_FORM_DIRECT_HOOK();
+#endif
+                const double D = sp.hc - sp.L;
+                return DBox{z.x0 + D, z.y0 + D, z.x1 - D, z.y1 - D};
+            }
+            o = z;
             double d = sp.d0;
             uint32_t dirs = sp.q << (32 - sp.m);  // the directions from the top bit down (m <= 31)
             for (int i = 0; i < sp.m; ++i) {

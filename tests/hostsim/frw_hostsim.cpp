@@ -24,10 +24,9 @@ static bool host_any_lane(bool v)
 }
 #define FRW_HOST_ANY_LANE(v) host_any_lane(v)
 // square searches answered by the closed form of their decided steps
-static long long g_closed_form = 0, g_closed_direct = 0, g_closed_tie = 0;
+static long long g_closed_form = 0, g_closed_direct = 0;
 #define FRW_HOST_CLOSED_FORM_HOOK() (++g_closed_form)
 #define FRW_HOST_CLOSED_FORM_DIRECT_HOOK() (++g_closed_direct)
-#define FRW_HOST_CLOSED_FORM_TIE_HOOK() (++g_closed_tie)
 
 #include "../../engineer-thesis_amd/csrc/frw_device.h"
 
@@ -199,7 +198,6 @@ void hs_set_stack(int runs) { g_runs = runs; }
 void hs_set_lane_pad(int pad) { g_pad = g_pad_left = pad; }
 long long hs_closed_form_count(void) { return g_closed_form; }
 long long hs_closed_form_direct_count(void) { return g_closed_direct; }
-long long hs_closed_form_tie_count(void) { return g_closed_tie; }
 
 void* hs_build(const int32_t* xyxy, int n, const int32_t* bounds, int cap)
 {

@@ -41,7 +41,6 @@ def hostsim():
     L.hs_set_lane_pad.argtypes = [C.c_int]
     L.hs_closed_form_count.restype = C.c_longlong
     L.hs_closed_form_direct_count.restype = C.c_longlong
-    L.hs_closed_form_tie_count.restype = C.c_longlong
     L.hs_walk_hinted.restype = C.c_int64
     L.hs_walk_hinted.argtypes = [vp, C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64,
                                  C.c_uint32, vp, vp, vp, vp, C.c_int]
@@ -206,25 +205,23 @@ def test_hostsim_square_closed_form(box):
     f = rng.random((5000, 2))
     extra = np.column_stack([b[0] + f[:, 0] * (b[2] - b[0]), b[1] + f[:, 1] * (b[3] - b[1])])
     # and points just below the top of a binade, where p + h crosses it and
-    # one running sum rounds (a tie when p is an odd multiple of its ulp)
+    # a running sum rounds (in a growth step, replayed as the loops add it)
     tops = [float(2 ** e) for e in range(10, 24) if b[0] < 2 ** e < b[2]]
     near = np.array([[t - rng.random() * 0.08 * (b[2] - b[0]), b[1] + rng.random() * (b[3] - b[1])]
                      for t in tops for _ in range(400)])
     pts = np.vstack([pts, extra, near, near[:, ::-1]])
     pts = pts[(pts[:, 1] > b[1]) & (pts[:, 1] < b[3]) & (pts[:, 0] > b[0]) & (pts[:, 0] < b[2])]
-    c0, d0, t0 = (ht.L.hs_closed_form_count(), ht.L.hs_closed_form_direct_count(),
-                  ht.L.hs_closed_form_tie_count())
+    c0, d0 = ht.L.hs_closed_form_count(), ht.L.hs_closed_form_direct_count()
     dist, q2, q3e, q3b, nfb = ht.queries(pts)
     used = ht.L.hs_closed_form_count() - c0
     direct = ht.L.hs_closed_form_direct_count() - d0
-    ties = ht.L.hs_closed_form_tie_count() - t0
     assert np.array_equal(q3e, q3b)
     free = int(np.count_nonzero(dist > 2.0))
     print(f"\nbox {box}: closed form on {used} of {free} searches past the first two tests, {direct} without "
-          f"replaying, {ties} coordinates through a tie")
+          f"replaying the halving steps")
     assert used > 0.9 * free
     if box != 1_999_999:  # h exact
-        assert direct > 0.8 * used and ties > 100
+        assert direct > 0.95 * used
 
 
 def test_hostsim_tree_invariants():
