@@ -185,7 +185,7 @@ def test_hostsim_large_layout(oracle_lib):
     assert np.array_equal(hit, ohit) and np.array_equal(hp, ohp)
 
 
-@pytest.mark.parametrize("box", [2_000_000, 1_999_999, 123_457])
+@pytest.mark.parametrize("box", [2_000_000, 1_999_999, 123_457, 10_000_000])
 def test_hostsim_square_closed_form(box):
     """The replayed square search takes its decisions from the closed form
     of its decided steps (growth by dist0, then a binary search down to
@@ -220,7 +220,7 @@ def test_hostsim_square_closed_form(box):
     print(f"\nbox {box}: closed form on {used} of {free} searches past the first two tests, {direct} without "
           f"replaying the halving steps")
     assert used > 0.9 * free
-    if box != 1_999_999:  # h exact
+    if box != 1_999_999:  # h exact (10^7: the default layout's geometry, dist0 = 510000)
         assert direct > 0.95 * used
 
 
