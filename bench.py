@@ -98,7 +98,7 @@ def parse_args():
     ap.add_argument("--no-autotune", action="store_true", help="batched: default launch shape")
     ap.add_argument("--compare-walks", type=int, default=10_000_000)
     ap.add_argument("--walks", type=int, default=0, help="walks per step (0 = workload default)")
-    ap.add_argument("--leaf-cap", type=int, default=8)
+    ap.add_argument("--leaf-cap", type=int, default=4)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")

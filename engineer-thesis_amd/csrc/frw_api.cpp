@@ -206,7 +206,7 @@ int frw_tree_build(const frw_layout* l, const frw_tree_params* p, void* stream, 
 {
     if (!l || !out) return set_error(FRW_E_INVALID, "null argument");
     if (l->rects.empty()) return set_error(FRW_E_RANGE, "layout has no rects");
-    const int cap = (p && p->leaf_cap > 0) ? p->leaf_cap : 8;
+    const int cap = (p && p->leaf_cap > 0) ? p->leaf_cap : 4;
     if (cap > 64) return set_error(FRW_E_RANGE, "leaf_cap must be <= 64");
     const int kind = p ? p->kind : FRW_TREE_MORTON;
     if (kind != FRW_TREE_MORTON && kind != FRW_TREE_REFCOMPAT) return set_error(FRW_E_INVALID, "unknown tree kind");

@@ -101,7 +101,7 @@ int frw_green_thresholds(uint32_t out[FRW_NSAMPLE]);
 /* ---------------------------------------------------------------- tree */
 
 typedef struct {
-    int32_t leaf_cap;     /* max rects per leaf (0 = default 8, <= 64) */
+    int32_t leaf_cap;     /* max rects per leaf (0 = default 4, <= 64) */
     int32_t kind;         /* FRW_TREE_MORTON, or FRW_TREE_REFCOMPAT: also build the reference's own
                              CPU tree (randomWalk/CPU/quadTree.cpp:73-146) for FRW_MODE_REFCOMPAT */
     int32_t reserved[6];
